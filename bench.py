@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""k-way partitioning throughput on one MI355X.
+
+A step is one full multilevel k-way partition (coarsening, initial
+partition, uncoarsening with greedy refinement) of the synthetic graph, whose
+CSR is already resident in HBM when the timed region starts.  Default
+workload: random geometric graph, 2^20 points in the unit square, r =
+sqrt(12/(pi n)), Morton-ordered ids, vertex weight 1, edge weights U{1..10},
+k = 64, SHEM + greedy, ufactor 30, niter 10, ncuts 1, seed 1.
+
+Prints one JSON line: graphs/s (value), per-pass edge rates, the roofline of
+the gain-table kernel on the finest level (HIP events on the library's
+stream), and a CPU baseline (the oracle restatement of the reference on the
+same graph, one host core).
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rgg20", help="rgg20 | grid256 | rmat22 | rggNN | rmatNN")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-budget-s", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+
+    import numpy as np
+    import rmetis_amd as R
+    from rmetis_amd import _lib, synth
+
+    L = _lib.lib()
+    if R.device_count() < 1:
+        raise SystemExit("no HIP device")
+    t0 = time.time()
+    g, k = synth.config_graph(args.workload, seed=args.seed)
+    log(f"graph {g} k={k} generated in {time.time() - t0:.1f}s")
+    DA = _lib.DeviceArray
+    d_xadj, d_adj = DA(g.xadj), DA(g.adjncy)
+    d_vw = DA(g.vwgt) if g.vwgt is not None else None
+    d_aw = DA(g.adjwgt) if g.adjwgt is not None else None
+    d_part = DA(count=g.n, dtype=np.int32)
+    sync = L.rmetis_hip_device_synchronize
+    sync()
+    opts = R.Options(rtype=R.RType.Greedy, seed=args.seed, ncuts=1, niter=10, ufactor=30).to_c_array()
+    stats = _lib.HipStats()
+    obj = C.c_int32()
+
+    def step(with_stats=False):
+        rc = L.rmetis_hip_kway_device(g.n, g.ncon, d_xadj.ptr, d_adj.ptr,
+                                      d_vw.ptr if d_vw is not None else None,
+                                      d_aw.ptr if d_aw is not None else None, k, None, None,
+                                      opts.ctypes.data_as(_lib.i32p), C.byref(obj), d_part.ptr,
+                                      C.byref(stats) if with_stats else None)
+        if rc != 1:
+            raise RuntimeError(f"kway failed rc={rc}: {_lib.last_error()}")
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i} cut={obj.value}")
+    L.rmetis_hip_profile_enable(1)
+    sync()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step(with_stats=(i == args.steps - 1))
+    sync()
+    elapsed = time.perf_counter() - t_start
+    log(f"timed {args.steps} steps in {elapsed:.3f}s cut={obj.value}")
+
+    def prof(tag, size=-1):
+        ln, ms, by = C.c_int64(), C.c_double(), C.c_double()
+        L.rmetis_hip_profile_read(tag, size, C.byref(ln), C.byref(ms), C.byref(by))
+        return ln.value, ms.value, by.value
+
+    prof_tags = {"gain": 0, "compact": 1, "admit": 2, "match": 3, "contract": 4}
+    kernel_ms = {name: round(prof(t)[1] / args.steps, 3) for name, t in prof_tags.items()}
+    g_launch, g_ms, g_bytes = prof(0, g.n)
+    L.rmetis_hip_profile_enable(0)
+    avg_ms = g_ms / max(g_launch, 1)
+    bytes_per_launch = g_bytes / max(g_launch, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+
+    part = d_part.numpy()
+    pw = np.bincount(part, weights=(g.vwgt if g.vwgt is not None else None), minlength=k)
+    total = float(g.vwgt.sum()) if g.vwgt is not None else float(g.n)
+    maxw = math.floor(total / k * 1.03)
+    violations = int((pw > maxw).sum())
+
+    levels = []
+    for i in range(stats.nlevels):
+        n_i, m_i = stats.level_n[i], stats.level_m[i]
+        cm, rm, ps = stats.coarsen_ms[i], stats.refine_ms[i], stats.passes[i]
+        levels.append({"n": n_i, "m": m_i, "coarsen_ms": round(cm, 3), "refine_ms": round(rm, 3),
+                       "passes": ps,
+                       "coarsen_edges_per_s": round(m_i / (cm * 1e-3), 1) if cm > 0 else None,
+                       "refine_edges_per_s_per_pass": round(m_i * ps / (rm * 1e-3), 1) if rm > 0 else None})
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        O.build()
+        runs, t_cpu = 0, 0.0
+        while t_cpu < args.cpu_budget_s and runs < 10:
+            t1 = time.perf_counter()
+            rc, cobj, _ = O.part_graph_kway(g.xadj, g.adjncy, k, vwgt=g.vwgt, adjwgt=g.adjwgt, ncon=g.ncon,
+                                            seed=args.seed + runs, niter=10, ufactor=30)
+            t_cpu += time.perf_counter() - t1
+            runs += 1
+            log(f"cpu oracle run {runs}: cut={cobj} ({t_cpu:.1f}s total)")
+        cpu = {"value": round(runs / t_cpu, 4), "unit": "graphs/s", "cores": 1, "kind": "port",
+               "sample": f"{runs} full k-way partitions of the same graph (seeds {args.seed}..{args.seed + runs - 1}) "
+                         f"by the oracle's C++ restatement of the reference, one host core, {t_cpu:.1f}s",
+               "cut": int(cobj)}
+
+    value = args.steps / elapsed
+    out = {
+        "metric": "graphs/s (k-way partitions of one graph per second)",
+        "value": round(value, 4),
+        "unit": "graphs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {g.name} n={g.n} m={g.m} k={k}",
+                   "ctype": "shem", "rtype": "greedy", "ufactor": 30, "niter": 10, "ncuts": 1,
+                   "seed": args.seed, "parallelism": "single GPU"},
+        "edge_cut": int(obj.value),
+        "balance_violations": violations,
+        "edges_per_s": round(g.m * value, 1),
+        "kernel_ms_per_step": kernel_ms,
+        "levels": levels,
+        "roofline": {"bound": "hbm", "kernel": "k_gain (finest level)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_launch": int(bytes_per_launch),
+                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": g_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,177 @@
+/*
+ * rmetis.h -- C ABI of the MI355X (gfx950) k-way partitioner, librmetis_hip.so.
+ *
+ * The first two entry points are drop-ins for the reference crate's C ABI
+ * (javagg/rmetis, src/ffi/c_api.rs) with identical signatures, argument
+ * meaning, option-array parsing and return codes.  The rmetis_hip_* entry
+ * points expose the device-resident path and the individual hot-path
+ * functions (each cites the reference function it reproduces) so that callers
+ * and parity tests can drive them through plain pointers and sizes.
+ *
+ * Index type idx_t = int32_t (src/types.rs:4), real_t = float (src/types.rs:6).
+ * Return codes (src/types.rs:164-167): METIS_OK 1, METIS_ERROR_INPUT -2,
+ * METIS_ERROR_MEMORY -3, METIS_ERROR -4.
+ */
+#ifndef RMETIS_HIP_H
+#define RMETIS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t idx_t;
+typedef float real_t;
+
+#define RMETIS_NOPTIONS 40
+#define METIS_OK 1
+#define METIS_ERROR_INPUT (-2)
+#define METIS_ERROR_MEMORY (-3)
+#define METIS_ERROR (-4)
+
+/* Option-array indices (src/types.rs:173-186). */
+#define METIS_OPTION_PTYPE 0
+#define METIS_OPTION_OBJTYPE 1
+#define METIS_OPTION_CTYPE 2
+#define METIS_OPTION_IPTYPE 3
+#define METIS_OPTION_RTYPE 4
+#define METIS_OPTION_DBGLVL 5
+#define METIS_OPTION_NITER 6
+#define METIS_OPTION_NCUTS 7
+#define METIS_OPTION_SEED 8
+#define METIS_OPTION_NO2HOP 9
+#define METIS_OPTION_MINCONN 10
+#define METIS_OPTION_CONTIG 11
+#define METIS_OPTION_UFACTOR 16
+#define METIS_OPTION_NUMBERING 17
+
+/* Replaces RMETIS_SetDefaultOptions, src/ffi/c_api.rs:14-23.  Writes the
+ * defaults of src/types.rs:77-95 via to_raw (types.rs:227-245); index 0 stays
+ * -1, so an array filled here parses as "all defaults" until options[0] is
+ * set.  Returns METIS_OK, or METIS_ERROR_INPUT for NULL. */
+int32_t RMETIS_SetDefaultOptions(idx_t* options);
+
+/* Replaces RMETIS_PartGraphKway, src/ffi/c_api.rs:55-81 (+ c_partition_impl
+ * 140-209).  Host pointers.  Required: nvtxs, ncon, xadj, adjncy, nparts,
+ * part; optional (NULL = None): vwgt[n*ncon], vsize[n], adjwgt[xadj[n]],
+ * tpwgts[nparts*ncon], ubvec[ncon], options[40], objval.  Inputs are copied;
+ * part/objval are written only on success.  The multilevel path runs on the
+ * current HIP device; RTYPE must be Greedy (1): the FM refinement of the
+ * reference is not provided and returns METIS_ERROR. */
+int32_t RMETIS_PartGraphKway(const idx_t* nvtxs, const idx_t* ncon, const idx_t* xadj, const idx_t* adjncy,
+                             const idx_t* vwgt, const idx_t* vsize, const idx_t* adjwgt, const idx_t* nparts,
+                             const real_t* tpwgts, const real_t* ubvec, const idx_t* options, idx_t* objval,
+                             idx_t* part);
+
+/* Per-level figures of the last attempt of a k-way call (finest first). */
+#define RMETIS_HIP_MAX_LEVELS 64
+typedef struct {
+    int32_t nlevels;
+    int32_t coarsest_n;
+    float initial_ms;
+    int32_t level_n[RMETIS_HIP_MAX_LEVELS];
+    int32_t level_m[RMETIS_HIP_MAX_LEVELS];
+    float coarsen_ms[RMETIS_HIP_MAX_LEVELS];
+    float refine_ms[RMETIS_HIP_MAX_LEVELS];
+    int32_t passes[RMETIS_HIP_MAX_LEVELS];
+} rmetis_hip_stats;
+
+/* Same computation as RMETIS_PartGraphKway (src/partition/kway.rs:20-141) on
+ * a graph already resident in device memory: d_xadj, d_adjncy, d_vwgt,
+ * d_adjwgt and the output d_part are device pointers (vwgt/adjwgt may be
+ * NULL); tpwgts, ubvec, options, objval and stats are host pointers (stats
+ * may be NULL).  The CSR is not validated (the caller provides a graph that
+ * src/graph/validate.rs:16-142 accepts). */
+int32_t rmetis_hip_kway_device(idx_t nvtxs, idx_t ncon, const idx_t* d_xadj, const idx_t* d_adjncy,
+                               const idx_t* d_vwgt, const idx_t* d_adjwgt, idx_t nparts, const real_t* tpwgts,
+                               const real_t* ubvec, const idx_t* options, idx_t* objval, idx_t* d_part,
+                               rmetis_hip_stats* stats);
+
+/* Graph::edge_cut, src/graph/mod.rs:123-137 (host pointers; adjwgt may be NULL). */
+int32_t rmetis_hip_edge_cut(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* adjwgt,
+                            const idx_t* part, idx_t* cut);
+
+/* Graph::partition_weights, src/graph/mod.rs:140-149; pw[nparts*ncon]. */
+int32_t rmetis_hip_partition_weights(idx_t nvtxs, idx_t ncon, const idx_t* vwgt, const idx_t* part,
+                                     idx_t nparts, idx_t* pw);
+
+/* Boundary set (Graph::boundary_vertices, src/graph/mod.rs:164-171) and gain
+ * table (compute_move_gain, src/refine/greedy.rs:197-218) of a partition.
+ * rec[4*v] = (v if boundary else -1, part[v], ncand, nbefore | zflag<<31);
+ * g0[v] = gain of moving v to any part it has no edge weight to;
+ * cand_part / cand_gain [xadj[n]] hold, at offset xadj[v], the ncand parts
+ * with a nonzero edge-weight sum, ordered by gain descending then part. */
+int32_t rmetis_hip_gain_table(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* adjwgt,
+                              const idx_t* part, idx_t nparts, idx_t* rec, idx_t* g0, idx_t* cand_part,
+                              idx_t* cand_gain);
+
+/* One synchronous greedy pass of src/refine/greedy.rs:36-193 (nprocs = 1):
+ * mode 0 = balance repair (60-123), 1 = one cut-optimisation round (127-182).
+ * tpwgts[nparts*ncon] are the f64 target weights and ubvec[ncon] the f64
+ * imbalance factors the reference's greedy_refine receives.  pw (in/out) is
+ * the running part-weight vector, part (in/out) is updated with the moves;
+ * moves[2*nmoves] receives (v, p) in admission order. */
+int32_t rmetis_hip_greedy_pass(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                               const idx_t* vwgt, const idx_t* adjwgt, idx_t* part, idx_t nparts,
+                               const double* tpwgts, const double* ubvec, idx_t mode, idx_t* pw, idx_t* moves,
+                               idx_t* nmoves);
+
+/* greedy_refine, src/refine/greedy.rs:12-22 / 36-193: repair pass plus up to
+ * niter cut rounds on part (in/out); *cut receives the final edge cut. */
+int32_t rmetis_hip_greedy_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                                 const idx_t* vwgt, const idx_t* adjwgt, idx_t* part, idx_t nparts,
+                                 const double* tpwgts, const double* ubvec, idx_t niter, idx_t* cut);
+
+/* The GPU matching used in place of shem_matching / random_matching
+ * (src/coarsen/shem.rs:13-56, src/coarsen/rm.rs:13-43): seeded, deterministic,
+ * symmetric (matching[matching[v]] == v); ctype 1 = heavy edge, 0 = random. */
+int32_t rmetis_hip_match(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* adjwgt, idx_t ctype,
+                         uint64_t seed, idx_t* matching);
+
+/* Graph::build_coarse_graph, src/graph/mod.rs:284-384, for a symmetric
+ * matching.  Output capacities: cxadj n+1, cadjncy/cadjwgt xadj[n],
+ * cvwgt n*ncon, cvsize n, cmap n.  *cn / *cm receive the coarse sizes;
+ * *flags bit0 = cvwgt present, bit1 = cvsize present, bit2 = coarse adjwgt
+ * present (the reference's Some; cadjwgt is always filled). */
+int32_t rmetis_hip_build_coarse_graph(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                                      const idx_t* vwgt, const idx_t* adjwgt, const idx_t* vsize,
+                                      const idx_t* matching, idx_t* cxadj, idx_t* cadjncy, idx_t* cadjwgt,
+                                      idx_t* cvwgt, idx_t* cvsize, idx_t* cmap, idx_t* cn, idx_t* cm,
+                                      idx_t* flags);
+
+/* validate_csr, src/graph/validate.rs:16-142 (what Graph::new checks, 38-50),
+ * with the array lengths given explicitly (vwgt/adjwgt may be NULL = None).
+ * Host only; returns METIS_OK or METIS_ERROR_INPUT. */
+int32_t rmetis_hip_validate_csr(idx_t nvtxs, idx_t ncon, const idx_t* xadj, int64_t xadj_len,
+                                const idx_t* adjncy, int64_t adjncy_len, const idx_t* vwgt, int64_t vwgt_len,
+                                const idx_t* adjwgt, int64_t adjwgt_len);
+
+/* Device-memory plumbing for callers that hold no other HIP runtime handle
+ * (no counterpart in the reference): plain hipMalloc/hipFree/hipMemcpy on the
+ * current device; kind 1 = host->device, 2 = device->host, 3 = device->device. */
+void* rmetis_hip_device_alloc(int64_t bytes);
+void rmetis_hip_device_free(void* p);
+int32_t rmetis_hip_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind);
+int32_t rmetis_hip_device_synchronize(void);
+
+/* Kernel timing (no counterpart in the reference; used by the benchmark).
+ * While enabled, launches of each kernel family are bracketed by HIP events
+ * on their own stream.  Tags: 0 gain tables/candidates, 1 candidate
+ * compaction, 2 ordered admission, 3 matching, 4 contraction, 5 other.
+ * enable(on) clears the counters; read returns launches, total ms and the
+ * algorithmic HBM bytes of the timed launches (0 where not modelled), over
+ * all levels (size < 0) or the levels with exactly `size` vertices. */
+void rmetis_hip_profile_enable(int32_t on);
+int32_t rmetis_hip_profile_read(int32_t tag, int64_t size, int64_t* launches, double* ms, double* bytes);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* rmetis_hip_last_error(void);
+
+/* Number of visible HIP devices (0 when none; never throws). */
+int32_t rmetis_hip_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,321 @@
+"""rmetis_amd -- MI355X (gfx950) multilevel k-way partitioning, a drop-in for
+javagg/rmetis's ``part_graph_kway`` / ``Graph`` / ``Options`` API (src/lib.rs:48-56,
+src/graph/mod.rs:13-50, src/types.rs:12-245) and ``RMETIS_PartGraphKway`` C ABI
+(src/ffi/c_api.rs:55-81).
+
+The host-side Python mirror calls the C ABI of ``librmetis_hip.so``; every
+computation runs in the HIP kernels on the current GPU.  Names, argument
+meanings and errors follow the reference: invalid input raises
+``InputError`` (METIS_ERROR_INPUT), device memory exhaustion ``MemoryError_``
+(METIS_ERROR_MEMORY), everything else ``InternalError`` (METIS_ERROR).
+"""
+import ctypes as C
+import enum
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+METIS_OK = 1
+METIS_ERROR_INPUT = -2
+METIS_ERROR_MEMORY = -3
+METIS_ERROR = -4
+METIS_NOPTIONS = 40
+
+# option-array indices, src/types.rs:173-186
+METIS_OPTION_PTYPE = 0
+METIS_OPTION_OBJTYPE = 1
+METIS_OPTION_CTYPE = 2
+METIS_OPTION_IPTYPE = 3
+METIS_OPTION_RTYPE = 4
+METIS_OPTION_DBGLVL = 5
+METIS_OPTION_NITER = 6
+METIS_OPTION_NCUTS = 7
+METIS_OPTION_SEED = 8
+METIS_OPTION_NO2HOP = 9
+METIS_OPTION_MINCONN = 10
+METIS_OPTION_CONTIG = 11
+METIS_OPTION_UFACTOR = 16
+METIS_OPTION_NUMBERING = 17
+
+
+class MetisError(Exception):
+    """src/types.rs:145-161."""
+    code = METIS_ERROR
+
+
+class InputError(MetisError):
+    code = METIS_ERROR_INPUT
+
+
+class MemoryError_(MetisError):
+    code = METIS_ERROR_MEMORY
+
+
+class InternalError(MetisError):
+    code = METIS_ERROR
+
+
+def _raise(code):
+    if code == METIS_OK:
+        return
+    msg = _lib.last_error()
+    if code == METIS_ERROR_INPUT:
+        raise InputError(msg or "input error")
+    if code == METIS_ERROR_MEMORY:
+        raise MemoryError_(msg or "out of memory")
+    raise InternalError(msg or "internal error")
+
+
+class CType(enum.IntEnum):
+    Rm = 0
+    Shem = 1
+
+
+class IpType(enum.IntEnum):
+    Grow = 0
+    Random = 1
+    MetisRb = 4
+
+
+class RType(enum.IntEnum):
+    Fm = 0
+    Greedy = 1
+
+
+class ObjType(enum.IntEnum):
+    Cut = 0
+    Vol = 1
+
+
+@dataclass
+class Options:
+    """src/types.rs:52-95 (defaults), presets 99-116, raw array 190-245."""
+    ctype: CType = CType.Shem
+    iptype: IpType = IpType.Grow
+    rtype: RType = RType.Fm
+    objtype: ObjType = ObjType.Cut
+    no2hop: bool = False
+    ncuts: int = 1
+    niter: int = 10
+    seed: int = 0
+    minconn: bool = False
+    contig: bool = False
+    ufactor: int = 30
+    numbering: int = 0
+    dbglvl: int = 0
+
+    @classmethod
+    def for_recursive(cls, **kw):
+        return cls(**{**dict(iptype=IpType.MetisRb, rtype=RType.Fm, ufactor=1), **kw})
+
+    @classmethod
+    def for_kway(cls, **kw):
+        return cls(**{**dict(iptype=IpType.Grow, rtype=RType.Fm, ufactor=30), **kw})
+
+    def to_raw(self):
+        """types.rs:227-245 (index 0 stays -1)."""
+        a = np.full(METIS_NOPTIONS, -1, np.int32)
+        a[METIS_OPTION_CTYPE] = int(self.ctype)
+        a[METIS_OPTION_IPTYPE] = int(self.iptype)
+        a[METIS_OPTION_RTYPE] = int(self.rtype)
+        a[METIS_OPTION_OBJTYPE] = int(self.objtype)
+        a[METIS_OPTION_NO2HOP] = int(self.no2hop)
+        a[METIS_OPTION_NCUTS] = self.ncuts
+        a[METIS_OPTION_NITER] = self.niter
+        a[METIS_OPTION_SEED] = np.int64(self.seed).astype(np.int32)
+        a[METIS_OPTION_MINCONN] = int(self.minconn)
+        a[METIS_OPTION_CONTIG] = int(self.contig)
+        a[METIS_OPTION_UFACTOR] = self.ufactor
+        a[METIS_OPTION_NUMBERING] = self.numbering
+        a[METIS_OPTION_DBGLVL] = self.dbglvl
+        return a
+
+    def to_c_array(self):
+        """Array handed to the C ABI: to_raw() with index 0 set so that the
+        C side parses it (c_api.rs:217 treats options[0] == -1 as defaults)."""
+        a = self.to_raw()
+        a[METIS_OPTION_PTYPE] = 0
+        return a
+
+
+@dataclass
+class PartitionResult:
+    """src/types.rs:125-130."""
+    part: np.ndarray
+    objval: int
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a, t=_lib.i32p):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Graph:
+    """CSR graph, src/graph/mod.rs:13-50.  Arrays are int32; None weights mean
+    unit weights.  Construction validates with graph/validate.rs:16-142 and
+    raises InputError like Graph::new."""
+
+    def __init__(self, nvtxs, ncon, xadj, adjncy, vwgt=None, adjwgt=None, vsize=None):
+        self.nvtxs = int(nvtxs)
+        self.ncon = int(ncon)
+        self.xadj = _i32(xadj)
+        self.adjncy = _i32(adjncy)
+        self.vwgt = _i32(vwgt)
+        self.adjwgt = _i32(adjwgt)
+        self.vsize = _i32(vsize)
+        rc = _lib.lib().rmetis_hip_validate_csr(
+            self.nvtxs, self.ncon, _p(self.xadj), len(self.xadj), _p(self.adjncy), len(self.adjncy),
+            _p(self.vwgt), 0 if self.vwgt is None else len(self.vwgt),
+            _p(self.adjwgt), 0 if self.adjwgt is None else len(self.adjwgt))
+        _raise(rc)
+        self.nedges = len(self.adjncy) // 2
+
+    @classmethod
+    def new_unweighted(cls, nvtxs, xadj, adjncy):
+        return cls(nvtxs, 1, xadj, adjncy)
+
+    def __repr__(self):
+        return f"Graph(nvtxs={self.nvtxs}, nedges={self.nedges}, ncon={self.ncon})"
+
+    def neighbors(self, v):
+        return self.adjncy[self.xadj[v]:self.xadj[v + 1]]
+
+    def degree(self, v):
+        return int(self.xadj[v + 1] - self.xadj[v])
+
+    def total_weight(self):
+        """graph/mod.rs:152-160 (i32 sums)."""
+        if self.vwgt is None:
+            return np.full(self.ncon, self.nvtxs, np.int32)
+        s = self.vwgt.reshape(self.nvtxs, self.ncon).astype(np.int64).sum(axis=0)
+        return s.astype(np.uint32).astype(np.int32)
+
+    def compute_target_weights(self, nparts, tpwgts=None):
+        """graph/mod.rs:392-410 -> float64 [nparts][ncon]."""
+        total = self.total_weight().astype(np.float64)
+        if tpwgts is None:
+            frac = np.full((nparts, self.ncon), 1.0 / float(nparts))
+        else:
+            frac = np.asarray(tpwgts, np.float32).astype(np.float64).reshape(nparts, self.ncon)
+        return frac * total[None, :]
+
+    # ---- reductions and gain tables on the GPU ----
+    def edge_cut(self, part):
+        """graph/mod.rs:123-137."""
+        part = _i32(part)
+        out = C.c_int32()
+        _raise(_lib.lib().rmetis_hip_edge_cut(self.nvtxs, _p(self.xadj), _p(self.adjncy), _p(self.adjwgt),
+                                              _p(part), C.byref(out)))
+        return out.value
+
+    def partition_weights(self, part, nparts):
+        """graph/mod.rs:140-149 -> int32 [nparts][ncon]."""
+        part = _i32(part)
+        pw = np.zeros(nparts * self.ncon, np.int32)
+        _raise(_lib.lib().rmetis_hip_partition_weights(self.nvtxs, self.ncon, _p(self.vwgt), _p(part), nparts,
+                                                       _p(pw)))
+        return pw.reshape(nparts, self.ncon)
+
+    def gain_table(self, part, nparts):
+        """Boundary records and gain lists (greedy.rs:197-218); see
+        rmetis_hip_gain_table in include/rmetis.h."""
+        part = _i32(part)
+        m = len(self.adjncy)
+        rec = np.zeros(4 * self.nvtxs, np.int32)
+        g0 = np.zeros(self.nvtxs, np.int32)
+        cp = np.zeros(max(m, 1), np.int32)
+        cg = np.zeros(max(m, 1), np.int32)
+        _raise(_lib.lib().rmetis_hip_gain_table(self.nvtxs, _p(self.xadj), _p(self.adjncy), _p(self.adjwgt),
+                                                _p(part), nparts, _p(rec), _p(g0), _p(cp), _p(cg)))
+        return rec.reshape(self.nvtxs, 4), g0, cp, cg
+
+    def boundary_vertices(self, part, nparts=None):
+        """graph/mod.rs:164-171 (ascending)."""
+        part = _i32(part)
+        k = int(part.max()) + 1 if nparts is None else nparts
+        rec, _, _, _ = self.gain_table(part, k)
+        return np.nonzero(rec[:, 0] >= 0)[0].astype(np.int64)
+
+    def build_coarse_graph(self, matching):
+        """graph/mod.rs:284-384 -> (Graph, cmap)."""
+        matching = _i32(matching)
+        n, m = self.nvtxs, len(self.adjncy)
+        cx = np.zeros(n + 1, np.int32)
+        ca = np.zeros(max(m, 1), np.int32)
+        cw = np.zeros(max(m, 1), np.int32)
+        cvw = np.zeros(max(n * self.ncon, 1), np.int32)
+        cvs = np.zeros(max(n, 1), np.int32)
+        cmap = np.zeros(n, np.int32)
+        cn, cm, fl = C.c_int32(), C.c_int32(), C.c_int32()
+        _raise(_lib.lib().rmetis_hip_build_coarse_graph(
+            n, self.ncon, _p(self.xadj), _p(self.adjncy), _p(self.vwgt), _p(self.adjwgt), _p(self.vsize),
+            _p(matching), _p(cx), _p(ca), _p(cw), _p(cvw), _p(cvs), _p(cmap), C.byref(cn), C.byref(cm),
+            C.byref(fl)))
+        cn, cm, f = cn.value, cm.value, fl.value
+        g = Graph(cn, self.ncon, cx[:cn + 1], ca[:cm],
+                  cvw[:cn * self.ncon] if f & 1 else None,
+                  cw[:cm] if f & 4 else None,
+                  cvs[:cn] if f & 2 else None)
+        return g, cmap
+
+
+def greedy_refine(graph, part, nparts, tpwgts, ubvec, niter):
+    """refine/greedy.rs:12-22 -> (new part, cut).  tpwgts: f64 targets
+    [nparts*ncon]; ubvec: f64 [ncon]."""
+    part = np.array(part, dtype=np.int32)
+    tp = np.ascontiguousarray(tpwgts, np.float64)
+    ub = np.ascontiguousarray(ubvec, np.float64)
+    cut = C.c_int32()
+    _raise(_lib.lib().rmetis_hip_greedy_refine(
+        graph.nvtxs, graph.ncon, _p(graph.xadj), _p(graph.adjncy), _p(graph.vwgt), _p(graph.adjwgt), _p(part),
+        nparts, _p(tp, _lib.f64p), _p(ub, _lib.f64p), niter, C.byref(cut)))
+    return part, cut.value
+
+
+def greedy_pass(graph, part, nparts, tpwgts, ubvec, mode, pw):
+    """One synchronous greedy pass (mode 0 repair, 1 cut) -> (part, pw, moves)."""
+    part = np.array(part, dtype=np.int32)
+    pw = np.array(pw, dtype=np.int32).reshape(-1)
+    tp = np.ascontiguousarray(tpwgts, np.float64)
+    ub = np.ascontiguousarray(ubvec, np.float64)
+    moves = np.zeros(2 * graph.nvtxs + 2, np.int32)
+    nm = C.c_int32()
+    _raise(_lib.lib().rmetis_hip_greedy_pass(
+        graph.nvtxs, graph.ncon, _p(graph.xadj), _p(graph.adjncy), _p(graph.vwgt), _p(graph.adjwgt), _p(part),
+        nparts, _p(tp, _lib.f64p), _p(ub, _lib.f64p), mode, _p(pw), _p(moves), C.byref(nm)))
+    return part, pw.reshape(nparts, graph.ncon), moves[:2 * nm.value].reshape(-1, 2)
+
+
+def gpu_matching(graph, seed, ctype=CType.Shem):
+    """The seeded GPU matching that replaces shem/rm matching."""
+    m = np.zeros(graph.nvtxs, np.int32)
+    _raise(_lib.lib().rmetis_hip_match(graph.nvtxs, _p(graph.xadj), _p(graph.adjncy), _p(graph.adjwgt),
+                                       int(ctype), C.c_uint64(seed), _p(m)))
+    return m
+
+
+def part_graph_kway(graph, nparts, tpwgts=None, ubvec=None, options=None):
+    """src/lib.rs:48-56 through RMETIS_PartGraphKway (c_api.rs:55-81)."""
+    options = options if options is not None else Options()
+    n = C.c_int32(graph.nvtxs)
+    nc = C.c_int32(graph.ncon)
+    np_ = C.c_int32(int(nparts))
+    tp = None if tpwgts is None else np.ascontiguousarray(tpwgts, np.float32)
+    ub = None if ubvec is None else np.ascontiguousarray(ubvec, np.float32)
+    opts = options.to_c_array()
+    part = np.zeros(graph.nvtxs, np.int32)
+    obj = C.c_int32()
+    rc = _lib.lib().RMETIS_PartGraphKway(C.byref(n), C.byref(nc), _p(graph.xadj), _p(graph.adjncy),
+                                         _p(graph.vwgt), _p(graph.vsize), _p(graph.adjwgt), C.byref(np_),
+                                         _p(tp, _lib.f32p), _p(ub, _lib.f32p), _p(opts), C.byref(obj), _p(part))
+    _raise(rc)
+    return PartitionResult(part=part, objval=obj.value)
+
+
+def device_count():
+    return _lib.lib().rmetis_hip_device_count()

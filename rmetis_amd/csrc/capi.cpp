@@ -1,0 +1,459 @@
+// capi.cpp -- extern "C" boundary of librmetis_hip.so (include/rmetis.h).
+//
+// RMETIS_SetDefaultOptions / RMETIS_PartGraphKway follow src/ffi/c_api.rs:
+// NULL checks and n/ncon/nparts checks (157-167), optional arrays as None
+// (169-177), option parsing (211-222), CSR validation with
+// graph/validate.rs:16-142 (186-187), part/objval written only on success
+// (196-204) and the error-code mapping (205-207).  Every call runs on its own
+// pooled stream and arena, so concurrent calls are safe.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rmetis.h"
+#include "driver.hpp"
+#include "profiler.hpp"
+
+namespace rk {
+
+namespace {
+std::mutex g_pool_mu;
+std::vector<Context*> g_pool;
+thread_local std::string g_last_error;
+}  // namespace
+
+Context* acquire_context() {
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0)
+        throw Error(-4, "no HIP device is available (the k-way path runs only on the GPU)");
+    int dev = 0;
+    RK_HIP(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        for (size_t i = 0; i < g_pool.size(); ++i)
+            if (g_pool[i]->device == dev) {
+                Context* c = g_pool[i];
+                g_pool.erase(g_pool.begin() + i);
+                return c;
+            }
+    }
+    Context* c = new Context();
+    c->device = dev;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        throw Error(-4, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+    }
+    return c;
+}
+
+void release_context(Context* c) {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_pool.push_back(c);
+}
+
+namespace {
+
+template <typename F>
+int32_t guarded(F&& f) {
+    try {
+        g_last_error.clear();
+        return f();
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host out of memory";
+        return -3;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return -4;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return -4;
+    }
+}
+
+// graph/validate.rs:16-142.  Returns false for MetisError::Input.
+bool validate_csr(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt) {
+    if (n <= 0 || ncon <= 0) return false;
+    if (xadj[0] != 0) return false;
+    for (int32_t i = 0; i < n; ++i)
+        if (xadj[i] > xadj[i + 1]) return false;
+    const int64_t ne = xadj[n];
+    for (int32_t v = 0; v < n; ++v)
+        for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j) {
+            const int32_t u = adjncy[j];
+            if (u < 0 || u >= n || u == v) return false;
+        }
+    std::vector<int32_t> cnt(n, 0);
+    for (int64_t j = 0; j < ne; ++j) cnt[adjncy[j]]++;
+    for (int32_t v = 0; v < n; ++v)
+        if (cnt[v] != xadj[v + 1] - xadj[v]) return false;
+    if (vwgt)
+        for (int64_t i = 0; i < (int64_t)n * ncon; ++i)
+            if (vwgt[i] <= 0) return false;
+    return true;
+}
+
+bool valid_part(int32_t n, const int32_t* part, int32_t nparts) {
+    for (int32_t v = 0; v < n; ++v)
+        if (part[v] < 0 || part[v] >= nparts) return false;
+    return true;
+}
+
+template <typename T>
+T* upload(Context& cx, const T* h, int64_t count) {
+    if (!h) return nullptr;
+    T* d = cx.arena.get<T>(count);
+    if (count > 0) RK_HIP(hipMemcpyAsync(d, h, sizeof(T) * count, hipMemcpyHostToDevice, cx.stream));
+    return d;
+}
+
+template <typename T>
+void download(Context& cx, T* h, const T* d, int64_t count) {
+    if (count > 0) RK_HIP(hipMemcpyAsync(h, d, sizeof(T) * count, hipMemcpyDeviceToHost, cx.stream));
+}
+
+DevGraph upload_graph(Context& cx, int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy,
+                      const int32_t* vwgt, const int32_t* adjwgt, const int32_t* vsize) {
+    DevGraph g;
+    g.n = n;
+    g.ncon = ncon;
+    g.m = xadj[n];
+    g.xadj = upload(cx, xadj, (int64_t)n + 1);
+    g.adjncy = upload(cx, adjncy, g.m);
+    g.vwgt = upload(cx, vwgt, (int64_t)n * ncon);
+    g.adjwgt = upload(cx, adjwgt, g.m);
+    g.vsize = upload(cx, vsize, n);
+    return g;
+}
+
+void fill_stats(const KwayStats& ks, rmetis_hip_stats* out) {
+    if (!out) return;
+    std::memset(out, 0, sizeof(*out));
+    out->nlevels = (int32_t)std::min<size_t>(ks.levels.size(), RMETIS_HIP_MAX_LEVELS);
+    out->coarsest_n = ks.coarsest_n;
+    out->initial_ms = ks.initial_ms;
+    for (int i = 0; i < out->nlevels; ++i) {
+        out->level_n[i] = ks.levels[i].n;
+        out->level_m[i] = ks.levels[i].m;
+        out->coarsen_ms[i] = ks.levels[i].coarsen_ms;
+        out->refine_ms[i] = ks.levels[i].refine_ms;
+        out->passes[i] = ks.levels[i].passes;
+    }
+}
+
+}  // namespace
+}  // namespace rk
+
+using namespace rk;
+
+extern "C" {
+
+int32_t RMETIS_SetDefaultOptions(idx_t* options) {
+    if (!options) return METIS_ERROR_INPUT;
+    default_options_raw(options);
+    return METIS_OK;
+}
+
+int32_t RMETIS_PartGraphKway(const idx_t* nvtxs, const idx_t* ncon, const idx_t* xadj, const idx_t* adjncy,
+                             const idx_t* vwgt, const idx_t* vsize, const idx_t* adjwgt, const idx_t* nparts,
+                             const real_t* tpwgts, const real_t* ubvec, const idx_t* options, idx_t* objval,
+                             idx_t* part) {
+    return guarded([&]() -> int32_t {
+        if (!nvtxs || !ncon || !xadj || !adjncy || !nparts || !part) return METIS_ERROR_INPUT;
+        const int32_t n = *nvtxs, nc = *ncon, np = *nparts;
+        if (n <= 0 || nc <= 0 || np < 2) return METIS_ERROR_INPUT;
+        if (!validate_csr(n, nc, xadj, adjncy, vwgt)) return METIS_ERROR_INPUT;
+        if (np > n) return METIS_ERROR_INPUT;                       // kway.rs:34-38
+        const Options o = parse_options(options);
+        ContextGuard cx;
+        DevGraph g = upload_graph(*cx.c, n, nc, xadj, adjncy, vwgt, adjwgt, nullptr);
+        (void)vsize;  // the k-way greedy path never reads vsize
+        int32_t* d_part = cx->arena.get<int32_t>(n);
+        const int32_t obj = kway_device(*cx.c, g, np, tpwgts, ubvec, o, d_part, nullptr);
+        std::vector<int32_t> hp(n);
+        download(*cx.c, hp.data(), d_part, n);
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        std::copy(hp.begin(), hp.end(), part);
+        if (objval) *objval = obj;
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_kway_device(idx_t nvtxs, idx_t ncon, const idx_t* d_xadj, const idx_t* d_adjncy,
+                               const idx_t* d_vwgt, const idx_t* d_adjwgt, idx_t nparts, const real_t* tpwgts,
+                               const real_t* ubvec, const idx_t* options, idx_t* objval, idx_t* d_part,
+                               rmetis_hip_stats* stats) {
+    return guarded([&]() -> int32_t {
+        if (!d_xadj || !d_adjncy || !d_part) return METIS_ERROR_INPUT;
+        if (nvtxs <= 0 || ncon <= 0 || nparts < 2) return METIS_ERROR_INPUT;
+        const Options o = parse_options(options);
+        ContextGuard cx;
+        DevGraph g;
+        g.n = nvtxs;
+        g.ncon = ncon;
+        RK_HIP(hipMemcpyAsync(&g.m, d_xadj + nvtxs, sizeof(int32_t), hipMemcpyDeviceToHost, cx->stream));
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        g.xadj = (int32_t*)d_xadj;
+        g.adjncy = (int32_t*)d_adjncy;
+        g.vwgt = (int32_t*)d_vwgt;
+        g.adjwgt = (int32_t*)d_adjwgt;
+        KwayStats ks;
+        const int32_t obj = kway_device(*cx.c, g, nparts, tpwgts, ubvec, o, d_part, stats ? &ks : nullptr);
+        if (stats) fill_stats(ks, stats);
+        if (objval) *objval = obj;
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_edge_cut(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* adjwgt,
+                            const idx_t* part, idx_t* cut) {
+    return guarded([&]() -> int32_t {
+        if (!xadj || !adjncy || !part || !cut || nvtxs <= 0) return METIS_ERROR_INPUT;
+        if (!validate_csr(nvtxs, 1, xadj, adjncy, nullptr)) return METIS_ERROR_INPUT;
+        ContextGuard cx;
+        DevGraph g = upload_graph(*cx.c, nvtxs, 1, xadj, adjncy, nullptr, adjwgt, nullptr);
+        int32_t* dp = upload(*cx.c, part, nvtxs);
+        auto* acc = cx->arena.get<unsigned long long>(1);
+        int32_t* out = cx->arena.get<int32_t>(1);
+        edge_cut(g, dp, acc, out, pick_group(g.n, g.m), cx->stream);
+        download(*cx.c, cut, out, 1);
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_partition_weights(idx_t nvtxs, idx_t ncon, const idx_t* vwgt, const idx_t* part,
+                                     idx_t nparts, idx_t* pw) {
+    return guarded([&]() -> int32_t {
+        if (!part || !pw || nvtxs <= 0 || ncon <= 0 || nparts <= 0) return METIS_ERROR_INPUT;
+        if (!valid_part(nvtxs, part, nparts)) return METIS_ERROR_INPUT;
+        ContextGuard cx;
+        DevGraph g;
+        g.n = nvtxs;
+        g.ncon = ncon;
+        g.vwgt = upload(*cx.c, vwgt, (int64_t)nvtxs * ncon);
+        int32_t* dp = upload(*cx.c, part, nvtxs);
+        int32_t* dpw = cx->arena.get<int32_t>((int64_t)nparts * ncon);
+        part_weights(g, dp, nparts, dpw, cx->stream);
+        download(*cx.c, pw, dpw, (int64_t)nparts * ncon);
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_gain_table(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* adjwgt,
+                              const idx_t* part, idx_t nparts, idx_t* rec, idx_t* g0, idx_t* cand_part,
+                              idx_t* cand_gain) {
+    return guarded([&]() -> int32_t {
+        if (!xadj || !adjncy || !part || !rec || !g0 || !cand_part || !cand_gain || nvtxs <= 0 || nparts < 1)
+            return METIS_ERROR_INPUT;
+        if (!validate_csr(nvtxs, 1, xadj, adjncy, nullptr) || !valid_part(nvtxs, part, nparts))
+            return METIS_ERROR_INPUT;
+        ContextGuard cx;
+        DevGraph g = upload_graph(*cx.c, nvtxs, 1, xadj, adjncy, nullptr, adjwgt, nullptr);
+        int32_t* dp = upload(*cx.c, part, nvtxs);
+        Buffers B = alloc_buffers(cx->arena, g.n, g.m, nparts, 1);
+        int4* vrec = cx->arena.get<int4>(nvtxs);
+        int32_t* dg0 = cx->arena.get<int32_t>(nvtxs);
+        int32_t* gains = cx->arena.get<int32_t>(g.m);
+        RK_HIP(hipMemsetAsync(dg0, 0, sizeof(int32_t) * nvtxs, cx->stream));
+        const int G = pick_group(g.n, g.m);
+        const int nheavy = refine_setup_level(g, G, B.rb, cx->stream);
+        gain_table(g, nparts, G, nheavy, B.rb, dp, vrec, dg0, gains, cx->stream);
+        download(*cx.c, (int4*)rec, vrec, nvtxs);
+        download(*cx.c, g0, dg0, nvtxs);
+        download(*cx.c, cand_part, B.rb.pool, g.m);
+        download(*cx.c, cand_gain, gains, g.m);
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_greedy_pass(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                               const idx_t* vwgt, const idx_t* adjwgt, idx_t* part, idx_t nparts,
+                               const double* tpwgts, const double* ubvec, idx_t mode, idx_t* pw, idx_t* moves,
+                               idx_t* nmoves) {
+    return guarded([&]() -> int32_t {
+        if (!xadj || !adjncy || !part || !tpwgts || !ubvec || !pw || nvtxs <= 0 || ncon <= 0 || nparts < 1 ||
+            (mode != 0 && mode != 1))
+            return METIS_ERROR_INPUT;
+        if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt) || !valid_part(nvtxs, part, nparts))
+            return METIS_ERROR_INPUT;
+        ContextGuard cx;
+        hipStream_t s = cx->stream;
+        DevGraph g = upload_graph(*cx.c, nvtxs, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
+        int32_t* dp = upload(*cx.c, part, nvtxs);
+        Buffers B = alloc_buffers(cx->arena, g.n, g.m, nparts, ncon);
+        const size_t kc = (size_t)nparts * ncon;
+        std::vector<long long> fm(kc), om(kc);
+        thresholds_from_targets(nparts, ncon, tpwgts, ubvec, fm.data(), om.data());
+        const int G = pick_group(g.n, g.m);
+        const int nheavy = refine_setup_level(g, G, B.rb, s);
+        RK_HIP(hipMemcpyAsync(B.rb.fmax, fm.data(), kc * sizeof(long long), hipMemcpyHostToDevice, s));
+        RK_HIP(hipMemcpyAsync(B.rb.omax, om.data(), kc * sizeof(long long), hipMemcpyHostToDevice, s));
+        RK_HIP(hipMemcpyAsync(B.rb.pw, pw, kc * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        RK_HIP(hipMemsetAsync(B.rb.done, 0, sizeof(int32_t), s));
+        RK_HIP(hipMemsetAsync(B.rb.passes, 0, sizeof(int32_t), s));
+        refine_pass(g, nparts, mode, G, nheavy, B.rb, dp, s);
+        int32_t nm = 0;
+        download(*cx.c, &nm, B.rb.nmoves, 1);
+        RK_HIP(hipStreamSynchronize(s));
+        download(*cx.c, part, dp, nvtxs);
+        download(*cx.c, pw, B.rb.pw, kc);
+        if (moves && nm > 0) download(*cx.c, (int2*)moves, B.rb.moves, nm);
+        RK_HIP(hipStreamSynchronize(s));
+        if (nmoves) *nmoves = nm;
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_greedy_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                                 const idx_t* vwgt, const idx_t* adjwgt, idx_t* part, idx_t nparts,
+                                 const double* tpwgts, const double* ubvec, idx_t niter, idx_t* cut) {
+    return guarded([&]() -> int32_t {
+        if (!xadj || !adjncy || !part || !tpwgts || !ubvec || nvtxs <= 0 || ncon <= 0 || nparts < 1 || niter < 0)
+            return METIS_ERROR_INPUT;
+        if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt) || !valid_part(nvtxs, part, nparts))
+            return METIS_ERROR_INPUT;
+        ContextGuard cx;
+        hipStream_t s = cx->stream;
+        DevGraph g = upload_graph(*cx.c, nvtxs, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
+        int32_t* dp = upload(*cx.c, part, nvtxs);
+        Buffers B = alloc_buffers(cx->arena, g.n, g.m, nparts, ncon);
+        const size_t kc = (size_t)nparts * ncon;
+        std::vector<long long> fm(kc), om(kc);
+        thresholds_from_targets(nparts, ncon, tpwgts, ubvec, fm.data(), om.data());
+        refine_level(*cx.c, g, nparts, fm.data(), om.data(), niter, dp, B, nullptr);
+        edge_cut(g, dp, B.cut_acc, B.cut_out, pick_group(g.n, g.m), s);
+        download(*cx.c, part, dp, nvtxs);
+        int32_t c = 0;
+        download(*cx.c, &c, B.cut_out, 1);
+        RK_HIP(hipStreamSynchronize(s));
+        if (cut) *cut = c;
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_match(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* adjwgt, idx_t ctype,
+                         uint64_t seed, idx_t* matching) {
+    return guarded([&]() -> int32_t {
+        if (!xadj || !adjncy || !matching || nvtxs <= 0) return METIS_ERROR_INPUT;
+        if (!validate_csr(nvtxs, 1, xadj, adjncy, nullptr)) return METIS_ERROR_INPUT;
+        ContextGuard cx;
+        DevGraph g = upload_graph(*cx.c, nvtxs, 1, xadj, adjncy, nullptr, adjwgt, nullptr);
+        Buffers B = alloc_buffers(cx->arena, g.n, g.m, 2, 1);
+        match_graph(g, ctype != 0, seed, 24, B.cb, cx->stream);
+        download(*cx.c, matching, B.cb.match, nvtxs);
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_build_coarse_graph(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                                      const idx_t* vwgt, const idx_t* adjwgt, const idx_t* vsize,
+                                      const idx_t* matching, idx_t* cxadj, idx_t* cadjncy, idx_t* cadjwgt,
+                                      idx_t* cvwgt, idx_t* cvsize, idx_t* cmap, idx_t* cn_out, idx_t* cm_out,
+                                      idx_t* flags) {
+    return guarded([&]() -> int32_t {
+        if (!xadj || !adjncy || !matching || !cxadj || !cadjncy || !cadjwgt || !cmap || nvtxs <= 0 || ncon <= 0)
+            return METIS_ERROR_INPUT;
+        if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt)) return METIS_ERROR_INPUT;
+        for (int32_t v = 0; v < nvtxs; ++v) {
+            const int32_t u = matching[v];
+            if (u < 0 || u >= nvtxs || matching[u] != v) return METIS_ERROR_INPUT;  // symmetric only
+        }
+        ContextGuard cx;
+        hipStream_t s = cx->stream;
+        DevGraph g = upload_graph(*cx.c, nvtxs, ncon, xadj, adjncy, vwgt, adjwgt, vsize);
+        Buffers B = alloc_buffers(cx->arena, g.n, g.m, 2, ncon);
+        int32_t* dm = upload(*cx.c, matching, nvtxs);
+        int32_t* dcmap = cx->arena.get<int32_t>(nvtxs);
+        struct A : CoarseAlloc {
+            Arena& a;
+            explicit A(Arena& ar) : a(ar) {}
+            int32_t* alloc_i32(int64_t c) override { return a.get<int32_t>(c); }
+        } al(cx->arena);
+        const int32_t cn = contract_numbering(g, dm, dcmap, B.cb, s);
+        DevGraph cg;
+        int not_one = 0;
+        const int32_t cm = contract_build(g, dm, dcmap, cn, cg, true, B.cb, al, &not_one, s);
+        download(*cx.c, cxadj, cg.xadj, (int64_t)cn + 1);
+        download(*cx.c, cadjncy, cg.adjncy, cm);
+        download(*cx.c, cadjwgt, cg.adjwgt, cm);
+        if (cg.vwgt && cvwgt) download(*cx.c, cvwgt, cg.vwgt, (int64_t)cn * ncon);
+        if (cg.vsize && cvsize) download(*cx.c, cvsize, cg.vsize, cn);
+        download(*cx.c, cmap, dcmap, nvtxs);
+        RK_HIP(hipStreamSynchronize(s));
+        if (cn_out) *cn_out = cn;
+        if (cm_out) *cm_out = cm;
+        if (flags)
+            *flags = (cg.vwgt ? 1 : 0) | (cg.vsize ? 2 : 0) | ((adjwgt != nullptr || not_one) ? 4 : 0);
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_validate_csr(idx_t nvtxs, idx_t ncon, const idx_t* xadj, int64_t xadj_len,
+                                const idx_t* adjncy, int64_t adjncy_len, const idx_t* vwgt, int64_t vwgt_len,
+                                const idx_t* adjwgt, int64_t adjwgt_len) {
+    // graph/validate.rs:16-142 with explicit array lengths (the Rust Vec lengths).
+    return guarded([&]() -> int32_t {
+        if (nvtxs <= 0 || ncon <= 0 || !xadj) return METIS_ERROR_INPUT;
+        if (xadj_len != (int64_t)nvtxs + 1) return METIS_ERROR_INPUT;
+        if (xadj[0] != 0) return METIS_ERROR_INPUT;
+        for (int32_t i = 0; i < nvtxs; ++i)
+            if (xadj[i] > xadj[i + 1]) return METIS_ERROR_INPUT;
+        if (adjncy_len != (int64_t)xadj[nvtxs]) return METIS_ERROR_INPUT;
+        if (adjncy_len > 0 && !adjncy) return METIS_ERROR_INPUT;
+        if (adjwgt && adjwgt_len != adjncy_len) return METIS_ERROR_INPUT;
+        if (vwgt && vwgt_len != (int64_t)nvtxs * ncon) return METIS_ERROR_INPUT;
+        return validate_csr(nvtxs, ncon, xadj, adjncy, vwgt) ? METIS_OK : METIS_ERROR_INPUT;
+    });
+}
+
+void* rmetis_hip_device_alloc(int64_t bytes) {
+    void* p = nullptr;
+    if (bytes <= 0 || hipMalloc(&p, (size_t)bytes) != hipSuccess) return nullptr;
+    return p;
+}
+
+void rmetis_hip_device_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+int32_t rmetis_hip_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind) {
+    // kind: 1 host->device, 2 device->host, 3 device->device
+    if (bytes < 0 || (bytes > 0 && (!dst || !src))) return METIS_ERROR_INPUT;
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                                     : kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    if (bytes > 0 && hipMemcpy(dst, src, (size_t)bytes, k) != hipSuccess) return METIS_ERROR;
+    return METIS_OK;
+}
+
+int32_t rmetis_hip_device_synchronize(void) {
+    return hipDeviceSynchronize() == hipSuccess ? METIS_OK : METIS_ERROR;
+}
+
+void rmetis_hip_profile_enable(int32_t on) { Profiler::get().enable(on != 0); }
+
+int32_t rmetis_hip_profile_read(int32_t tag, int64_t size, int64_t* launches, double* ms, double* bytes) {
+    if (!launches || !ms || !bytes || tag < 0 || tag >= kProfTags) return METIS_ERROR_INPUT;
+    long long l = 0;
+    Profiler::get().read(tag, size, &l, ms, bytes);
+    *launches = l;
+    return METIS_OK;
+}
+
+const char* rmetis_hip_last_error(void) { return g_last_error.c_str(); }
+
+int32_t rmetis_hip_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// common.hpp -- shared definitions for the gfx950 k-way kernels and host code.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace rk {
+
+// Error type carrying a METIS return code (-2 input, -3 memory, -4 other).
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define RK_HIP(call)                                                                    \
+    do {                                                                                \
+        hipError_t _e = (call);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            throw ::rk::Error(_e == hipErrorOutOfMemory ? -3 : -4,                      \
+                              std::string("HIP error: ") + hipGetErrorString(_e) +      \
+                                  " at " __FILE__ ":" + std::to_string(__LINE__));      \
+        }                                                                               \
+    } while (0)
+
+constexpr int kBlock = 256;        // threads per block for streaming kernels
+constexpr int kTile = 256;         // vertices per gain-kernel tile (one block)
+constexpr int kNone = -1;
+
+// Integer arithmetic with the reference's i32 wrap-around semantics.
+__host__ __device__ inline int32_t wadd(int32_t a, int32_t b) {
+    return (int32_t)((uint32_t)a + (uint32_t)b);
+}
+__host__ __device__ inline int32_t wsub(int32_t a, int32_t b) {
+    return (int32_t)((uint32_t)a - (uint32_t)b);
+}
+
+// A graph level resident in HBM.  vwgt == nullptr means unit vertex weights
+// (the reference's None), adjwgt == nullptr unit edge weights.
+struct DevGraph {
+    int32_t n = 0;
+    int32_t m = 0;         // xadj[n] (directed entries)
+    int32_t ncon = 1;
+    int32_t* xadj = nullptr;
+    int32_t* adjncy = nullptr;
+    int32_t* adjwgt = nullptr;
+    int32_t* vwgt = nullptr;
+    int32_t* vsize = nullptr;
+};
+
+// Lane-group helpers for G-lane sub-wave groups (G divides 64).
+template <int G>
+__device__ inline uint64_t group_ballot(bool p) {
+    uint64_t b = __ballot(p);
+    if constexpr (G == 64) {
+        return b;
+    } else {
+        const int base = (__lane_id() / G) * G;
+        return (b >> base) & ((1ull << G) - 1ull);
+    }
+}
+
+template <int G, typename T>
+__device__ inline T group_sum(T x) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, G);
+    return x;
+}
+
+template <int G>
+__device__ inline int group_lane() { return __lane_id() & (G - 1); }
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Exclusive prefix of a 0/1 flag over a whole block (blockDim.x multiple of
+// 64, at most 1024).  s_tmp needs blockDim.x/64 ints.  Contains barriers.
+__device__ inline int block_flag_scan(bool flag, int* s_tmp, int* total) {
+    const int lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    uint64_t b = __ballot(flag);
+    int pre = __popcll(b & ((1ull << lane) - 1ull));
+    if (lane == 0) s_tmp[wave] = __popcll(b);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+        int c = s_tmp[w];
+        if (w < wave) off += c;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + pre;
+}
+
+// Inclusive wave scan of int64 values.
+__device__ inline long long wave_incl_scan(long long x) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        long long y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+}  // namespace rk

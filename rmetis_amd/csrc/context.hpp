@@ -1,0 +1,113 @@
+// context.hpp -- per-call device context: one HIP stream (default priority),
+// a bump arena of device memory and pinned staging.  Contexts are pooled per
+// device so that concurrent C-ABI calls each own a stream and arena (the
+// reference's entry points are reentrant, c_api.rs:55-81) and repeated calls
+// reuse their memory instead of calling hipMalloc again.
+#pragma once
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+
+namespace rk {
+
+class Arena {
+public:
+    struct Mark {
+        size_t chunk, off;
+    };
+    ~Arena() { release(); }
+    void* alloc(size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        if (bytes == 0) bytes = 256;
+        while (cur_ < chunks_.size()) {
+            if (off_ + bytes <= chunks_[cur_].size) {
+                char* p = chunks_[cur_].base + off_;
+                off_ += bytes;
+                return p;
+            }
+            ++cur_;
+            off_ = 0;
+        }
+        size_t sz = bytes > kChunk ? bytes : kChunk;
+        void* p = nullptr;
+        RK_HIP(hipMalloc(&p, sz));
+        chunks_.push_back({(char*)p, sz});
+        cur_ = chunks_.size() - 1;
+        off_ = bytes;
+        return p;
+    }
+    template <typename T>
+    T* get(int64_t count) {
+        return (T*)alloc((size_t)(count > 0 ? count : 1) * sizeof(T));
+    }
+    Mark mark() const { return {cur_, off_}; }
+    void rewind(Mark m) {
+        cur_ = m.chunk;
+        off_ = m.off;
+    }
+    void reset() {
+        cur_ = 0;
+        off_ = 0;
+    }
+    void release() {
+        for (auto& c : chunks_) (void)hipFree(c.base);
+        chunks_.clear();
+        reset();
+    }
+    size_t reserved() const {
+        size_t t = 0;
+        for (auto& c : chunks_) t += c.size;
+        return t;
+    }
+
+private:
+    static constexpr size_t kChunk = 64ull << 20;
+    struct Chunk {
+        char* base;
+        size_t size;
+    };
+    std::vector<Chunk> chunks_;
+    size_t cur_ = 0, off_ = 0;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Arena arena;
+    void* pinned = nullptr;
+    size_t pinned_size = 0;
+
+    void* host_stage(size_t bytes) {
+        if (bytes > pinned_size) {
+            if (pinned) (void)hipHostFree(pinned);
+            pinned = nullptr;
+            size_t sz = bytes < (1u << 20) ? (1u << 20) : bytes;
+            RK_HIP(hipHostMalloc(&pinned, sz, hipHostMallocDefault));
+            pinned_size = sz;
+        }
+        return pinned;
+    }
+    ~Context() {
+        if (pinned) (void)hipHostFree(pinned);
+        arena.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// Acquire a context for the current device (creates one when the pool is
+// empty).  Throws rk::Error(-4) when no HIP device is usable.
+Context* acquire_context();
+void release_context(Context* c);
+
+struct ContextGuard {
+    Context* c;
+    ContextGuard() : c(acquire_context()) {}
+    ~ContextGuard() {
+        c->arena.reset();
+        release_context(c);
+    }
+    Context* operator->() { return c; }
+};
+
+}  // namespace rk

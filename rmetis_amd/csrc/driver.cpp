@@ -1,0 +1,418 @@
+// driver.cpp -- multilevel k-way scheduler on the host (partition/kway.rs:20-162).
+//
+// Per attempt: coarsen on the device until the level is small or stalls
+// (coarsen/mod.rs:36-71), grow an initial partition of the coarsest level on
+// the host (initial/grow.rs:12-103), then refine the coarsest level and every
+// finer level after projecting (kway.rs:73-92).  Keeps the strictly best cut
+// over the ncuts attempts (kway.rs:96-98).
+#include "driver.hpp"
+
+#include "profiler.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <deque>
+#include <random>
+
+namespace rk {
+
+Options parse_options(const int32_t* a) {
+    Options o;
+    if (!a || a[0] == -1) return o;                 // c_api.rs:211-222
+    o.ctype_shem = a[2] == 0 ? 0 : 1;               // types.rs:192-194
+    o.iptype = a[3] == 1 ? 1 : (a[3] == 4 ? 4 : 0); // types.rs:195-199
+    o.rtype_greedy = a[4] == 1 ? 1 : 0;             // types.rs:200-202
+    o.objtype = a[1] == 1 ? 1 : 0;
+    o.no2hop = a[9] != 0 ? 1 : 0;
+    if (a[7] > 0) o.ncuts = a[7];
+    if (a[6] > 0) o.niter = a[6];
+    o.seed = a[8] > 0 ? (uint64_t)a[8] : 0;         // types.rs:215: max(0)
+    o.minconn = a[10] != 0;
+    o.contig = a[11] != 0;
+    if (a[16] > 0) o.ufactor = a[16];
+    o.numbering = (uint8_t)a[17];
+    o.dbglvl = a[5];
+    return o;
+}
+
+void default_options_raw(int32_t* arr) {
+    // types.rs:227-245 with Options::default() (types.rs:77-95)
+    for (int i = 0; i < kNOptions; ++i) arr[i] = -1;
+    arr[2] = 1;    // CTYPE  Shem
+    arr[3] = 0;    // IPTYPE Grow
+    arr[4] = 0;    // RTYPE  Fm
+    arr[1] = 0;    // OBJTYPE Cut
+    arr[9] = 0;    // NO2HOP
+    arr[7] = 1;    // NCUTS
+    arr[6] = 10;   // NITER
+    arr[8] = 0;    // SEED
+    arr[10] = 0;   // MINCONN
+    arr[11] = 0;   // CONTIG
+    arr[16] = 30;  // UFACTOR
+    arr[17] = 0;   // NUMBERING
+    arr[5] = 0;    // DBGLVL
+}
+
+namespace {
+
+// For integer w and real t: (w as f64 <= t) <=> w <= floor(t), and
+// (w as f64 > t) <=> w > floor(t).  NaN compares false both ways.
+long long floor_threshold(double t, bool feasibility) {
+    if (std::isnan(t)) return feasibility ? LLONG_MIN : LLONG_MAX;
+    if (t >= 9.0e18) return LLONG_MAX;
+    if (t <= -9.0e18) return LLONG_MIN;
+    return (long long)std::floor(t);
+}
+
+struct ArenaAlloc : CoarseAlloc {
+    Arena& a;
+    explicit ArenaAlloc(Arena& ar) : a(ar) {}
+    int32_t* alloc_i32(int64_t count) override { return a.get<int32_t>(count); }
+};
+
+constexpr int kMatchRounds = 24;
+
+}  // namespace
+
+void level_thresholds(int32_t k, int32_t ncon, const float* tpwgts, const int32_t* total, const double* ub,
+                      long long* fmax, long long* omax) {
+    for (int32_t p = 0; p < k; ++p)
+        for (int32_t c = 0; c < ncon; ++c) {
+            const size_t i = (size_t)p * ncon + c;
+            const double f = tpwgts ? (double)tpwgts[i] : 1.0 / (double)k;   // graph/mod.rs:401-405
+            const double target = f * (double)total[c];                        // graph/mod.rs:406
+            const double t = target * ub[c];                                   // greedy.rs:70,87,144
+            fmax[i] = floor_threshold(t, true);
+            omax[i] = floor_threshold(t, false);
+        }
+}
+
+void thresholds_from_targets(int32_t k, int32_t ncon, const double* tpw, const double* ub, long long* fmax,
+                             long long* omax) {
+    for (int32_t p = 0; p < k; ++p)
+        for (int32_t c = 0; c < ncon; ++c) {
+            const size_t i = (size_t)p * ncon + c;
+            const double t = tpw[i] * ub[c];
+            fmax[i] = floor_threshold(t, true);
+            omax[i] = floor_threshold(t, false);
+        }
+}
+
+Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon) {
+    Buffers B;
+    const int64_t ntiles = ((int64_t)n + kTile - 1) / kTile;
+    const int64_t kc = (int64_t)k * ncon;
+    RefineBufs& r = B.rb;
+    r.tile_rec = a.get<int4>(ntiles * kTile);
+    r.tile_cnt = a.get<int32_t>(ntiles);
+    r.tile_off = a.get<int32_t>(ntiles);
+    r.mov = a.get<int4>(n);
+    r.pool = a.get<int32_t>(m);
+    r.heavy_list = a.get<int32_t>(n);
+    r.heavy_rec = a.get<int4>(n);
+    r.heavy_scratch = k > 4096 ? a.get<int32_t>((int64_t)kHeavyGlobalBlocks * (3 * (int64_t)k + 2)) : nullptr;
+    r.moves = a.get<int2>(n);
+    r.pw = a.get<int32_t>(kc);
+    r.fmax = a.get<long long>(kc);
+    r.omax = a.get<long long>(kc);
+    r.gA = a.get<int32_t>(kc);
+    r.gR = a.get<int32_t>(kc);
+    int32_t* cnt = a.get<int32_t>(64);
+    r.heavy_cnt = cnt + 0;
+    r.nmov = cnt + 1;
+    r.nmoves = cnt + 2;
+    r.done = cnt + 3;
+    r.passes = cnt + 4;
+    r.scan_tmp = a.alloc(scan_temp_bytes(std::max<int64_t>(ntiles, n)));
+    CoarsenBufs& c = B.cb;
+    c.match = a.get<int32_t>(n);
+    c.cand = a.get<int32_t>(n);
+    c.leader = a.get<int32_t>(n);
+    c.cid = a.get<int32_t>(n);
+    c.members = a.get<int32_t>(2 * (int64_t)n);
+    c.degsum = a.get<int32_t>(n);
+    c.tmp_off = a.get<int32_t>(n);
+    c.tmp_adj = a.get<int32_t>(m);
+    c.tmp_w = a.get<int32_t>(m);
+    c.cdeg = a.get<int32_t>(n);
+    c.big_list = a.get<int32_t>(n);
+    c.hash_keys = a.get<int32_t>(2 * (int64_t)m + n + 1);
+    c.hash_first = a.get<int32_t>(2 * (int64_t)m + n + 1);
+    c.hash_sum = a.get<int32_t>(2 * (int64_t)m + n + 1);
+    c.counters = a.get<int32_t>(16);
+    c.round_flags = a.get<int32_t>(kMaxMatchRounds);
+    c.scan_tmp = r.scan_tmp;
+    B.cut_acc = a.get<unsigned long long>(1);
+    B.cut_out = a.get<int32_t>(1);
+    B.wsum = a.get<long long>(std::max(ncon, 1));
+    return B;
+}
+
+std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffers& B) {
+    std::vector<int32_t> tot(g.ncon, g.n);
+    if (!g.vwgt) return tot;
+    std::vector<long long> s(g.ncon);
+    column_sums(g, B.wsum, cx.stream);
+    RK_HIP(hipMemcpyAsync(s.data(), B.wsum, sizeof(long long) * g.ncon, hipMemcpyDeviceToHost, cx.stream));
+    RK_HIP(hipStreamSynchronize(cx.stream));
+    for (int c = 0; c < g.ncon; ++c) {
+        if (s[c] > INT32_MAX)
+            throw Error(-4, "total vertex weight of a constraint exceeds the int32 range");
+        tot[c] = (int32_t)s[c];
+    }
+    return tot;
+}
+
+void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* h_fmax, const long long* h_omax,
+                  int niter, int32_t* d_part, Buffers& B, LevelStat* st) {
+    hipStream_t s = cx.stream;
+    const int G = pick_group(g.n, g.m);
+    const int nheavy = refine_setup_level(g, G, B.rb, s);   // synchronises the stream
+    const size_t kc = (size_t)k * g.ncon;
+    long long* stage = (long long*)cx.host_stage(2 * kc * sizeof(long long));
+    std::copy(h_fmax, h_fmax + kc, stage);
+    std::copy(h_omax, h_omax + kc, stage + kc);
+    RK_HIP(hipMemcpyAsync(B.rb.fmax, stage, kc * sizeof(long long), hipMemcpyHostToDevice, s));
+    RK_HIP(hipMemcpyAsync(B.rb.omax, stage + kc, kc * sizeof(long long), hipMemcpyHostToDevice, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (st) {
+        RK_HIP(hipEventCreate(&e0));
+        RK_HIP(hipEventCreate(&e1));
+        RK_HIP(hipEventRecord(e0, s));
+        RK_HIP(hipMemsetAsync(B.rb.passes, 0, sizeof(int32_t), s));
+    }
+    part_weights(g, d_part, k, B.rb.pw, s);                        // greedy.rs:50
+    RK_HIP(hipMemsetAsync(B.rb.done, 0, sizeof(int32_t), s));
+    refine_pass(g, k, 0, G, nheavy, B.rb, d_part, s);               // greedy.rs:60-123
+    for (int it = 0; it < niter; ++it) refine_pass(g, k, 1, G, nheavy, B.rb, d_part, s);  // 126-190
+    if (st) {
+        RK_HIP(hipEventRecord(e1, s));
+        int32_t passes = 0;
+        RK_HIP(hipMemcpyAsync(&passes, B.rb.passes, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        RK_HIP(hipEventSynchronize(e1));
+        RK_HIP(hipStreamSynchronize(s));
+        RK_HIP(hipEventElapsedTime(&st->refine_ms, e0, e1));
+        st->passes = passes;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    // The host staging buffer is reused by the next level only after the
+    // next refine_setup_level synchronisation.
+}
+
+bool grow_host(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
+               int32_t k, const double* tpw, Rng& rng, int32_t* part) {
+    // initial/grow.rs:12-103
+    if (n < k) return false;
+    std::fill(part, part + n, -1);
+    std::vector<int64_t> pwt((size_t)k * ncon, 0);
+    auto vw = [&](int32_t v, int32_t c) -> int64_t { return vwgt ? vwgt[(size_t)v * ncon + c] : 1; };
+    std::vector<int32_t> seeds;
+    std::vector<char> used(n, 0);
+    while ((int32_t)seeds.size() < k) {                                  // grow.rs:24-32
+        const int32_t v = (int32_t)rng.below((uint64_t)n);
+        if (!used[v]) {
+            used[v] = 1;
+            seeds.push_back(v);
+        }
+    }
+    std::vector<std::deque<int32_t>> q(k);
+    for (int32_t p = 0; p < k; ++p) {
+        q[p].push_back(seeds[p]);
+        part[seeds[p]] = p;
+        for (int32_t c = 0; c < ncon; ++c) pwt[(size_t)p * ncon + c] += vw(seeds[p], c);
+    }
+    int32_t active = k;
+    while (active > 0) {                                                 // grow.rs:50-86
+        active = 0;
+        for (int32_t p = 0; p < k; ++p) {
+            bool at_target = true;
+            for (int32_t c = 0; c < ncon; ++c)
+                if (!((double)pwt[(size_t)p * ncon + c] >= tpw[(size_t)p * ncon + c])) {
+                    at_target = false;
+                    break;
+                }
+            if (at_target) continue;
+            bool grew = false;
+            while (!q[p].empty()) {
+                const int32_t v = q[p].front();
+                q[p].pop_front();
+                for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j) {
+                    const int32_t u = adjncy[j];
+                    if (part[u] == -1) {
+                        part[u] = p;
+                        for (int32_t c = 0; c < ncon; ++c) pwt[(size_t)p * ncon + c] += vw(u, c);
+                        q[p].push_back(u);
+                        grew = true;
+                    }
+                }
+                if (grew) break;
+            }
+            if (grew) ++active;
+        }
+    }
+    // grow.rs:89-100: leftovers to the lightest part on constraint 0 (first
+    // minimum).  A running minimum over parts is kept in a min-heap keyed by
+    // (weight, part) so the scan is O(log k) per leftover vertex.
+    std::vector<std::pair<int64_t, int32_t>> heap;
+    heap.reserve(k);
+    for (int32_t p = 0; p < k; ++p) heap.push_back({pwt[(size_t)p * ncon], p});
+    auto cmp = [](const std::pair<int64_t, int32_t>& x, const std::pair<int64_t, int32_t>& y) {
+        return x.first != y.first ? x.first > y.first : x.second > y.second;
+    };
+    std::make_heap(heap.begin(), heap.end(), cmp);
+    for (int32_t v = 0; v < n; ++v) {
+        if (part[v] != -1) continue;
+        std::pop_heap(heap.begin(), heap.end(), cmp);
+        auto top = heap.back();
+        const int32_t best = top.second;
+        part[v] = best;
+        for (int32_t c = 0; c < ncon; ++c) pwt[(size_t)best * ncon + c] += vw(v, c);
+        heap.back().first = pwt[(size_t)best * ncon];
+        std::push_heap(heap.begin(), heap.end(), cmp);
+    }
+    return true;
+}
+
+int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwgts, const float* ubvec,
+                    const Options& o, int32_t* d_part, KwayStats* st) {
+    if (k < 2) throw Error(-2, "nparts must be >= 2");                  // kway.rs:31-33
+    if (k > g.n) throw Error(-2, "nparts > nvtxs");                     // kway.rs:34-38
+    if (!o.rtype_greedy)
+        throw Error(-4, "rtype Fm is not provided by the GPU k-way path; set options[METIS_OPTION_RTYPE]=1 "
+                        "(Greedy)");
+    hipStream_t s = cx.stream;
+    const int32_t ncon = g.ncon;
+    Buffers B = alloc_buffers(cx.arena, g.n, g.m, k, ncon);
+    const std::vector<int32_t> total = total_weight_checked(cx, g, B);
+    std::vector<double> ub(ncon);
+    for (int32_t c = 0; c < ncon; ++c)                                  // kway.rs:44-49
+        ub[c] = ubvec ? (double)ubvec[c] : 1.0 + (double)o.ufactor / 1000.0;
+    const int32_t limit = std::max(20 * k, 20);                         // coarsen/mod.rs:43
+    const size_t kc = (size_t)k * ncon;
+    std::vector<long long> fmax(kc), omax(kc);
+    std::vector<int32_t> ltot(ncon);
+    bool have_best = false;
+    int32_t best = 0;
+    if (st) *st = KwayStats{};
+    ArenaAlloc alloc(cx.arena);
+    for (int attempt = 0; attempt < o.ncuts; ++attempt) {
+        uint64_t seed = o.seed;
+        if (seed == 0) {
+            std::random_device rd;
+            seed = ((uint64_t)rd() << 32) ^ rd();
+        } else {
+            seed = o.seed + (uint64_t)attempt;                          // kway.rs:55-61
+        }
+        Rng rng(seed);
+        const Arena::Mark mark = cx.arena.mark();
+        struct Level {
+            DevGraph g;
+            int32_t* cmap;
+        };
+        std::vector<Level> levels;
+        std::vector<LevelStat> lst;
+        const DevGraph* cur = &g;
+        for (;;) {                                                      // coarsen/mod.rs:49-69
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (st) {
+                RK_HIP(hipEventCreate(&e0));
+                RK_HIP(hipEventCreate(&e1));
+                RK_HIP(hipEventRecord(e0, s));
+            }
+            {
+                ProfScope ps(kProfMatch, 0.0, cur->n, s);
+                match_graph(*cur, o.ctype_shem, rng.next_u64(), kMatchRounds, B.cb, s);
+            }
+            int32_t* cmap = cx.arena.get<int32_t>(cur->n);
+            DevGraph cg;
+            int32_t cn = 0;
+            {
+                ProfScope ps(kProfContract, 0.0, cur->n, s);
+                cn = contract_numbering(*cur, B.cb.match, cmap, B.cb, s);
+                contract_build(*cur, B.cb.match, cmap, cn, cg, false, B.cb, alloc, nullptr, s);
+            }
+            if (st) {
+                LevelStat ls;
+                ls.n = cur->n;
+                ls.m = cur->m;
+                RK_HIP(hipEventRecord(e1, s));
+                RK_HIP(hipEventSynchronize(e1));
+                RK_HIP(hipEventElapsedTime(&ls.coarsen_ms, e0, e1));
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                lst.push_back(ls);
+            }
+            const float ratio = (float)cn / (float)cur->n;              // coarsen/mod.rs:61
+            levels.push_back({cg, cmap});
+            if (cn <= limit || ratio > 0.95f) break;                    // coarsen/mod.rs:65
+            cur = &levels.back().g;
+        }
+        const DevGraph& cg = levels.back().g;
+        if (cg.n < k) throw Error(-4, "coarsest graph has fewer vertices than nparts");
+        // initial partition of the coarsest level on the host (kway.rs:66-70)
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<int32_t> hx(cg.n + 1), ha(std::max(cg.m, 1)), hv, hp(cg.n);
+        RK_HIP(hipMemcpyAsync(hx.data(), cg.xadj, sizeof(int32_t) * (cg.n + 1), hipMemcpyDeviceToHost, s));
+        RK_HIP(hipMemcpyAsync(ha.data(), cg.adjncy, sizeof(int32_t) * cg.m, hipMemcpyDeviceToHost, s));
+        if (cg.vwgt) {
+            hv.resize((size_t)cg.n * ncon);
+            RK_HIP(hipMemcpyAsync(hv.data(), cg.vwgt, sizeof(int32_t) * hv.size(), hipMemcpyDeviceToHost, s));
+        }
+        RK_HIP(hipStreamSynchronize(s));
+        for (int c = 0; c < ncon; ++c) ltot[c] = cg.vwgt ? total[c] : cg.n;
+        std::vector<double> tgt(kc);
+        for (int32_t p = 0; p < k; ++p)
+            for (int32_t c = 0; c < ncon; ++c)
+                tgt[(size_t)p * ncon + c] =
+                    (tpwgts ? (double)tpwgts[(size_t)p * ncon + c] : 1.0 / (double)k) * (double)ltot[c];
+        if (o.iptype == 1) {
+            for (int32_t v = 0; v < cg.n; ++v) hp[v] = (int32_t)rng.below((uint64_t)k);   // initial/random.rs:8-10
+        } else if (!grow_host(cg.n, ncon, hx.data(), ha.data(), cg.vwgt ? hv.data() : nullptr, k, tgt.data(), rng,
+                              hp.data())) {
+            throw Error(-4, "coarsest graph has fewer vertices than nparts");
+        }
+        int32_t* pc = cx.arena.get<int32_t>(cg.n);
+        RK_HIP(hipMemcpyAsync(pc, hp.data(), sizeof(int32_t) * cg.n, hipMemcpyHostToDevice, s));
+        RK_HIP(hipStreamSynchronize(s));
+        auto t1 = std::chrono::steady_clock::now();
+        // refine the coarsest level, then project and refine every finer level
+        LevelStat cstat;
+        cstat.n = cg.n;
+        cstat.m = cg.m;
+        level_thresholds(k, ncon, tpwgts, ltot.data(), ub.data(), fmax.data(), omax.data());
+        refine_level(cx, cg, k, fmax.data(), omax.data(), o.niter, pc, B, st ? &cstat : nullptr);
+        for (int64_t i = (int64_t)levels.size() - 1; i >= 0; --i) {     // kway.rs:76-92
+            const DevGraph& fine = i == 0 ? g : levels[i - 1].g;
+            int32_t* pf = cx.arena.get<int32_t>(fine.n);
+            project(fine.n, levels[i].cmap, pc, pf, s);
+            for (int c = 0; c < ncon; ++c) ltot[c] = fine.vwgt ? total[c] : fine.n;
+            level_thresholds(k, ncon, tpwgts, ltot.data(), ub.data(), fmax.data(), omax.data());
+            refine_level(cx, fine, k, fmax.data(), omax.data(), o.niter, pf, B, st ? &lst[i] : nullptr);
+            pc = pf;
+        }
+        edge_cut(g, pc, B.cut_acc, B.cut_out, pick_group(g.n, g.m), s);    // kway.rs:94
+        int32_t obj = 0;
+        RK_HIP(hipMemcpyAsync(&obj, B.cut_out, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        RK_HIP(hipStreamSynchronize(s));
+        if (!have_best || obj < best) {                                 // kway.rs:96-98
+            have_best = true;
+            best = obj;
+            RK_HIP(hipMemcpyAsync(d_part, pc, sizeof(int32_t) * g.n, hipMemcpyDeviceToDevice, s));
+        }
+        if (st) {
+            lst.push_back(cstat);
+            st->levels = lst;
+            st->coarsest_n = cg.n;
+            st->initial_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+            st->attempts = attempt + 1;
+        }
+        cx.arena.rewind(mark);
+    }
+    RK_HIP(hipStreamSynchronize(s));
+    return best;
+}
+
+}  // namespace rk

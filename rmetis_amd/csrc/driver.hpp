@@ -1,0 +1,84 @@
+// driver.hpp -- multilevel k-way scheduler (host C++) over the gfx950 kernels.
+#pragma once
+#include <vector>
+
+#include "context.hpp"
+#include "kernels.hpp"
+#include "rng.hpp"
+
+namespace rk {
+
+// Parsed options (types.rs:52-95).
+struct Options {
+    int ctype_shem = 1;     // CType::Shem
+    int iptype = 0;         // 0 Grow, 1 Random, 4 MetisRb (grows, initial/mod.rs:17)
+    int rtype_greedy = 0;   // RType::Fm is the reference default
+    int objtype = 0;
+    int no2hop = 0;
+    int ncuts = 1;
+    int niter = 10;
+    uint64_t seed = 0;      // 0 = entropy
+    int minconn = 0;
+    int contig = 0;
+    int ufactor = 30;
+    int numbering = 0;
+    int dbglvl = 0;
+};
+
+constexpr int kNOptions = 40;  // types.rs:170
+
+// c_api.rs:211-222 + types.rs:190-224 (NULL or options[0] == -1 => defaults).
+Options parse_options(const int32_t* options);
+// types.rs:227-245 applied to Options::default() (index 0 stays -1).
+void default_options_raw(int32_t* out);
+
+// floor(target*ub) thresholds of one level (graph/mod.rs:392-410, kway.rs:44-49).
+void level_thresholds(int32_t k, int32_t ncon, const float* tpwgts, const int32_t* total, const double* ub,
+                      long long* fmax, long long* omax);
+void thresholds_from_targets(int32_t k, int32_t ncon, const double* tpw, const double* ub, long long* fmax,
+                             long long* omax);
+
+struct LevelStat {
+    int32_t n = 0, m = 0;
+    float coarsen_ms = 0.f;    // matching + contraction producing the next level
+    float refine_ms = 0.f;     // all greedy passes of this level
+    int32_t passes = 0;        // greedy passes executed (repair + cut rounds)
+};
+
+struct KwayStats {
+    std::vector<LevelStat> levels;   // finest first (last attempt)
+    int32_t coarsest_n = 0;
+    float initial_ms = 0.f;
+    int32_t attempts = 0;
+};
+
+struct Buffers {
+    RefineBufs rb;
+    CoarsenBufs cb;
+    unsigned long long* cut_acc = nullptr;
+    int32_t* cut_out = nullptr;
+    long long* wsum = nullptr;
+};
+
+// Allocates the per-call work buffers for a finest level of n vertices,
+// m entries, k parts, ncon constraints.
+Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon);
+
+// Column sums of vwgt (or n per constraint when vwgt is absent), checked
+// against the int32 range the reference's sums live in.
+std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffers& B);
+
+// One level's greedy refinement (greedy.rs:36-193) on the device partition.
+void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* h_fmax, const long long* h_omax,
+                  int niter, int32_t* d_part, Buffers& B, LevelStat* st);
+
+// Full k-way partition of a device-resident graph (partition/kway.rs:20-141);
+// writes d_part and returns objval.
+int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwgts, const float* ubvec,
+                    const Options& o, int32_t* d_part, KwayStats* st);
+
+// initial/grow.rs:12-103 on the host; false when n < k.
+bool grow_host(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
+               int32_t k, const double* tpw, Rng& rng, int32_t* part);
+
+}  // namespace rk

@@ -1,0 +1,103 @@
+// kernels.hpp -- host-side launchers of the gfx950 kernels.
+#pragma once
+#include "common.hpp"
+
+namespace rk {
+
+// ---- scan.hip ----
+size_t scan_temp_bytes(int64_t n);
+// out[i] = sum(in[0..i)); *total_dev (device) = sum(in) as int32.
+void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* total_dev, void* temp,
+                        hipStream_t s);
+
+// ---- reduce.hip ----
+// Lanes per vertex for a level with n vertices and m directed entries.
+int pick_group(int32_t n, int32_t m);
+void edge_cut(const DevGraph& g, const int32_t* part, unsigned long long* acc, int32_t* cut_out, int G,
+              hipStream_t s);
+void part_weights(const DevGraph& g, const int32_t* part, int32_t k, int32_t* pw, hipStream_t s);
+void project(int32_t n_fine, const int32_t* cmap, const int32_t* part_c, int32_t* part_f, hipStream_t s);
+// out[c] = sum over v of vwgt[v*ncon+c] as int64 (g.vwgt must be present).
+void column_sums(const DevGraph& g, long long* out, hipStream_t s);
+
+// ---- refine.hip ----
+constexpr int kHeavyGlobalBlocks = 256;
+
+// Work buffers of the greedy rounds, sized for the finest level of a call.
+struct RefineBufs {
+    int4* tile_rec = nullptr;      // [ntiles * kTile]
+    int32_t* tile_cnt = nullptr;   // [ntiles]
+    int32_t* tile_off = nullptr;   // [ntiles]
+    int4* mov = nullptr;           // [n] dense candidate records, vertex order
+    int32_t* pool = nullptr;       // [m] candidate parts
+    int32_t* heavy_list = nullptr; // [n]
+    int4* heavy_rec = nullptr;     // [n]
+    int32_t* heavy_scratch = nullptr;  // [kHeavyGlobalBlocks * (3k+2)] when k > LDS capacity
+    int2* moves = nullptr;         // [n] admitted (v, p) in admission order
+    int32_t* pw = nullptr;         // [k*ncon]
+    long long* fmax = nullptr;     // [k*ncon] feasibility thresholds floor(tpw*ub)
+    long long* omax = nullptr;     // [k*ncon] overweight thresholds floor(tpw*ub)
+    int32_t* gA = nullptr;         // [k*ncon] admission flows (when not in LDS)
+    int32_t* gR = nullptr;
+    int32_t* heavy_cnt = nullptr;  // device counters
+    int32_t* nmov = nullptr;
+    int32_t* nmoves = nullptr;
+    int32_t* done = nullptr;
+    int32_t* passes = nullptr;     // count of executed passes (statistics)
+    void* scan_tmp = nullptr;
+};
+
+// Builds the heavy-vertex list (degree > 4G) of a level; returns its size.
+int refine_setup_level(const DevGraph& g, int G, RefineBufs& b, hipStream_t s);
+// One greedy pass: mode 0 balance repair, mode 1 cut optimisation.  Reads
+// b.pw (running weights) and updates it, part, b.moves/b.nmoves; a
+// cut-optimisation pass with no move sets *b.done, which turns every later
+// pass of the level into a no-op.
+void refine_pass(const DevGraph& g, int32_t k, int mode, int G, int nheavy, RefineBufs& b, int32_t* part,
+                 hipStream_t s);
+// Gain table of every vertex (repair-mode candidate lists with every part
+// treated as overweight): per-vertex record (v or -1, pv, ncand, nbefore |
+// zero-group flag << 31), gain of the implicit zero group, and candidate
+// parts / gains in admission order at xadj[v].
+void gain_table(const DevGraph& g, int32_t k, int G, int nheavy, RefineBufs& b, const int32_t* part, int4* vrec,
+                int32_t* g0, int32_t* gains, hipStream_t s);
+
+// ---- coarsen.hip ----
+struct CoarsenBufs {
+    int32_t* match = nullptr;      // [n]
+    int32_t* cand = nullptr;       // [n]
+    int32_t* leader = nullptr;     // [n] flags, reused as deg_sum
+    int32_t* cid = nullptr;        // [n] coarse id of leaders (scan)
+    int32_t* members = nullptr;    // [2n] (a, b) per coarse vertex
+    int32_t* degsum = nullptr;     // [n]
+    int32_t* tmp_off = nullptr;    // [n]
+    int32_t* tmp_adj = nullptr;    // [m]
+    int32_t* tmp_w = nullptr;      // [m]
+    int32_t* cdeg = nullptr;       // [n]
+    int32_t* big_list = nullptr;   // [n] coarse vertices handled by the global-hash path
+    int32_t* hash_keys = nullptr;  // [2m] global hash tables of the big path
+    int32_t* hash_first = nullptr;
+    int32_t* hash_sum = nullptr;
+    int32_t* counters = nullptr;   // [16]: 0 cn, 2 nbig, 4 weight-not-one flag, 6-7 hash bump (u64)
+    int32_t* round_flags = nullptr;  // [kMaxMatchRounds] "round r changed the matching"
+    void* scan_tmp = nullptr;
+};
+
+constexpr int kMaxMatchRounds = 64;
+
+// Deterministic seeded handshake matching (ctype 1 heavy edge, 0 random).
+void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, CoarsenBufs& b, hipStream_t s);
+// Coarse numbering from a symmetric matching; returns cn (synchronises).
+int32_t contract_numbering(const DevGraph& g, const int32_t* match, int32_t* cmap, CoarsenBufs& b,
+                           hipStream_t s);
+// Builds the coarse graph into cg (arrays preallocated with cn / cm capacity
+// by the caller through alloc callback).  Returns cm (synchronises).
+struct CoarseAlloc {
+    virtual int32_t* alloc_i32(int64_t count) = 0;
+    virtual ~CoarseAlloc() = default;
+};
+int32_t contract_build(const DevGraph& g, const int32_t* match, const int32_t* cmap, int32_t cn, DevGraph& cg,
+                       bool aggregate_vsize, CoarsenBufs& b, CoarseAlloc& al, int* adjwgt_not_one,
+                       hipStream_t s);
+
+}  // namespace rk

@@ -1,0 +1,109 @@
+// scan.hip -- device-wide exclusive prefix sum of int32 (reduce-then-scan).
+//
+// Used for the contraction's coarse numbering (ascending-leader order of
+// graph/mod.rs:288-297), coarse CSR offsets and the ordered compaction of
+// greedy move candidates.  Three launches: per-chunk sums, a single-block scan
+// of the chunk sums, and a per-chunk rescan that adds the chunk offset.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace rk {
+
+namespace {
+
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = kBlock * kScanItems;   // 2048 elements per block
+
+__device__ inline long long block_excl_scan_ll(long long x, long long* s_w, long long* total) {
+    const int lane = __lane_id();
+    const int wave = threadIdx.x >> 6;
+    long long inc = wave_incl_scan(x);
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    long long off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        long long c = s_w[w];
+        if (w < wave) off += c;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const int32_t* __restrict__ in, int64_t n,
+                                                        long long* __restrict__ chunk_sums) {
+    __shared__ long long s_w[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    long long sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        int64_t idx = base + (int64_t)i * kBlock + threadIdx.x;
+        if (idx < n) sum += in[idx];
+    }
+    long long tot;
+    block_excl_scan_ll(sum, s_w, &tot);
+    if (threadIdx.x == 0) chunk_sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_chunks(long long* __restrict__ chunk_sums, int nchunks,
+                                                        int32_t* __restrict__ total_out) {
+    __shared__ long long s_w[kBlock / 64];
+    long long carry = 0;
+    for (int b = 0; b < nchunks; b += kBlock) {
+        int idx = b + threadIdx.x;
+        long long x = idx < nchunks ? chunk_sums[idx] : 0;
+        long long tot;
+        long long ex = block_excl_scan_ll(x, s_w, &tot);
+        if (idx < nchunks) chunk_sums[idx] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = (int32_t)carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_down(const int32_t* __restrict__ in, int64_t n,
+                                                      const long long* __restrict__ chunk_off,
+                                                      int32_t* __restrict__ out) {
+    __shared__ long long s_w[kBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    long long sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        int64_t idx = base + i;
+        v[i] = idx < n ? in[idx] : 0;
+        sum += v[i];
+    }
+    long long tot;
+    long long run = block_excl_scan_ll(sum, s_w, &tot) + chunk_off[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        int64_t idx = base + i;
+        if (idx < n) out[idx] = (int32_t)run;
+        run += v[i];
+    }
+}
+
+}  // namespace
+
+size_t scan_temp_bytes(int64_t n) {
+    int64_t nchunks = (n + kScanChunk - 1) / kScanChunk;
+    return (size_t)(nchunks + 1) * sizeof(long long);
+}
+
+void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* total_dev, void* temp,
+                        hipStream_t s) {
+    if (n <= 0) {
+        if (total_dev) RK_HIP(hipMemsetAsync(total_dev, 0, sizeof(int32_t), s));
+        return;
+    }
+    int nchunks = (int)((n + kScanChunk - 1) / kScanChunk);
+    long long* sums = (long long*)temp;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nchunks), dim3(kBlock), 0, s, in, n, sums);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(kBlock), 0, s, sums, nchunks, total_dev);
+    hipLaunchKernelGGL(k_scan_down, dim3(nchunks), dim3(kBlock), 0, s, in, n, sums, out);
+    RK_HIP(hipGetLastError());
+}
+
+}  // namespace rk

@@ -1,0 +1,314 @@
+"""GPU parity: every hot-path function of librmetis_hip.so against the oracle
+(a CPU restatement of javagg/rmetis) on the same seeded inputs.  Integer
+results must be bit-exact."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+import rmetis_amd as R
+from graphs import cycle4, grid_parallel_order, grid_sorted, path, random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph(xadj, adjncy, vwgt=None, adjwgt=None, ncon=1, vsize=None):
+    return R.Graph(len(xadj) - 1, ncon, xadj, adjncy, vwgt, adjwgt, vsize)
+
+
+def _cases():
+    """(name, xadj, adjncy, adjwgt) small/medium graphs."""
+    out = [("cycle4",) + cycle4() + (None,), ("path9",) + path(9) + (None,),
+           ("grid16x16",) + grid_sorted(16, 16) + (None,),
+           ("gridpar12x7",) + grid_parallel_order(12, 7) + (None,)]
+    for i, (n, p, w) in enumerate([(60, 0.1, 1), (300, 0.03, 10), (2000, 0.004, 5), (1500, 0.02, 3)]):
+        x, a, aw = random_graph(n, p, 100 + i, wmax=w)
+        out.append((f"er{n}", x, a, aw))
+    return out
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("k", [2, 5, 64])
+def test_edge_cut(case, k):
+    _, x, a, aw = case
+    n = len(x) - 1
+    rng = np.random.default_rng(k)
+    for _ in range(3):
+        part = rng.integers(0, k, n).astype(np.int32)
+        g = _graph(x, a, adjwgt=aw)
+        assert g.edge_cut(part) == O.edge_cut(x, a, aw, part)
+
+
+def test_edge_cut_reference_values():
+    x, a = cycle4()
+    g = _graph(x, a)
+    assert g.edge_cut([0, 0, 1, 1]) == 2          # graph/mod.rs:455-461
+    # 256x256 grid split into k horizontal bands: cut = (k-1)*cols
+    from rmetis_amd import synth
+    s = synth.grid(256, 256)
+    g = R.Graph(s.n, 1, s.xadj, s.adjncy)
+    for k in (2, 8, 64):
+        part = (np.arange(s.n) // 256 * k // 256).astype(np.int32)
+        assert g.edge_cut(part) == (k - 1) * 256
+
+
+@pytest.mark.parametrize("ncon", [1, 2, 3])
+@pytest.mark.parametrize("k", [2, 7, 300, 9000])
+def test_partition_weights(ncon, k):
+    rng = np.random.default_rng(ncon * 1000 + k)
+    n = 5000
+    x, a = path(n)
+    part = rng.integers(0, k, n).astype(np.int32)
+    vw = rng.integers(1, 9, n * ncon).astype(np.int32)
+    g = _graph(x, a, vwgt=vw, ncon=ncon)
+    assert np.array_equal(g.partition_weights(part, k), O.partition_weights(ncon, vw, part, k))
+    g1 = _graph(x, a, ncon=ncon)
+    assert np.array_equal(g1.partition_weights(part, k), O.partition_weights(ncon, None, part, k))
+
+
+def _check_gain_table(x, a, aw, part, k):
+    g = _graph(x, a, adjwgt=aw)
+    rec, g0, cp, cg = g.gain_table(part, k)
+    n = len(x) - 1
+    bnd = O.boundary_vertices(x, a, part)
+    assert np.array_equal(np.nonzero(rec[:, 0] >= 0)[0], bnd)
+    for v in bnd:
+        pv = part[v]
+        nc = rec[v, 2]
+        nb = rec[v, 3] & 0x7fffffff
+        lst = [(int(cp[x[v] + i]), int(cg[x[v] + i])) for i in range(nc)]
+        expl = {p for p, _ in lst}
+        for p in range(k):
+            if p == pv:
+                continue
+            og = O.move_gain(x, a, aw, v, part, pv, p)
+            got = dict(lst).get(p, g0[v])
+            assert got == og, (v, p)
+            if p not in expl:
+                assert og == g0[v]
+        # admission order: gain descending, ties to the lowest part
+        assert lst == sorted(lst, key=lambda t: (-t[1], t[0]))
+        assert nb == sum(1 for _, gg in lst if gg > g0[v])
+    return rec
+
+
+@pytest.mark.parametrize("case", CASES[:6], ids=[c[0] for c in CASES[:6]])
+@pytest.mark.parametrize("k", [2, 4, 9])
+def test_gain_table(case, k):
+    _, x, a, aw = case
+    n = len(x) - 1
+    rng = np.random.default_rng(7 * k)
+    part = rng.integers(0, k, n).astype(np.int32)
+    _check_gain_table(x, a, aw, part, k)
+
+
+def test_gain_table_heavy_vertices():
+    # a star hub (degree 900) plus a dense block: exercises the block-per-vertex path
+    n = 1000
+    edges = [(0, i) for i in range(1, 901)] + [(i, i + 1) for i in range(901, 999)]
+    from graphs import build_undirected
+    x, a = build_undirected(n, edges)
+    rng = np.random.default_rng(3)
+    aw_u = rng.integers(1, 6, len(a) // 2)
+    # symmetric weights
+    src = np.repeat(np.arange(n), np.diff(x))
+    key = np.minimum(src, a).astype(np.int64) * n + np.maximum(src, a)
+    _, inv = np.unique(key, return_inverse=True)
+    aw = aw_u[inv].astype(np.int32)
+    for k in (3, 40, 700):
+        part = rng.integers(0, k, n).astype(np.int32)
+        _check_gain_table(x, a, aw, part, k)
+
+
+def _balanced_targets(n_total, k, ncon, ub):
+    tpw = np.full(k * ncon, n_total / k, np.float64)
+    return tpw, np.full(ncon, ub, np.float64)
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("k,ub,skew", [(2, 1.03, 0.0), (4, 1.001, 0.6), (8, 1.05, 0.3), (16, 1.0, 0.9)])
+def test_greedy_pass(case, k, ub, skew):
+    _, x, a, aw = case
+    n = len(x) - 1
+    rng = np.random.default_rng(k * 31 + n)
+    # skewed starting partition so that the repair pass has work
+    part = rng.integers(0, k, n).astype(np.int32)
+    part[rng.random(n) < skew] = 0
+    vw = rng.integers(1, 4, n).astype(np.int32)
+    g = _graph(x, a, vwgt=vw, adjwgt=aw)
+    tpw, ubv = _balanced_targets(vw.sum(), k, 1, ub)
+    pw = O.partition_weights(1, vw, part, k)
+    for mode in (0, 1, 1, 1):
+        p_o, pw_o, mv_o = O.greedy_pass(1, x, a, vw, aw, part, k, tpw, ubv, mode, pw)
+        p_g, pw_g, mv_g = R.greedy_pass(g, part, k, tpw, ubv, mode, pw)
+        assert np.array_equal(mv_g, mv_o), mode
+        assert np.array_equal(p_g, p_o)
+        assert np.array_equal(pw_g, pw_o)
+        part, pw = p_o, pw_o
+
+
+@pytest.mark.parametrize("ncon", [1, 2])
+@pytest.mark.parametrize("k", [3, 8, 33])
+def test_greedy_refine(ncon, k):
+    x, a, aw = random_graph(3000, 0.003, 77 + k, wmax=7)
+    n = len(x) - 1
+    rng = np.random.default_rng(k)
+    vw = rng.integers(1, 5, n * ncon).astype(np.int32)
+    part = rng.integers(0, k, n).astype(np.int32)
+    part[: n // 3] = 0
+    tot = vw.reshape(n, ncon).sum(0)
+    tpw = np.tile(tot / k, k).astype(np.float64)
+    ub = np.full(ncon, 1.03)
+    g = _graph(x, a, vwgt=vw, adjwgt=aw, ncon=ncon)
+    p_o, cut_o = O.greedy_refine(ncon, x, a, vw, aw, part, k, tpw, ub, 10)
+    p_g, cut_g = R.greedy_refine(g, part, k, tpw, ub, 10)
+    assert cut_g == cut_o
+    assert np.array_equal(p_g, p_o)
+
+
+def test_greedy_reference_cases():
+    # tests/parallel.rs:91-116: cycle4 with tpwgts [2,2], ub 1.03
+    x, a = cycle4()
+    g = _graph(x, a)
+    for init, cut in (([0, 0, 1, 1], 2), ([0, 1, 0, 1], 4), ([0, 1, 1, 0], 2)):
+        p, c = R.greedy_refine(g, init, 2, [2.0, 2.0], [1.03], 10)
+        assert c == cut and list(p) == init
+    # tests/parallel.rs:118-132: grid4x4 row stripes, ub 1.05 -> no cut increase
+    x, a = grid_parallel_order(4, 4)
+    g = _graph(x, a)
+    part = np.arange(16) // 4
+    before = g.edge_cut(part)
+    p, c = R.greedy_refine(g, part, 4, [4.0] * 4, [1.05], 10)
+    assert c <= before
+    p_o, c_o = O.greedy_refine(1, x, a, None, None, part, 4, [4.0] * 4, [1.05], 10)
+    assert c == c_o and np.array_equal(p, p_o)
+
+
+def _coarse_equal(x, a, vw, aw, vs, ncon, match):
+    g = _graph(x, a, vwgt=vw, adjwgt=aw, ncon=ncon, vsize=vs)
+    cg, cmap = g.build_coarse_graph(match)
+    o = O.build_coarse_graph(ncon, x, a, vw, aw, vs, match)
+    assert cg.nvtxs == o["n"]
+    assert np.array_equal(cmap, o["cmap"])
+    assert np.array_equal(cg.xadj, o["cxadj"])
+    assert np.array_equal(cg.adjncy, o["cadjncy"])
+    cw = cg.adjwgt if cg.adjwgt is not None else np.ones(len(cg.adjncy), np.int32)
+    assert np.array_equal(cw, o["cadjwgt"])
+    assert (cg.adjwgt is not None) == o["adjwgt_present"]
+    assert (cg.vwgt is None) == (o["cvwgt"] is None)
+    if cg.vwgt is not None:
+        assert np.array_equal(cg.vwgt, o["cvwgt"])
+    assert (cg.vsize is None) == (o["cvsize"] is None)
+    if cg.vsize is not None:
+        assert np.array_equal(cg.vsize, o["cvsize"])
+
+
+def test_coarse_reference_path():
+    # graph/mod.rs:499-514: path 0-1-2-3 matched (0,1),(2,3)
+    x, a = path(4)
+    g = _graph(x, a)
+    cg, cmap = g.build_coarse_graph([1, 0, 3, 2])
+    assert cg.nvtxs == 2 and cg.nedges == 1
+    assert list(cmap) == [0, 0, 1, 1]
+    assert list(cg.xadj) == [0, 1, 2] and list(cg.adjncy) == [1, 0]
+    assert cg.adjwgt is None and cg.vwgt is None
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("ncon,weighted", [(1, False), (1, True), (2, True)])
+def test_build_coarse_graph(case, ncon, weighted):
+    _, x, a, aw = case
+    n = len(x) - 1
+    rng = np.random.default_rng(n + ncon)
+    vw = rng.integers(1, 6, n * ncon).astype(np.int32) if weighted else None
+    vs = rng.integers(1, 4, n).astype(np.int32) if weighted else None
+    match = O.shem_matching(x, a, aw, O.rng(n))
+    _coarse_equal(x, a, vw, aw, vs, ncon, match)
+    match = O.random_matching(x, a, O.rng(n + 1))
+    _coarse_equal(x, a, vw, aw, vs, ncon, match)
+
+
+def test_build_coarse_graph_hub():
+    # degree sums far above the per-group LDS capacity -> global-hash path
+    n = 3000
+    from graphs import build_undirected
+    edges = [(0, i) for i in range(1, 2500)] + [(1, i) for i in range(2, 1800)] + \
+            [(i, i + 1) for i in range(2500, 2999)]
+    x, a = build_undirected(n, edges)
+    for seed in (1, 2):
+        match = O.shem_matching(x, a, None, O.rng(seed))
+        _coarse_equal(x, a, None, None, None, 1, match)
+
+
+def test_gpu_matching_properties():
+    from rmetis_amd import synth
+    s = synth.rgg(1 << 14, 5)
+    g = R.Graph(s.n, 1, s.xadj, s.adjncy, s.vwgt, s.adjwgt)
+    m1 = R.gpu_matching(g, 99)
+    m2 = R.gpu_matching(g, 99)
+    assert np.array_equal(m1, m2)                      # deterministic
+    assert np.array_equal(m1[m1], np.arange(s.n))      # symmetric (shem.rs:96-99)
+    src = np.repeat(np.arange(s.n), np.diff(s.xadj))
+    pairs = m1 != np.arange(s.n)
+    # matched partners are neighbours
+    nbr = set(zip(src.tolist(), s.adjncy.tolist()))
+    assert all((v, int(m1[v])) in nbr for v in np.nonzero(pairs)[0])
+    assert pairs.mean() > 0.7
+    m3 = R.gpu_matching(g, 100)
+    assert not np.array_equal(m1, m3)                  # seeded
+
+
+def test_kway_reference_cases():
+    # tests/correctness.rs:119-128 (greedy, ncuts 5, balanced)
+    x, a = grid_sorted(4, 4)
+    g = _graph(x, a)
+    opts = R.Options.for_kway(seed=1, ncuts=5, rtype=R.RType.Greedy)
+    r = R.part_graph_kway(g, 4, options=opts)
+    assert r.part.min() >= 0 and r.part.max() < 4
+    pw = g.partition_weights(r.part, 4)
+    assert (pw[:, 0] <= 16 * 1.03 / 4 + 1).all()
+    assert r.objval == g.edge_cut(r.part)
+    # tests/correctness.rs:100-108 with greedy refinement
+    x, a = cycle4()
+    g = _graph(x, a)
+    r = R.part_graph_kway(g, 2, options=R.Options.for_kway(seed=1, rtype=R.RType.Greedy))
+    assert r.objval <= 4
+    # determinism (tests/correctness.rs:179-187)
+    x, a = grid_sorted(4, 4)
+    g = _graph(x, a)
+    o = R.Options.for_kway(seed=123, rtype=R.RType.Greedy)
+    r1, r2 = R.part_graph_kway(g, 4, options=o), R.part_graph_kway(g, 4, options=o)
+    assert r1.objval == r2.objval and np.array_equal(r1.part, r2.part)
+    # tests/parallel.rs:163-187: objval == edge_cut, every part non-empty
+    x, a = grid_parallel_order(4, 4)
+    g = _graph(x, a)
+    r = R.part_graph_kway(g, 4, options=R.Options.for_kway(seed=1, ncuts=3, rtype=R.RType.Greedy))
+    assert r.objval == g.edge_cut(r.part)
+    assert (np.bincount(r.part, minlength=4) > 0).all()
+    # tests/parallel.rs:202-215: more attempts never worse
+    x, a = grid_parallel_order(6, 6)
+    g = _graph(x, a)
+    r1 = R.part_graph_kway(g, 4, options=R.Options.for_kway(seed=42, ncuts=1, rtype=R.RType.Greedy))
+    r4 = R.part_graph_kway(g, 4, options=R.Options.for_kway(seed=42, ncuts=4, rtype=R.RType.Greedy))
+    assert r4.objval <= r1.objval
+
+
+def test_kway_errors():
+    x, a = cycle4()
+    g = _graph(x, a)
+    with pytest.raises(R.InputError):
+        R.part_graph_kway(g, 10, options=R.Options(rtype=R.RType.Greedy))
+    with pytest.raises(R.InputError):
+        R.part_graph_kway(g, 1, options=R.Options(rtype=R.RType.Greedy))
+    with pytest.raises(R.InternalError):
+        R.part_graph_kway(g, 2, options=R.Options())   # FM is not provided
+
+
+def test_kway_weighted():
+    # tests/correctness.rs:221-235 with greedy refinement
+    g = R.Graph(4, 1, [0, 2, 4, 6, 8], [1, 3, 0, 2, 1, 3, 0, 2], [2, 1, 2, 1], [3, 1, 3, 1, 3, 1, 3, 1])
+    r = R.part_graph_kway(g, 2, options=R.Options.for_kway(seed=5, rtype=R.RType.Greedy))
+    assert r.part.min() >= 0 and r.part.max() < 2
+    assert r.objval == g.edge_cut(r.part)
