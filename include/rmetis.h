@@ -147,6 +147,13 @@ int32_t rmetis_hip_validate_csr(idx_t nvtxs, idx_t ncon, const idx_t* xadj, int6
                                 const idx_t* adjncy, int64_t adjncy_len, const idx_t* vwgt, int64_t vwgt_len,
                                 const idx_t* adjwgt, int64_t adjwgt_len);
 
+/* Options parsing exactly as RMETIS_PartGraphKway applies it (parse_options,
+ * src/ffi/c_api.rs:211-222, Options::from_raw, src/types.rs:190-224).  Host
+ * only.  out[13] = ctype (0 Rm, 1 Shem), iptype, rtype (0 Fm, 1 Greedy),
+ * objtype, no2hop, ncuts, niter, seed, minconn, contig, ufactor, numbering,
+ * dbglvl. */
+int32_t rmetis_hip_parse_options(const idx_t* options, int64_t* out);
+
 /* Device-memory plumbing for callers that hold no other HIP runtime handle
  * (no counterpart in the reference): plain hipMalloc/hipFree/hipMemcpy on the
  * current device; kind 1 = host->device, 2 = device->host, 3 = device->device. */

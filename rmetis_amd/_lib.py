@@ -47,6 +47,7 @@ SIGNATURES = {
                                                   i32p, i32p, i32p, i32p, i32p, i32p, i32p, i32p, i32p]),
     "rmetis_hip_validate_csr": (C.c_int32, [C.c_int32, C.c_int32, i32p, C.c_int64, i32p, C.c_int64, i32p,
                                             C.c_int64, i32p, C.c_int64]),
+    "rmetis_hip_parse_options": (C.c_int32, [i32p, C.POINTER(C.c_int64)]),
     "rmetis_hip_device_alloc": (C.c_void_p, [C.c_int64]),
     "rmetis_hip_device_free": (None, [C.c_void_p]),
     "rmetis_hip_memcpy": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]),

@@ -415,6 +415,18 @@ int32_t rmetis_hip_validate_csr(idx_t nvtxs, idx_t ncon, const idx_t* xadj, int6
     });
 }
 
+int32_t rmetis_hip_parse_options(const idx_t* options, int64_t* out) {
+    // c_api.rs:211-222 + types.rs:190-224; out[13] = ctype, iptype, rtype,
+    // objtype, no2hop, ncuts, niter, seed, minconn, contig, ufactor, numbering, dbglvl
+    if (!out) return METIS_ERROR_INPUT;
+    const Options o = parse_options(options);
+    const int64_t v[13] = {o.ctype_shem, o.iptype,  o.rtype_greedy, o.objtype,  o.no2hop,
+                           o.ncuts,      o.niter,   (int64_t)o.seed, o.minconn, o.contig,
+                           o.ufactor,    o.numbering, o.dbglvl};
+    for (int i = 0; i < 13; ++i) out[i] = v[i];
+    return METIS_OK;
+}
+
 void* rmetis_hip_device_alloc(int64_t bytes) {
     void* p = nullptr;
     if (bytes <= 0 || hipMalloc(&p, (size_t)bytes) != hipSuccess) return nullptr;

@@ -1,0 +1,152 @@
+"""CPU-side checks of the product library: it loads, exports every symbol that
+include/rmetis.h declares, and its host-side boundary logic (CSR validation,
+option parsing, argument checks, error codes) follows src/ffi/c_api.rs and
+src/graph/validate.rs.  No compute call reaches a GPU here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import rmetis_amd as R
+from rmetis_amd import _lib, synth
+from graphs import cycle4
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "rmetis.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:RMETIS_|rmetis_hip_)\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    syms = _declared_symbols()
+    assert "RMETIS_PartGraphKway" in syms and "RMETIS_SetDefaultOptions" in syms
+    lib = C.CDLL(_lib.LIB_PATH)
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_lib.SIGNATURES), "ctypes table and header disagree"
+
+
+def test_set_default_options():
+    # c_api.rs:14-23 + types.rs:227-245: all -1, then the defaults; index 0 stays -1
+    o = np.zeros(40, np.int32)
+    assert _lib.lib().RMETIS_SetDefaultOptions(o.ctypes.data_as(_lib.i32p)) == 1
+    exp = np.full(40, -1, np.int32)
+    exp[[2, 3, 4, 1, 9, 7, 6, 8, 10, 11, 16, 17, 5]] = [1, 0, 0, 0, 0, 1, 10, 0, 0, 0, 30, 0, 0]
+    assert np.array_equal(o, exp)
+    assert _lib.lib().RMETIS_SetDefaultOptions(None) == -2
+    assert np.array_equal(R.Options().to_raw(), exp)
+
+
+def _parse(arr):
+    out = (C.c_int64 * 13)()
+    a = None if arr is None else np.ascontiguousarray(arr, np.int32)
+    assert _lib.lib().rmetis_hip_parse_options(None if a is None else a.ctypes.data_as(_lib.i32p), out) == 1
+    return list(out)
+
+
+DEFAULTS = [1, 0, 0, 0, 0, 1, 10, 0, 0, 0, 30, 0, 0]
+
+
+def test_option_parsing_quirks():
+    assert _parse(None) == DEFAULTS
+    a = np.zeros(40, np.int32)
+    _lib.lib().RMETIS_SetDefaultOptions(a.ctypes.data_as(_lib.i32p))
+    a[4] = 1                     # edited but options[0] == -1: still all defaults
+    assert _parse(a) == DEFAULTS
+    a[0] = 0
+    p = _parse(a)
+    assert p[2] == 1             # RTYPE 1 -> Greedy
+    b = np.zeros(40, np.int32)   # all zero: CTYPE 0 -> Rm, RTYPE 0 -> Fm, counts keep defaults
+    assert _parse(b) == [0, 0, 0, 0, 0, 1, 10, 0, 0, 0, 30, 0, 0]
+    b[3] = 1; b[8] = -5; b[16] = 7; b[7] = 3; b[6] = 2; b[9] = 2
+    p = _parse(b)
+    assert p[1] == 1 and p[7] == 0 and p[10] == 7 and p[5] == 3 and p[6] == 2 and p[4] == 1
+    b[3] = 4; b[8] = 99
+    p = _parse(b)
+    assert p[1] == 4 and p[7] == 99
+
+
+def test_validate_csr_reference_cases():
+    # graph/validate.rs:151-188 through Graph.__init__
+    x, a = cycle4()
+    R.Graph(4, 1, x, a)
+    for bad in ([0, 2, 4], [0, 3, 5, 7, 9]):
+        with pytest.raises(R.InputError):
+            R.Graph(4, 1, bad, a)
+    with pytest.raises(R.InputError):   # self loop
+        R.Graph(4, 1, [0, 3, 5, 7, 9], [0, 1, 3, 0, 2, 1, 3, 0, 2])
+    with pytest.raises(R.InputError):   # asymmetric degree counts
+        R.Graph(4, 1, [0, 1, 1, 1, 1], [1])
+    with pytest.raises(R.InputError):   # vwgt length
+        R.Graph(4, 1, x, a, vwgt=[1, 1, 1])
+    with pytest.raises(R.InputError):   # non-positive vertex weight
+        R.Graph(4, 1, x, a, vwgt=[1, 0, 1, 1])
+    with pytest.raises(R.InputError):   # adjwgt length
+        R.Graph(4, 1, x, a, adjwgt=[1, 1])
+    with pytest.raises(R.InputError):
+        R.Graph(4, 0, x, a)
+
+
+def _kway_raw(n, ncon, xadj, adjncy, nparts, part=True):
+    L = _lib.lib()
+    nv, nc, npt = C.c_int32(n), C.c_int32(ncon), C.c_int32(nparts)
+    xa = None if xadj is None else np.ascontiguousarray(xadj, np.int32)
+    aa = None if adjncy is None else np.ascontiguousarray(adjncy, np.int32)
+    out = np.zeros(max(n, 1), np.int32) if part else None
+    obj = C.c_int32(-99)
+    rc = L.RMETIS_PartGraphKway(C.byref(nv), C.byref(nc), None if xa is None else xa.ctypes.data_as(_lib.i32p),
+                                None if aa is None else aa.ctypes.data_as(_lib.i32p), None, None, None,
+                                C.byref(npt), None, None, None, C.byref(obj),
+                                None if out is None else out.ctypes.data_as(_lib.i32p))
+    return rc, obj.value, out
+
+
+def test_c_abi_input_errors():
+    # c_api.rs:157-167 and kway.rs:31-38: errors come back before any GPU work
+    x, a = cycle4()
+    assert _kway_raw(4, 1, None, a, 2)[0] == -2
+    assert _kway_raw(4, 1, x, None, 2)[0] == -2
+    assert _kway_raw(4, 1, x, a, 2, part=False)[0] == -2
+    assert _kway_raw(0, 1, x, a, 2)[0] == -2
+    assert _kway_raw(4, 0, x, a, 2)[0] == -2
+    assert _kway_raw(4, 1, x, a, 1)[0] == -2
+    assert _kway_raw(4, 1, x, a, 10)[0] == -2
+    rc, obj, part = _kway_raw(4, 1, [0, 1, 1, 1, 1], [1], 2)
+    assert rc == -2 and obj == -99 and not part.any()      # outputs untouched on error
+
+
+def test_compute_without_gpu_fails_loudly():
+    if R.device_count() > 0:
+        pytest.skip("a GPU is present")
+    x, a = cycle4()
+    g = R.Graph(4, 1, x, a)
+    with pytest.raises(R.InternalError, match="no HIP device"):
+        g.edge_cut([0, 0, 1, 1])
+    with pytest.raises(R.InternalError):
+        R.part_graph_kway(g, 2, options=R.Options(rtype=R.RType.Greedy))
+    rc, obj, part = _kway_raw(4, 1, x, a, 2)
+    assert rc == -4 and obj == -99
+
+
+def test_generators_shape():
+    g = synth.grid(256, 256)
+    assert g.n == 65536 and g.m == 261120 and g.vwgt is None and g.adjwgt is None
+    R.Graph(g.n, 1, g.xadj, g.adjncy)
+    r = synth.rgg(1 << 12, 3)
+    R.Graph(r.n, 1, r.xadj, r.adjncy, r.vwgt, r.adjwgt)
+    assert 1 <= r.adjwgt.min() and r.adjwgt.max() <= 10 and (r.vwgt == 1).all()
+    src = np.repeat(np.arange(r.n), np.diff(r.xadj))
+    o1 = np.lexsort((r.adjncy, src))
+    o2 = np.lexsort((src, r.adjncy))
+    assert np.array_equal(r.adjwgt[o1], r.adjwgt[o2])     # symmetric weights
+    m = synth.rmat(12, 16, 1)
+    R.Graph(m.n, 1, m.xadj, m.adjncy)
+    b = synth.uneven_batch(6, 1, nmin_log2=8, nmax_log2=10)
+    for gg, k in b:
+        R.Graph(gg.n, 1, gg.xadj, gg.adjncy, gg.vwgt, gg.adjwgt)
+        assert k <= gg.n
