@@ -98,6 +98,18 @@ bool validate_csr(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* a
     return true;
 }
 
+// Per-constraint vertex-weight totals below 2^30, so that running part
+// weights and weight+vw never leave the int32 range in the greedy kernels.
+bool totals_in_range(int32_t n, int32_t ncon, const int32_t* vwgt, std::vector<int64_t>* tot = nullptr) {
+    std::vector<int64_t> t(ncon, 0);
+    for (int64_t v = 0; v < n; ++v)
+        for (int c = 0; c < ncon; ++c) t[c] += vwgt ? vwgt[v * ncon + c] : 1;
+    for (int c = 0; c < ncon; ++c)
+        if (t[c] >= (1ll << 30)) return false;
+    if (tot) *tot = t;
+    return true;
+}
+
 bool valid_part(int32_t n, const int32_t* part, int32_t nparts) {
     for (int32_t v = 0; v < n; ++v)
         if (part[v] < 0 || part[v] >= nparts) return false;
@@ -262,9 +274,9 @@ int32_t rmetis_hip_gain_table(idx_t nvtxs, const idx_t* xadj, const idx_t* adjnc
         int32_t* dg0 = cx->arena.get<int32_t>(nvtxs);
         int32_t* gains = cx->arena.get<int32_t>(g.m);
         RK_HIP(hipMemsetAsync(dg0, 0, sizeof(int32_t) * nvtxs, cx->stream));
-        const int G = pick_group(g.n, g.m);
-        const int nheavy = refine_setup_level(g, G, B.rb, cx->stream);
-        gain_table(g, nparts, G, nheavy, B.rb, dp, vrec, dg0, gains, cx->stream);
+        int32_t* heavy = cx->arena.get<int32_t>(nvtxs);
+        const int nheavy = heavy_list_sync(g, heavy, B.rb, cx->stream);
+        gain_table(g, nparts, nheavy, heavy, B.rb, dp, vrec, dg0, gains, cx->stream);
         download(*cx.c, (int4*)rec, vrec, nvtxs);
         download(*cx.c, g0, dg0, nvtxs);
         download(*cx.c, cand_part, B.rb.pool, g.m);
@@ -284,6 +296,10 @@ int32_t rmetis_hip_greedy_pass(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const
             return METIS_ERROR_INPUT;
         if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt) || !valid_part(nvtxs, part, nparts))
             return METIS_ERROR_INPUT;
+        std::vector<int64_t> tot;
+        if (!totals_in_range(nvtxs, ncon, vwgt, &tot)) return METIS_ERROR_INPUT;
+        for (int64_t i = 0; i < (int64_t)nparts * ncon; ++i)
+            if (pw[i] < 0 || pw[i] > tot[i % ncon]) return METIS_ERROR_INPUT;
         ContextGuard cx;
         hipStream_t s = cx->stream;
         DevGraph g = upload_graph(*cx.c, nvtxs, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
@@ -292,14 +308,14 @@ int32_t rmetis_hip_greedy_pass(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const
         const size_t kc = (size_t)nparts * ncon;
         std::vector<long long> fm(kc), om(kc);
         thresholds_from_targets(nparts, ncon, tpwgts, ubvec, fm.data(), om.data());
-        const int G = pick_group(g.n, g.m);
-        const int nheavy = refine_setup_level(g, G, B.rb, s);
+        int32_t* heavy = cx->arena.get<int32_t>(nvtxs);
+        const int nheavy = heavy_list_sync(g, heavy, B.rb, s);
         RK_HIP(hipMemcpyAsync(B.rb.fmax, fm.data(), kc * sizeof(long long), hipMemcpyHostToDevice, s));
         RK_HIP(hipMemcpyAsync(B.rb.omax, om.data(), kc * sizeof(long long), hipMemcpyHostToDevice, s));
         RK_HIP(hipMemcpyAsync(B.rb.pw, pw, kc * sizeof(int32_t), hipMemcpyHostToDevice, s));
         RK_HIP(hipMemsetAsync(B.rb.done, 0, sizeof(int32_t), s));
         RK_HIP(hipMemsetAsync(B.rb.passes, 0, sizeof(int32_t), s));
-        refine_pass(g, nparts, mode, G, nheavy, B.rb, dp, s);
+        refine_pass(g, nparts, mode, nheavy, heavy, B.rb, dp, s);
         int32_t nm = 0;
         download(*cx.c, &nm, B.rb.nmoves, 1);
         RK_HIP(hipStreamSynchronize(s));
@@ -318,7 +334,8 @@ int32_t rmetis_hip_greedy_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, con
     return guarded([&]() -> int32_t {
         if (!xadj || !adjncy || !part || !tpwgts || !ubvec || nvtxs <= 0 || ncon <= 0 || nparts < 1 || niter < 0)
             return METIS_ERROR_INPUT;
-        if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt) || !valid_part(nvtxs, part, nparts))
+        if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt) || !valid_part(nvtxs, part, nparts) ||
+            !totals_in_range(nvtxs, ncon, vwgt))
             return METIS_ERROR_INPUT;
         ContextGuard cx;
         hipStream_t s = cx->stream;
@@ -328,7 +345,11 @@ int32_t rmetis_hip_greedy_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, con
         const size_t kc = (size_t)nparts * ncon;
         std::vector<long long> fm(kc), om(kc);
         thresholds_from_targets(nparts, ncon, tpwgts, ubvec, fm.data(), om.data());
-        refine_level(*cx.c, g, nparts, fm.data(), om.data(), niter, dp, B, nullptr);
+        int32_t* heavy = cx->arena.get<int32_t>(nvtxs);
+        const int nheavy = heavy_list_sync(g, heavy, B.rb, s);
+        RK_HIP(hipMemcpyAsync(B.rb.fmax, fm.data(), kc * sizeof(long long), hipMemcpyHostToDevice, s));
+        RK_HIP(hipMemcpyAsync(B.rb.omax, om.data(), kc * sizeof(long long), hipMemcpyHostToDevice, s));
+        refine_level(*cx.c, g, nparts, B.rb.fmax, B.rb.omax, niter, dp, nheavy, heavy, B, nullptr);
         edge_cut(g, dp, B.cut_acc, B.cut_out, pick_group(g.n, g.m), s);
         download(*cx.c, part, dp, nvtxs);
         int32_t c = 0;
@@ -378,10 +399,10 @@ int32_t rmetis_hip_build_coarse_graph(idx_t nvtxs, idx_t ncon, const idx_t* xadj
             explicit A(Arena& ar) : a(ar) {}
             int32_t* alloc_i32(int64_t c) override { return a.get<int32_t>(c); }
         } al(cx->arena);
-        const int32_t cn = contract_numbering(g, dm, dcmap, B.cb, s);
         DevGraph cg;
-        int not_one = 0;
-        const int32_t cm = contract_build(g, dm, dcmap, cn, cg, true, B.cb, al, &not_one, s);
+        const ContractResult cr = contract(g, dm, dcmap, cg, true, B.cb, al, nullptr, s);
+        const int32_t cn = cr.cn, cm = cr.cm;
+        const bool not_one = cr.adjwgt_not_one;
         download(*cx.c, cxadj, cg.xadj, (int64_t)cn + 1);
         download(*cx.c, cadjncy, cg.adjncy, cm);
         download(*cx.c, cadjwgt, cg.adjwgt, cm);

@@ -3,8 +3,7 @@
 // Matching (coarsen/shem.rs:13-56, coarsen/rm.rs:13-43).  The reference's
 // matchings are sequential greedy scans.  On the GPU every unmatched vertex
 // proposes to its best unmatched neighbour under a strict total order on
-// edges -- (edge weight, seeded edge hash) for heavy-edge matching, the hash
-// alone for random matching -- and mutual proposals become pairs (a
+// edges (EdgeKey below) and mutual proposals become pairs (a
 // locally-dominant handshake).  Rounds repeat until nothing changes or the
 // round limit; leftovers are self-matched.  The result is symmetric,
 // deterministic for a given seed and independent of scheduling.
@@ -16,10 +15,10 @@
 //    has none (300-314);
 //  * coarse adjacency of {a < b}: first occurrence over adj(a) then adj(b)
 //    in CSR order, internal edges dropped, weights of merged entries summed
-//    (323-341).  Small coarse vertices use a per-group LDS hash that records
-//    each coarse neighbour's first position and weight sum; a second pass
-//    emits first occurrences with a ballot prefix.  Large ones (degree sum
-//    > 8G) use a block per vertex and a hash table in global memory.
+//    (323-341).  Coarse vertices with degree sum <= kContractBig are merged
+//    in LDS tiles (k_contract_tile); larger ones use a block per vertex and
+//    a hash table in global memory that records each coarse neighbour's
+//    first position and weight sum (k_contract_big).
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -120,7 +119,8 @@ __global__ __launch_bounds__(kBlock) void k_match_accept(int32_t n, const int32_
             ch = true;
         }
     }
-    if (__ballot(ch) && __lane_id() == 0) atomicOr(changed, 1);
+    // one plain store per block that changed something (every writer stores 1)
+    if (__syncthreads_or(ch) && threadIdx.x == 0) *changed = 1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_match_finish(int32_t n, int32_t* __restrict__ match) {
@@ -164,94 +164,154 @@ __device__ inline int32_t seq_edge(const int32_t* xadj, int32_t a, int32_t b, in
     return t < dega ? xadj[a] + t : xadj[b] + (t - dega);
 }
 
-__device__ inline uint32_t hslot(int32_t key, uint32_t mask) { return (mix32((uint32_t)key) & mask); }
 
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_contract(int32_t cn, const int32_t* __restrict__ xadj,
-                                                     const int32_t* __restrict__ adjncy,
-                                                     const int32_t* __restrict__ adjwgt,
-                                                     const int32_t* __restrict__ cmap,
-                                                     const int32_t* __restrict__ members,
-                                                     const int32_t* __restrict__ degsum,
-                                                     const int32_t* __restrict__ tmp_off,
-                                                     int32_t* __restrict__ tmp_adj, int32_t* __restrict__ tmp_w,
-                                                     int32_t* __restrict__ cdeg, int32_t* __restrict__ big_list,
-                                                     int32_t* __restrict__ counters) {
-    constexpr int NG = kBlock / G;
-    constexpr int S = 8 * G;
-    __shared__ int32_t s_key[NG * S];
-    __shared__ int32_t s_first[NG * S];
-    __shared__ int32_t s_sum[NG * S];
-    const int grp = threadIdx.x / G;
-    const int gl = group_lane<G>();
-    int32_t* hk = s_key + grp * S;
-    int32_t* hf = s_first + grp * S;
-    int32_t* hs = s_sum + grp * S;
-    for (int s = gl; s < S; s += G) {
-        hk[s] = -1;
-        hf[s] = INT32_MAX;
-        hs[s] = 0;
+constexpr int kCT = 256;                 // coarse vertices (= threads) per contraction tile
+constexpr int kCItems = 16;              // staged positions per thread per batch
+constexpr int kCCap = kCT * kCItems;     // merged-sequence positions staged per batch
+
+__device__ inline int block_excl_scan_c(int x, int* s_w, int* total) {
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    int inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
     }
-    wave_sync();
-    bool not_one = false;
-    for (int64_t cv = (int64_t)blockIdx.x * NG + grp; cv < cn; cv += (int64_t)gridDim.x * NG) {
-        const int32_t ds = degsum[cv];
-        if (ds > S) {
-            if (gl == 0) big_list[atomicAdd(&counters[2], 1)] = (int32_t)cv;
-            continue;
-        }
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kCT / 64; ++w) {
+        const int c = s_w[w];
+        if (w < wave) off += c;
+        tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - x;
+}
+
+// One block per tile of 256 consecutive coarse vertices.  Their merged
+// adjacency sequences adj(a) ++ adj(b) are streamed in batches of kCCap
+// positions (independent coalesced loads of adjncy/adjwgt, then the cmap
+// gathers) into LDS; each thread then merges its own coarse vertex in LDS:
+// internal entries dropped, equal coarse neighbours summed onto their first
+// occurrence, and the survivors written in sequence order.
+__global__ __launch_bounds__(kCT) void k_contract_tile(const int32_t* __restrict__ cn_dev,
+                                                       const int32_t* __restrict__ xadj,
+                                                       const int32_t* __restrict__ adjncy,
+                                                       const int32_t* __restrict__ adjwgt,
+                                                       const int32_t* __restrict__ cmap,
+                                                       const int32_t* __restrict__ members,
+                                                       const int32_t* __restrict__ degsum,
+                                                       const int32_t* __restrict__ tmp_off,
+                                                       int32_t* __restrict__ tmp_adj, int32_t* __restrict__ tmp_w,
+                                                       int32_t* __restrict__ cdeg, int32_t* __restrict__ big_list,
+                                                       int32_t* __restrict__ counters) {
+    __shared__ int32_t s_cu[kCCap];
+    __shared__ int32_t s_w[kCCap];
+    __shared__ int32_t s_loff[kCT + 1];
+    __shared__ int32_t s_xa[kCT], s_xb[kCT], s_da[kCT];
+    __shared__ int32_t s_bstart[kCT + 2];
+    __shared__ int s_tmp[kCT / 64];
+    __shared__ int s_nb;
+    const int t = threadIdx.x;
+    const int32_t cn = *cn_dev;
+    if ((int64_t)blockIdx.x * kCT >= cn) return;   // block-uniform
+    const int64_t cv = (int64_t)blockIdx.x * kCT + t;
+    const bool own = cv < cn;
+    int32_t ds = 0, xa = 0, xb = 0, da = 0;
+    bool big = false;
+    if (own) {
         const int32_t a = members[2 * cv], b = members[2 * cv + 1];
-        const int32_t dega = xadj[a + 1] - xadj[a];
-        for (int32_t t = gl; t < ds; t += G) {
-            const int32_t j = seq_edge(xadj, a, b, dega, t);
-            const int32_t cu = cmap[adjncy[j]];
-            if (cu == (int32_t)cv) continue;
-            const int32_t w = adjwgt ? adjwgt[j] : 1;
-            uint32_t s = hslot(cu, S - 1);
-            for (;;) {
-                const int32_t prev = atomicCAS(&hk[s], -1, cu);
-                if (prev == -1 || prev == cu) {
-                    atomicMin(&hf[s], t);
-                    atomicAdd(&hs[s], w);
-                    break;
+        ds = degsum[cv];
+        xa = xadj[a];
+        da = xadj[a + 1] - xa;
+        xb = b >= 0 ? xadj[b] : 0;
+        big = ds > kContractBig;
+        if (big) big_list[atomicAdd(&counters[2], 1)] = (int32_t)cv;
+    }
+    int total;
+    const int loff = block_excl_scan_c((own && !big) ? ds : 0, s_tmp, &total);
+    s_loff[t] = loff;
+    s_xa[t] = xa;
+    s_xb[t] = xb;
+    s_da[t] = da;
+    if (t == 0) s_loff[kCT] = total;
+    __syncthreads();
+    if (t == 0) {
+        int nb = 0, vs = 0;
+        while (vs < kCT) {
+            s_bstart[nb++] = vs;
+            int lo = vs + 1, hi = kCT;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_loff[mid] - s_loff[vs] <= kCCap) lo = mid;
+                else hi = mid - 1;
+            }
+            vs = lo;
+        }
+        s_bstart[nb] = kCT;
+        s_nb = nb;
+    }
+    __syncthreads();
+    bool not_one = false;
+    const int nb = s_nb;
+    for (int bb = 0; bb < nb; ++bb) {
+        const int vs = s_bstart[bb], ve = s_bstart[bb + 1];
+        const int P0 = s_loff[vs], P1 = s_loff[ve];
+        int32_t uu[kCItems], ww[kCItems];
+#pragma unroll
+        for (int i = 0; i < kCItems; ++i) {
+            const int p = P0 + t + i * kCT;
+            uu[i] = 0;
+            ww[i] = 1;
+            if (p < P1) {
+                int lo = vs, hi = ve - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_loff[mid] <= p) lo = mid;
+                    else hi = mid - 1;
                 }
-                s = (s + 1) & (S - 1);
+                const int32_t tl = p - s_loff[lo];
+                const int32_t j = tl < s_da[lo] ? s_xa[lo] + tl : s_xb[lo] + (tl - s_da[lo]);
+                uu[i] = adjncy[j];
+                if (adjwgt) ww[i] = adjwgt[j];
             }
         }
-        wave_sync();
-        const int32_t off = tmp_off[cv];
-        int32_t carry = 0;
-        for (int32_t t0 = 0; t0 < ds; t0 += G) {
-            const int32_t t = t0 + gl;
-            bool first = false;
-            int32_t cu = -1, sum = 0;
-            if (t < ds) {
-                const int32_t j = seq_edge(xadj, a, b, dega, t);
-                cu = cmap[adjncy[j]];
-                if (cu != (int32_t)cv) {
-                    uint32_t s = hslot(cu, S - 1);
-                    while (hk[s] != cu) s = (s + 1) & (S - 1);
-                    first = hf[s] == t;
-                    sum = hs[s];
-                }
+#pragma unroll
+        for (int i = 0; i < kCItems; ++i) {
+            const int p = P0 + t + i * kCT;
+            if (p < P1) {
+                s_cu[p - P0] = cmap[uu[i]];
+                s_w[p - P0] = ww[i];
             }
-            const uint64_t bm = group_ballot<G>(first);
-            if (first) {
-                const int pos = carry + __popcll(bm & ((1ull << gl) - 1ull));
+        }
+        __syncthreads();
+        if (own && !big && t >= vs && t < ve) {
+            const int lb = loff - P0, le = lb + ds;
+            const int32_t me = (int32_t)cv;
+            for (int e = lb; e < le; ++e)
+                if (s_cu[e] == me) s_cu[e] = -1;                     // internal edge (graph/mod.rs:330-332)
+            const int32_t off = tmp_off[cv];
+            int32_t pos = 0;
+            for (int e = lb; e < le; ++e) {
+                const int32_t cu = s_cu[e];
+                if (cu < 0) continue;
+                uint32_t sum = (uint32_t)s_w[e];
+                for (int e2 = e + 1; e2 < le; ++e2)
+                    if (s_cu[e2] == cu) {                             // merge (graph/mod.rs:335-339)
+                        sum += (uint32_t)s_w[e2];
+                        s_cu[e2] = -1;
+                    }
                 tmp_adj[off + pos] = cu;
-                tmp_w[off + pos] = sum;
-                if (sum != 1) not_one = true;
+                tmp_w[off + pos] = (int32_t)sum;
+                if ((int32_t)sum != 1) not_one = true;
+                ++pos;
             }
-            carry += __popcll(bm);
+            cdeg[cv] = pos;
         }
-        if (gl == 0) cdeg[cv] = carry;
-        wave_sync();
-        for (int s = gl; s < S; s += G) {
-            hk[s] = -1;
-            hf[s] = INT32_MAX;
-            hs[s] = 0;
-        }
-        wave_sync();
+        __syncthreads();
     }
     if (__ballot(not_one) && __lane_id() == 0) atomicOr(&counters[4], 1);
 }
@@ -345,14 +405,15 @@ __global__ __launch_bounds__(kBlock) void k_contract_big(const int32_t* __restri
     if (__ballot(not_one) && __lane_id() == 0) atomicOr(&counters[4], 1);
 }
 
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_contract_copy(int32_t cn, const int32_t* __restrict__ tmp_off,
+__global__ __launch_bounds__(kBlock) void k_contract_copy(int32_t n_max, const int32_t* __restrict__ cn_dev,
+                                                          const int32_t* __restrict__ tmp_off,
                                                           const int32_t* __restrict__ tmp_adj,
                                                           const int32_t* __restrict__ tmp_w,
                                                           const int32_t* __restrict__ cxadj,
                                                           int32_t* __restrict__ cadj, int32_t* __restrict__ cw) {
-    constexpr int NG = kBlock / G;
+    constexpr int G = 16, NG = kBlock / G;
     const int gl = group_lane<G>();
+    const int32_t cn = *cn_dev;
     for (int64_t cv = (int64_t)blockIdx.x * NG + threadIdx.x / G; cv < cn; cv += (int64_t)gridDim.x * NG) {
         const int32_t src = tmp_off[cv];
         const int32_t dst = cxadj[cv];
@@ -404,61 +465,53 @@ void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, C
     RK_HIP(hipGetLastError());
 }
 
-int32_t contract_numbering(const DevGraph& g, const int32_t* match, int32_t* cmap, CoarsenBufs& b,
-                           hipStream_t s) {
-    (void)cmap;
-    const int nb = (g.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_leader, dim3(nb), dim3(kBlock), 0, s, g.n, match, b.leader);
-    exclusive_scan_i32(b.leader, b.cid, g.n, b.counters + 0, b.scan_tmp, s);
-    int32_t cn = 0;
-    RK_HIP(hipMemcpyAsync(&cn, b.counters + 0, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    RK_HIP(hipStreamSynchronize(s));
-    return cn;
-}
-
-int32_t contract_build(const DevGraph& g, const int32_t* match, const int32_t* cmap, int32_t cn, DevGraph& cg,
-                       bool aggregate_vsize, CoarsenBufs& b, CoarseAlloc& al, int* adjwgt_not_one, hipStream_t s) {
-    const int nb = (g.n + kBlock - 1) / kBlock;
-    cg.n = cn;
+ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, DevGraph& cg, bool aggregate_vsize,
+                        CoarsenBufs& b, CoarseAlloc& al, int32_t* heavy_list, hipStream_t s) {
+    // Coarse sizes stay on the device until the single synchronisation at the
+    // end; arrays are allocated with the upper bounds cn <= n, cm <= m and
+    // every kernel reads cn = counters[0].
+    const int32_t n = g.n;
+    const int nb = (n + kBlock - 1) / kBlock;
+    int32_t* cn_dev = b.counters + 0;
     cg.ncon = g.ncon;
-    cg.vwgt = g.vwgt ? al.alloc_i32((int64_t)cn * g.ncon) : nullptr;
-    cg.vsize = (aggregate_vsize && g.vsize) ? al.alloc_i32(cn) : nullptr;
-    cg.xadj = al.alloc_i32((int64_t)cn + 1);
-    hipLaunchKernelGGL(k_cmap, dim3(nb), dim3(kBlock), 0, s, g.n, g.ncon, g.xadj, g.vwgt, g.vsize, match, b.cid,
-                       (int32_t*)cmap, b.members, b.degsum, cg.vwgt, cg.vsize);
-    exclusive_scan_i32(b.degsum, b.tmp_off, cn, nullptr, b.scan_tmp, s);
-    RK_HIP(hipMemsetAsync(b.counters + 2, 0, 6 * sizeof(int32_t), s));
-    const int G = pick_group(g.n, g.m);
-    const int NG = kBlock / G;
-    const int grid = grid_cap(cn, NG, 65536);
-#define RK_CONTRACT(GG)                                                                                     \
-    hipLaunchKernelGGL(k_contract<GG>, dim3(grid), dim3(kBlock), 0, s, cn, g.xadj, g.adjncy, g.adjwgt, cmap, \
-                       b.members, b.degsum, b.tmp_off, b.tmp_adj, b.tmp_w, b.cdeg, b.big_list, b.counters)
-    switch (G) {
-        case 4: RK_CONTRACT(4); break;
-        case 8: RK_CONTRACT(8); break;
-        case 16: RK_CONTRACT(16); break;
-        case 32: RK_CONTRACT(32); break;
-        default: RK_CONTRACT(64); break;
-    }
-#undef RK_CONTRACT
+    cg.vwgt = g.vwgt ? al.alloc_i32((int64_t)n * g.ncon) : nullptr;
+    cg.vsize = (aggregate_vsize && g.vsize) ? al.alloc_i32(n) : nullptr;
+    cg.xadj = al.alloc_i32((int64_t)n + 1);
+    cg.adjncy = al.alloc_i32(g.m > 0 ? g.m : 1);
+    cg.adjwgt = al.alloc_i32(g.m > 0 ? g.m : 1);
+    hipLaunchKernelGGL(k_leader, dim3(nb), dim3(kBlock), 0, s, n, match, b.leader);
+    exclusive_scan_i32(b.leader, b.cid, n, cn_dev, b.scan_tmp, s);      // coarse ids (graph/mod.rs:288-297)
+    RK_HIP(hipMemsetAsync(b.degsum, 0, sizeof(int32_t) * n, s));
+    RK_HIP(hipMemsetAsync(b.cdeg, 0, sizeof(int32_t) * n, s));
+    RK_HIP(hipMemsetAsync(b.counters + 2, 0, 8 * sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_cmap, dim3(nb), dim3(kBlock), 0, s, n, g.ncon, g.xadj, g.vwgt, g.vsize, match, b.cid, cmap,
+                       b.members, b.degsum, cg.vwgt, cg.vsize);
+    exclusive_scan_i32(b.degsum, b.tmp_off, n, nullptr, b.scan_tmp, s);
+    hipLaunchKernelGGL(k_contract_tile, dim3((n + kCT - 1) / kCT), dim3(kCT), 0, s, cn_dev, g.xadj, g.adjncy,
+                       g.adjwgt, cmap, b.members, b.degsum, b.tmp_off, b.tmp_adj, b.tmp_w, b.cdeg, b.big_list,
+                       b.counters);
     hipLaunchKernelGGL(k_contract_big, dim3(1024), dim3(kBlock), 0, s, g.xadj, g.adjncy, g.adjwgt, cmap,
                        b.members, b.degsum, b.tmp_off, b.tmp_adj, b.tmp_w, b.cdeg, b.big_list, b.counters,
                        b.hash_keys, b.hash_first, b.hash_sum);
-    exclusive_scan_i32(b.cdeg, cg.xadj, cn, cg.xadj + cn, b.scan_tmp, s);
-    int32_t hc[2] = {0, 0};
-    RK_HIP(hipMemcpyAsync(hc, cg.xadj + cn, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    RK_HIP(hipMemcpyAsync(hc + 1, b.counters + 4, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    RK_HIP(hipStreamSynchronize(s));
-    const int32_t cm = hc[0];
-    if (adjwgt_not_one) *adjwgt_not_one = hc[1];
-    cg.m = cm;
-    cg.adjncy = al.alloc_i32(cm > 0 ? cm : 1);
-    cg.adjwgt = al.alloc_i32(cm > 0 ? cm : 1);
-    hipLaunchKernelGGL(k_contract_copy<16>, dim3(grid_cap(cn, kBlock / 16, 65536)), dim3(kBlock), 0, s, cn,
+    exclusive_scan_i32(b.cdeg, cg.xadj, n, cg.xadj + n, b.scan_tmp, s);   // cxadj[cn..n] = cm
+    hipLaunchKernelGGL(k_contract_copy, dim3(grid_cap(n, kBlock / 16, 65536)), dim3(kBlock), 0, s, n, cn_dev,
                        b.tmp_off, b.tmp_adj, b.tmp_w, cg.xadj, cg.adjncy, cg.adjwgt);
+    if (heavy_list) heavy_list_build(n, cn_dev, cg.xadj, heavy_list, b.counters + 8, s);
+    int32_t hc[4] = {0, 0, 0, 0};
+    RK_HIP(hipMemcpyAsync(hc + 0, cn_dev, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    RK_HIP(hipMemcpyAsync(hc + 1, cg.xadj + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    RK_HIP(hipMemcpyAsync(hc + 2, b.counters + 4, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    RK_HIP(hipMemcpyAsync(hc + 3, b.counters + 8, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    RK_HIP(hipStreamSynchronize(s));
     RK_HIP(hipGetLastError());
-    return cm;
+    cg.n = hc[0];
+    cg.m = hc[1];
+    ContractResult r;
+    r.cn = hc[0];
+    r.cm = hc[1];
+    r.adjwgt_not_one = hc[2] != 0;
+    r.nheavy = heavy_list ? hc[3] : 0;
+    return r;
 }
 
 }  // namespace rk

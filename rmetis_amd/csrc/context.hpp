@@ -78,18 +78,26 @@ struct Context {
     void* pinned = nullptr;
     size_t pinned_size = 0;
 
-    void* host_stage(size_t bytes) {
-        if (bytes > pinned_size) {
-            if (pinned) (void)hipHostFree(pinned);
-            pinned = nullptr;
+    void* pinned2 = nullptr;
+    size_t pinned2_size = 0;
+
+    // Two independent pinned staging buffers; a buffer may be reused only
+    // after the stream has been synchronised past its last copy.
+    void* host_stage(size_t bytes) { return grow(pinned, pinned_size, bytes); }
+    void* host_stage2(size_t bytes) { return grow(pinned2, pinned2_size, bytes); }
+    static void* grow(void*& p, size_t& cap, size_t bytes) {
+        if (bytes > cap) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
             size_t sz = bytes < (1u << 20) ? (1u << 20) : bytes;
-            RK_HIP(hipHostMalloc(&pinned, sz, hipHostMallocDefault));
-            pinned_size = sz;
+            RK_HIP(hipHostMalloc(&p, sz, hipHostMallocDefault));
+            cap = sz;
         }
-        return pinned;
+        return p;
     }
     ~Context() {
         if (pinned) (void)hipHostFree(pinned);
+        if (pinned2) (void)hipHostFree(pinned2);
         arena.release();
         if (stream) (void)hipStreamDestroy(stream);
     }

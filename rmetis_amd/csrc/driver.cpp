@@ -110,15 +110,13 @@ Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon) {
     r.tile_off = a.get<int32_t>(ntiles);
     r.mov = a.get<int4>(n);
     r.pool = a.get<int32_t>(m);
-    r.heavy_list = a.get<int32_t>(n);
     r.heavy_rec = a.get<int4>(n);
     r.heavy_scratch = k > 4096 ? a.get<int32_t>((int64_t)kHeavyGlobalBlocks * (3 * (int64_t)k + 2)) : nullptr;
     r.moves = a.get<int2>(n);
     r.pw = a.get<int32_t>(kc);
     r.fmax = a.get<long long>(kc);
     r.omax = a.get<long long>(kc);
-    r.gA = a.get<int32_t>(kc);
-    r.gR = a.get<int32_t>(kc);
+    r.adm_scratch = a.get<int32_t>(4 * kc);
     int32_t* cnt = a.get<int32_t>(64);
     r.heavy_cnt = cnt + 0;
     r.nmov = cnt + 1;
@@ -165,17 +163,14 @@ std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffer
     return tot;
 }
 
-void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* h_fmax, const long long* h_omax,
-                  int niter, int32_t* d_part, Buffers& B, LevelStat* st) {
+void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* d_fmax, const long long* d_omax,
+                  int niter, int32_t* d_part, int nheavy, const int32_t* heavy_list, Buffers& B, LevelStat* st) {
+    // No host synchronisation unless statistics are requested: the
+    // thresholds already live on the device and the pass count is fixed
+    // (a cut pass that moves nothing turns the remaining passes into no-ops).
     hipStream_t s = cx.stream;
-    const int G = pick_group(g.n, g.m);
-    const int nheavy = refine_setup_level(g, G, B.rb, s);   // synchronises the stream
-    const size_t kc = (size_t)k * g.ncon;
-    long long* stage = (long long*)cx.host_stage(2 * kc * sizeof(long long));
-    std::copy(h_fmax, h_fmax + kc, stage);
-    std::copy(h_omax, h_omax + kc, stage + kc);
-    RK_HIP(hipMemcpyAsync(B.rb.fmax, stage, kc * sizeof(long long), hipMemcpyHostToDevice, s));
-    RK_HIP(hipMemcpyAsync(B.rb.omax, stage + kc, kc * sizeof(long long), hipMemcpyHostToDevice, s));
+    B.rb.fmax = const_cast<long long*>(d_fmax);
+    B.rb.omax = const_cast<long long*>(d_omax);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (st) {
         RK_HIP(hipEventCreate(&e0));
@@ -183,10 +178,10 @@ void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* h_
         RK_HIP(hipEventRecord(e0, s));
         RK_HIP(hipMemsetAsync(B.rb.passes, 0, sizeof(int32_t), s));
     }
-    part_weights(g, d_part, k, B.rb.pw, s);                        // greedy.rs:50
+    part_weights(g, d_part, k, B.rb.pw, s);                                          // greedy.rs:50
     RK_HIP(hipMemsetAsync(B.rb.done, 0, sizeof(int32_t), s));
-    refine_pass(g, k, 0, G, nheavy, B.rb, d_part, s);               // greedy.rs:60-123
-    for (int it = 0; it < niter; ++it) refine_pass(g, k, 1, G, nheavy, B.rb, d_part, s);  // 126-190
+    refine_pass(g, k, 0, nheavy, heavy_list, B.rb, d_part, s);                       // greedy.rs:60-123
+    for (int it = 0; it < niter; ++it) refine_pass(g, k, 1, nheavy, heavy_list, B.rb, d_part, s);  // 126-190
     if (st) {
         RK_HIP(hipEventRecord(e1, s));
         int32_t passes = 0;
@@ -198,8 +193,6 @@ void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* h_
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
-    // The host staging buffer is reused by the next level only after the
-    // next refine_setup_level synchronisation.
 }
 
 bool grow_host(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
@@ -287,12 +280,13 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
     const int32_t ncon = g.ncon;
     Buffers B = alloc_buffers(cx.arena, g.n, g.m, k, ncon);
     const std::vector<int32_t> total = total_weight_checked(cx, g, B);
+    int32_t* fin_heavy = cx.arena.get<int32_t>(g.n);
+    const int32_t fin_nheavy = heavy_list_sync(g, fin_heavy, B.rb, s);
     std::vector<double> ub(ncon);
     for (int32_t c = 0; c < ncon; ++c)                                  // kway.rs:44-49
         ub[c] = ubvec ? (double)ubvec[c] : 1.0 + (double)o.ufactor / 1000.0;
     const int32_t limit = std::max(20 * k, 20);                         // coarsen/mod.rs:43
     const size_t kc = (size_t)k * ncon;
-    std::vector<long long> fmax(kc), omax(kc);
     std::vector<int32_t> ltot(ncon);
     bool have_best = false;
     int32_t best = 0;
@@ -310,7 +304,9 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
         const Arena::Mark mark = cx.arena.mark();
         struct Level {
             DevGraph g;
-            int32_t* cmap;
+            int32_t* cmap;        // fine vertex -> vertex of this level
+            int32_t* heavy;       // heavy-vertex list of this level
+            int32_t nheavy;
         };
         std::vector<Level> levels;
         std::vector<LevelStat> lst;
@@ -327,12 +323,12 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
                 match_graph(*cur, o.ctype_shem, rng.next_u64(), kMatchRounds, B.cb, s);
             }
             int32_t* cmap = cx.arena.get<int32_t>(cur->n);
+            int32_t* heavy = cx.arena.get<int32_t>(cur->n);
             DevGraph cg;
-            int32_t cn = 0;
+            ContractResult cr;
             {
                 ProfScope ps(kProfContract, 0.0, cur->n, s);
-                cn = contract_numbering(*cur, B.cb.match, cmap, B.cb, s);
-                contract_build(*cur, B.cb.match, cmap, cn, cg, false, B.cb, alloc, nullptr, s);
+                cr = contract(*cur, B.cb.match, cmap, cg, false, B.cb, alloc, heavy, s);   // one sync
             }
             if (st) {
                 LevelStat ls;
@@ -345,13 +341,25 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
                 (void)hipEventDestroy(e1);
                 lst.push_back(ls);
             }
+            const int32_t cn = cr.cn;
             const float ratio = (float)cn / (float)cur->n;              // coarsen/mod.rs:61
-            levels.push_back({cg, cmap});
+            levels.push_back({cg, cmap, heavy, cr.nheavy});
             if (cn <= limit || ratio > 0.95f) break;                    // coarsen/mod.rs:65
             cur = &levels.back().g;
         }
         const DevGraph& cg = levels.back().g;
         if (cg.n < k) throw Error(-4, "coarsest graph has fewer vertices than nparts");
+        // Threshold tables of every level (graph/mod.rs:392-410, kway.rs:44-49,
+        // 67-69, 86-88) in one upload: level i of the hierarchy (0 = input).
+        const size_t nl = levels.size() + 1;
+        long long* stage = (long long*)cx.host_stage(2 * kc * nl * sizeof(long long));
+        for (size_t i = 0; i < nl; ++i) {
+            const DevGraph& lg = i == 0 ? g : levels[i - 1].g;
+            for (int c = 0; c < ncon; ++c) ltot[c] = lg.vwgt ? total[c] : lg.n;
+            level_thresholds(k, ncon, tpwgts, ltot.data(), ub.data(), stage + 2 * kc * i, stage + 2 * kc * i + kc);
+        }
+        long long* d_thr = cx.arena.get<long long>((int64_t)(2 * kc * nl));
+        RK_HIP(hipMemcpyAsync(d_thr, stage, 2 * kc * nl * sizeof(long long), hipMemcpyHostToDevice, s));
         // initial partition of the coarsest level on the host (kway.rs:66-70)
         auto t0 = std::chrono::steady_clock::now();
         std::vector<int32_t> hx(cg.n + 1), ha(std::max(cg.m, 1)), hv, hp(cg.n);
@@ -375,22 +383,27 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
             throw Error(-4, "coarsest graph has fewer vertices than nparts");
         }
         int32_t* pc = cx.arena.get<int32_t>(cg.n);
-        RK_HIP(hipMemcpyAsync(pc, hp.data(), sizeof(int32_t) * cg.n, hipMemcpyHostToDevice, s));
-        RK_HIP(hipStreamSynchronize(s));
+        int32_t* hstage = (int32_t*)cx.host_stage2(sizeof(int32_t) * cg.n);
+        std::copy(hp.begin(), hp.end(), hstage);
+        RK_HIP(hipMemcpyAsync(pc, hstage, sizeof(int32_t) * cg.n, hipMemcpyHostToDevice, s));
         auto t1 = std::chrono::steady_clock::now();
         // refine the coarsest level, then project and refine every finer level
         LevelStat cstat;
         cstat.n = cg.n;
         cstat.m = cg.m;
-        level_thresholds(k, ncon, tpwgts, ltot.data(), ub.data(), fmax.data(), omax.data());
-        refine_level(cx, cg, k, fmax.data(), omax.data(), o.niter, pc, B, st ? &cstat : nullptr);
+        {
+            const size_t li = nl - 1;
+            refine_level(cx, cg, k, d_thr + 2 * kc * li, d_thr + 2 * kc * li + kc, o.niter, pc,
+                         levels.back().nheavy, levels.back().heavy, B, st ? &cstat : nullptr);
+        }
         for (int64_t i = (int64_t)levels.size() - 1; i >= 0; --i) {     // kway.rs:76-92
             const DevGraph& fine = i == 0 ? g : levels[i - 1].g;
+            const int32_t fheavy = i == 0 ? fin_nheavy : levels[i - 1].nheavy;
+            const int32_t* flist = i == 0 ? fin_heavy : levels[i - 1].heavy;
             int32_t* pf = cx.arena.get<int32_t>(fine.n);
             project(fine.n, levels[i].cmap, pc, pf, s);
-            for (int c = 0; c < ncon; ++c) ltot[c] = fine.vwgt ? total[c] : fine.n;
-            level_thresholds(k, ncon, tpwgts, ltot.data(), ub.data(), fmax.data(), omax.data());
-            refine_level(cx, fine, k, fmax.data(), omax.data(), o.niter, pf, B, st ? &lst[i] : nullptr);
+            refine_level(cx, fine, k, d_thr + 2 * kc * i, d_thr + 2 * kc * i + kc, o.niter, pf, fheavy, flist, B,
+                         st ? &lst[i] : nullptr);
             pc = pf;
         }
         edge_cut(g, pc, B.cut_acc, B.cut_out, pick_group(g.n, g.m), s);    // kway.rs:94

@@ -22,6 +22,7 @@ void column_sums(const DevGraph& g, long long* out, hipStream_t s);
 
 // ---- refine.hip ----
 constexpr int kHeavyGlobalBlocks = 256;
+constexpr int kGainHeavy = 64;   // vertices of higher degree get a block each in the gain kernels
 
 // Work buffers of the greedy rounds, sized for the finest level of a call.
 struct RefineBufs {
@@ -30,15 +31,13 @@ struct RefineBufs {
     int32_t* tile_off = nullptr;   // [ntiles]
     int4* mov = nullptr;           // [n] dense candidate records, vertex order
     int32_t* pool = nullptr;       // [m] candidate parts
-    int32_t* heavy_list = nullptr; // [n]
     int4* heavy_rec = nullptr;     // [n]
     int32_t* heavy_scratch = nullptr;  // [kHeavyGlobalBlocks * (3k+2)] when k > LDS capacity
     int2* moves = nullptr;         // [n] admitted (v, p) in admission order
     int32_t* pw = nullptr;         // [k*ncon]
     long long* fmax = nullptr;     // [k*ncon] feasibility thresholds floor(tpw*ub)
     long long* omax = nullptr;     // [k*ncon] overweight thresholds floor(tpw*ub)
-    int32_t* gA = nullptr;         // [k*ncon] admission flows (when not in LDS)
-    int32_t* gR = nullptr;
+    int32_t* adm_scratch = nullptr;  // [4*k*ncon] admission tables when k*ncon exceeds LDS
     int32_t* heavy_cnt = nullptr;  // device counters
     int32_t* nmov = nullptr;
     int32_t* nmoves = nullptr;
@@ -47,20 +46,30 @@ struct RefineBufs {
     void* scan_tmp = nullptr;
 };
 
-// Builds the heavy-vertex list (degree > 4G) of a level; returns its size.
-int refine_setup_level(const DevGraph& g, int G, RefineBufs& b, hipStream_t s);
+// Heavy-vertex list (degree > kGainHeavy) of a level with n = *n_dev (or n_max
+// when n_dev is null) vertices; the count lands in *count (no host sync).
+void heavy_list_build(int32_t n_max, const int32_t* n_dev, const int32_t* xadj, int32_t* list, int32_t* count,
+                      hipStream_t s);
+// heavy_list_build for a level whose size is known on the host, returning the
+// count (synchronises).
+int heavy_list_sync(const DevGraph& g, int32_t* list, RefineBufs& b, hipStream_t s);
 // One greedy pass: mode 0 balance repair, mode 1 cut optimisation.  Reads
 // b.pw (running weights) and updates it, part, b.moves/b.nmoves; a
 // cut-optimisation pass with no move sets *b.done, which turns every later
 // pass of the level into a no-op.
-void refine_pass(const DevGraph& g, int32_t k, int mode, int G, int nheavy, RefineBufs& b, int32_t* part,
-                 hipStream_t s);
+void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32_t* heavy_list, RefineBufs& b,
+                 int32_t* part, hipStream_t s);
+// ---- admit.hip ----
+// Ordered admission of the candidates that the gain kernels left in
+// b.tile_rec / b.tile_cnt (one workgroup; updates b.pw, part, b.moves).
+void admit_pass(const DevGraph& g, int32_t k, int mode, int ntiles, RefineBufs& b, int32_t* part, hipStream_t s);
+
 // Gain table of every vertex (repair-mode candidate lists with every part
 // treated as overweight): per-vertex record (v or -1, pv, ncand, nbefore |
 // zero-group flag << 31), gain of the implicit zero group, and candidate
 // parts / gains in admission order at xadj[v].
-void gain_table(const DevGraph& g, int32_t k, int G, int nheavy, RefineBufs& b, const int32_t* part, int4* vrec,
-                int32_t* g0, int32_t* gains, hipStream_t s);
+void gain_table(const DevGraph& g, int32_t k, int nheavy, const int32_t* heavy_list, RefineBufs& b,
+                const int32_t* part, int4* vrec, int32_t* g0, int32_t* gains, hipStream_t s);
 
 // ---- coarsen.hip ----
 struct CoarsenBufs {
@@ -84,20 +93,23 @@ struct CoarsenBufs {
 };
 
 constexpr int kMaxMatchRounds = 64;
+constexpr int kContractBig = 64;   // coarse vertices of larger degree sum use the global-hash path
 
 // Deterministic seeded handshake matching (ctype 1 heavy edge, 0 random).
 void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, CoarsenBufs& b, hipStream_t s);
-// Coarse numbering from a symmetric matching; returns cn (synchronises).
-int32_t contract_numbering(const DevGraph& g, const int32_t* match, int32_t* cmap, CoarsenBufs& b,
-                           hipStream_t s);
-// Builds the coarse graph into cg (arrays preallocated with cn / cm capacity
-// by the caller through alloc callback).  Returns cm (synchronises).
 struct CoarseAlloc {
     virtual int32_t* alloc_i32(int64_t count) = 0;
     virtual ~CoarseAlloc() = default;
 };
-int32_t contract_build(const DevGraph& g, const int32_t* match, const int32_t* cmap, int32_t cn, DevGraph& cg,
-                       bool aggregate_vsize, CoarsenBufs& b, CoarseAlloc& al, int* adjwgt_not_one,
-                       hipStream_t s);
+struct ContractResult {
+    int32_t cn = 0, cm = 0;
+    bool adjwgt_not_one = false;   // some merged edge weight differs from 1
+    int32_t nheavy = 0;            // coarse vertices of degree > kGainHeavy (when listed)
+};
+// Coarse graph of a symmetric matching (graph/mod.rs:284-384) into cg, with
+// cmap[n]; arrays come from `al` sized by the fine graph.  When heavy_list is
+// given, the coarse heavy-vertex list is built as well.  One synchronisation.
+ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, DevGraph& cg, bool aggregate_vsize,
+                        CoarsenBufs& b, CoarseAlloc& al, int32_t* heavy_list, hipStream_t s);
 
 }  // namespace rk

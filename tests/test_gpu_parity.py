@@ -149,6 +149,40 @@ def test_greedy_pass(case, k, ub, skew):
         part, pw = p_o, pw_o
 
 
+def test_greedy_pass_with_hubs():
+    # tiles mixing heavy (block-per-vertex) and ordinary vertices
+    from graphs import build_undirected
+    rng = np.random.default_rng(8)
+    n = 3000
+    edges = set()
+    for hub in (5, 700, 2900):
+        for u in rng.choice(n, 400, replace=False):
+            if u != hub:
+                edges.add((min(hub, u), max(hub, u)))
+    for _ in range(6000):
+        u, v = rng.integers(0, n, 2)
+        if u != v:
+            edges.add((min(u, v), max(u, v)))
+    x, a = build_undirected(n, sorted(edges))
+    src = np.repeat(np.arange(n), np.diff(x))
+    key = np.minimum(src, a).astype(np.int64) * n + np.maximum(src, a)
+    _, inv = np.unique(key, return_inverse=True)
+    aw = rng.integers(1, 7, inv.max() + 1)[inv].astype(np.int32)
+    vw = rng.integers(1, 4, n).astype(np.int32)
+    g = _graph(x, a, vwgt=vw, adjwgt=aw)
+    for k in (4, 37):
+        part = rng.integers(0, k, n).astype(np.int32)
+        part[: n // 4] = 1
+        tpw = np.full(k, vw.sum() / k)
+        ub = np.array([1.02])
+        pw = O.partition_weights(1, vw, part, k)
+        for mode in (0, 1, 1):
+            p_o, pw_o, mv_o = O.greedy_pass(1, x, a, vw, aw, part, k, tpw, ub, mode, pw)
+            p_g, pw_g, mv_g = R.greedy_pass(g, part, k, tpw, ub, mode, pw)
+            assert np.array_equal(mv_g, mv_o) and np.array_equal(p_g, p_o) and np.array_equal(pw_g, pw_o)
+            part, pw = p_o, pw_o
+
+
 @pytest.mark.parametrize("ncon", [1, 2])
 @pytest.mark.parametrize("k", [3, 8, 33])
 def test_greedy_refine(ncon, k):
