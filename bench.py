@@ -40,6 +40,11 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host-api", action="store_true",
+                    help="skip timing the host-buffer C entry point (PCIe-inclusive rate)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="issue one device-to-device copy of this many bytes first (a kernel of known "
+                         "traffic, to scale FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -63,6 +68,15 @@ def main():
     d_part = DA(count=g.n, dtype=np.int32)
     sync = L.rmetis_hip_device_synchronize
     sync()
+    if args.copy_bytes > 0:
+        src = L.rmetis_hip_device_alloc(args.copy_bytes)
+        dst = L.rmetis_hip_device_alloc(args.copy_bytes)
+        if not src or not dst or L.rmetis_hip_memcpy(dst, src, args.copy_bytes, 3) != 1:
+            raise RuntimeError(f"calibration copy failed: {_lib.last_error()}")
+        sync()
+        L.rmetis_hip_device_free(src)
+        L.rmetis_hip_device_free(dst)
+        log(f"calibration copy of {args.copy_bytes} bytes done")
     opts = R.Options(rtype=R.RType.Greedy, seed=args.seed, ncuts=1, niter=10, ufactor=30).to_c_array()
     stats = _lib.HipStats()
     obj = C.c_int32()
@@ -116,6 +130,30 @@ def main():
                        "coarsen_edges_per_s": round(m_i / (cm * 1e-3), 1) if cm > 0 else None,
                        "refine_edges_per_s_per_pass": round(m_i * ps / (rm * 1e-3), 1) if rm > 0 else None})
 
+    # the same partition through the host-buffer drop-in entry point (copies
+    # the CSR in and the partition out over PCIe on every call)
+    host_api = None
+    if not args.no_host_api:
+        nv, nc, npart = C.c_int32(g.n), C.c_int32(g.ncon), C.c_int32(k)
+        i32 = _lib.i32p
+        hx, ha = g.xadj.ctypes.data_as(i32), g.adjncy.ctypes.data_as(i32)
+        hvw = g.vwgt.ctypes.data_as(i32) if g.vwgt is not None else None
+        haw = g.adjwgt.ctypes.data_as(i32) if g.adjwgt is not None else None
+        hpart = np.zeros(g.n, np.int32)
+        hobj = C.c_int32()
+        reps = max(1, min(args.steps, 3))
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            rc = L.RMETIS_PartGraphKway(C.byref(nv), C.byref(nc), hx, ha, hvw, None, haw, C.byref(npart), None,
+                                        None, opts.ctypes.data_as(i32), C.byref(hobj), hpart.ctypes.data_as(i32))
+            if rc != 1:
+                raise RuntimeError(f"RMETIS_PartGraphKway failed rc={rc}: {_lib.last_error()}")
+        t_host = time.perf_counter() - t1
+        host_api = {"value": round(reps / t_host, 4), "unit": "graphs/s", "calls": reps,
+                    "note": "RMETIS_PartGraphKway on host buffers: CSR upload, validation and the "
+                            "partition download included"}
+        log(f"host API: {reps} calls in {t_host:.3f}s cut={hobj.value}")
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -133,6 +171,16 @@ def main():
                "sample": f"{runs} full k-way partitions of the same graph (seeds {args.seed}..{args.seed + runs - 1}) "
                          f"by the oracle's C++ restatement of the reference, one host core, {t_cpu:.1f}s",
                "cut": int(cobj)}
+
+    # HBM bytes per launch of the same kernel from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known size)
+    traffic, traffic_src = None, None
+    tpath = os.path.join(ROOT, "profiles", f"k_gain_tile_{args.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tj = json.load(fh)
+        if tj.get("level_n") == g.n:
+            traffic, traffic_src = tj["traffic_bytes"], tj["summary"]
 
     value = args.steps / elapsed
     out = {
@@ -158,9 +206,10 @@ def main():
         "levels": levels,
         "roofline": {"bound": "hbm", "kernel": "k_gain (finest level)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": int(bytes_per_launch),
+                     "traffic": traffic, "traffic_source": traffic_src, "bytes_per_launch": int(bytes_per_launch),
                      "avg_launch_us": round(avg_ms * 1e3, 2), "launches": g_launch},
         "cpu_baseline": cpu,
+        "host_api": host_api,
     }
     print(json.dumps(out), flush=True)
 

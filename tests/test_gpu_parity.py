@@ -149,6 +149,48 @@ def test_greedy_pass(case, k, ub, skew):
         part, pw = p_o, pw_o
 
 
+@pytest.mark.parametrize("k,ncon,wmax", [
+    (64, 1, 3),      # register slack, one block; speculation with 2 weight planes
+    (64, 2, 200),    # two constraints, 8 weight planes
+    (100, 1, 255),   # two register blocks
+    (200, 2, 3),     # four register blocks, two constraints
+    (64, 1, 1000),   # weights past the speculation limit: exact walk only
+    (300, 1, 5),     # slack table in LDS
+    (100, 3, 4),     # three constraints: table walk
+    (1500, 3, 2),    # k*ncon > 4096: tables in global memory
+])
+def test_greedy_pass_admission_variants(k, ncon, wmax):
+    # every admission variant (register / LDS / global slack, speculation on
+    # and off) against the oracle, on a grid whose vertices are nearly all
+    # boundary movers
+    x, a = grid_sorted(60, 60)
+    n = len(x) - 1
+    rng = np.random.default_rng(k * 7 + ncon)
+    aw = rng.integers(1, 6, len(a)).astype(np.int32)
+    # symmetric edge weights: weight of (u, v) from the smaller endpoint's entry
+    src = np.repeat(np.arange(n), np.diff(x))
+    key = {}
+    for e in range(len(a)):
+        u, v = int(src[e]), int(a[e])
+        key.setdefault((min(u, v), max(u, v)), int(aw[e]))
+        aw[e] = key[(min(u, v), max(u, v))]
+    vw = rng.integers(1, wmax + 1, n * ncon).astype(np.int32)
+    part = rng.integers(0, k, n).astype(np.int32)
+    part[rng.random(n) < 0.3] = 0
+    tot = vw.reshape(n, ncon).sum(0).astype(np.float64)
+    tpw = np.tile(tot / k, k)
+    ubv = np.full(ncon, 1.05)
+    g = _graph(x, a, vwgt=vw, adjwgt=aw, ncon=ncon)
+    pw = O.partition_weights(ncon, vw, part, k)
+    for mode in (0, 1, 1, 1):
+        p_o, pw_o, mv_o = O.greedy_pass(ncon, x, a, vw, aw, part, k, tpw, ubv, mode, pw)
+        p_g, pw_g, mv_g = R.greedy_pass(g, part, k, tpw, ubv, mode, pw)
+        assert np.array_equal(mv_g, mv_o), (mode, len(mv_g), len(mv_o))
+        assert np.array_equal(p_g, p_o)
+        assert np.array_equal(pw_g, pw_o)
+        part, pw = p_o, pw_o
+
+
 def test_greedy_pass_with_hubs():
     # tiles mixing heavy (block-per-vertex) and ordinary vertices
     from graphs import build_undirected
