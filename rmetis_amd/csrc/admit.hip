@@ -8,9 +8,9 @@
 // carry tens to a few thousand candidates, and every decision may depend on
 // the one before it, so this step is a latency-bound sequential walk:
 //
-//  * all 256 threads read the per-tile candidate counts, gather the records
-//    into one dense ordered list, and stage windows of 256 movers (record,
-//    vertex weights, first candidates) in LDS;
+//  * all 256 threads list the tiles with movers (a bitmap set by the gain
+//    kernel), gather their records into one dense ordered list, and stage
+//    windows of 256 movers (record, vertex weights, first candidates) in LDS;
 //  * the running state is one LDS table slack[p][c] = fmax[p][c] - pw[p][c]:
 //    part p can take a vertex of weight w iff w <= slack[p][c] for every c;
 //  * wave 0 walks the window one mover at a time: lane t reads the slack of
@@ -40,13 +40,15 @@ constexpr int kAT = 256;        // threads per admission workgroup
 constexpr int kLdsKC = 4096;    // k*ncon entries of slack/thresholds kept in LDS
 constexpr int kCL = 8;          // candidates per mover staged in LDS
 constexpr int kWin = 256;       // movers staged per window
-constexpr int kCntWin = 4096;   // tile counts read per window
+constexpr int kCntWin = 2048;   // tiles with movers gathered per window
 constexpr int kNoDest = -1;
 constexpr size_t kLdsBytes = 64 * 1024;   // dynamic LDS of one admission workgroup
 
 struct AdmArgs {
     int32_t mode, k, ncon, ntiles;
     const int32_t* tile_cnt;
+    uint32_t* tile_bits;         // bit per tile with movers (cleared here)
+    int32_t* tile_list;          // [ntiles] ordered ids of those tiles
     const int4* tile_rec;
     int4* mov;
     const int32_t* pool;
@@ -524,37 +526,63 @@ __global__ __launch_bounds__(kAT) void k_admit(AdmArgs a) {
         om[i] = clamp_thr(a.omax[i]);
         slack[i] = f - a.pw[i];
     }
-    // 1. ordered gather of the per-tile records into the dense list a.mov:
-    //    per window of kCntWin tiles, the union region holds the tile counts,
-    //    turned into offsets by a block scan, and every thread copies records
-    //    located by a binary search over those offsets
+    // 1. ordered gather of the per-tile records into the dense list a.mov.
+    //    The gain kernel set one bit per tile with movers: the set bits give
+    //    the ordered list of those tiles (the words are cleared for the next
+    //    pass); per window of kCntWin listed tiles, their counts become
+    //    offsets by a block scan and every thread copies records located by a
+    //    binary search over the offsets.
+    const int nwords = (a.ntiles + 31) / 32;
+    const int wper = (nwords + kAT - 1) / kAT;
+    const int wd0 = min(tid * wper, nwords), wd1 = min(wd0 + wper, nwords);
+    int nz = 0;
+    for (int wd = wd0; wd < wd1; ++wd) nz += __popc(a.tile_bits[wd]);
+    int nlisted;
+    {
+        int off = block_excl_scan_int(nz, s_w, &nlisted);
+        for (int wd = wd0; wd < wd1; ++wd) {
+            uint32_t bits = a.tile_bits[wd];
+            if (bits) a.tile_bits[wd] = 0;
+            while (bits) {
+                a.tile_list[off++] = wd * 32 + __ffs(bits) - 1;
+                bits &= bits - 1;
+            }
+        }
+    }
+    __threadfence();
+    __syncthreads();
+    int32_t* ids = uni;             // window of listed tiles
+    int32_t* offs = uni + kCntWin;  // their record counts, then offsets
     int nmov = 0;
-    for (int tw = 0; tw < a.ntiles; tw += kCntWin) {
-        const int nt = min(kCntWin, a.ntiles - tw);
-        for (int i = tid; i < nt; i += kAT) uni[i] = a.tile_cnt[tw + i];
+    for (int lw = 0; lw < nlisted; lw += kCntWin) {
+        const int nt = min(kCntWin, nlisted - lw);
+        for (int i = tid; i < nt; i += kAT) {
+            const int32_t tl = a.tile_list[lw + i];
+            ids[i] = tl;
+            offs[i] = a.tile_cnt[tl];
+        }
         __syncthreads();
         const int per = (nt + kAT - 1) / kAT;
         const int t0 = min(tid * per, nt), t1 = min(t0 + per, nt);
         int cnt = 0;
-        for (int t = t0; t < t1; ++t) cnt += uni[t];
+        for (int t = t0; t < t1; ++t) cnt += offs[t];
         int wtot;
         int off = block_excl_scan_int(cnt, s_w, &wtot);
-        // counts -> exclusive offsets within this window of tiles, in place
         for (int t = t0; t < t1; ++t) {
-            const int c = uni[t];
-            uni[t] = off;
+            const int c = offs[t];
+            offs[t] = off;
             off += c;
         }
         __syncthreads();
-        // record i of the window comes from the last tile whose offset is <= i
+        // record i of the window comes from the last listed tile whose offset is <= i
         for (int i = tid; i < wtot; i += kAT) {
             int lo = 0, hi = nt - 1;
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (uni[mid] <= i) lo = mid;
+                if (offs[mid] <= i) lo = mid;
                 else hi = mid - 1;
             }
-            a.mov[nmov + i] = a.tile_rec[(int64_t)(tw + lo) * kTile + (i - uni[lo])];
+            a.mov[nmov + i] = a.tile_rec[(int64_t)ids[lo] * kTile + (i - offs[lo])];
         }
         nmov += wtot;
         __syncthreads();
@@ -620,11 +648,11 @@ __global__ __launch_bounds__(kAT) void k_admit(AdmArgs a) {
 }  // namespace
 
 void admit_pass(const DevGraph& g, int32_t k, int mode, int ntiles, RefineBufs& b, int32_t* part, hipStream_t s) {
-    AdmArgs aa{mode, k, g.ncon, ntiles, b.tile_cnt, b.tile_rec, b.mov, b.pool, g.xadj, g.vwgt, b.fmax, b.omax,
+    AdmArgs aa{mode, k, g.ncon, ntiles, b.tile_cnt, b.tile_bits, b.tile_off, b.tile_rec, b.mov, b.pool, g.xadj, g.vwgt, b.fmax, b.omax,
                b.pw, part, b.moves, b.nmoves, b.done, b.passes, b.adm_scratch};
     const int kc = k * g.ncon;
     const size_t win = (size_t)kWinInts * sizeof(int32_t);
-    const size_t cntb = (size_t)kCntWin * sizeof(int32_t);
+    const size_t cntb = 2 * (size_t)kCntWin * sizeof(int32_t);
     const size_t uni = win > cntb ? win : cntb;
     if (kc <= kLdsKC) {
         const size_t base = (size_t)((3 * kc + 3) & ~3) * sizeof(int32_t) + uni;

@@ -26,7 +26,7 @@ struct Error : std::runtime_error {
     } while (0)
 
 constexpr int kBlock = 256;        // threads per block for streaming kernels
-constexpr int kTile = 256;         // vertices per gain-kernel tile (one block)
+constexpr int kTile = 64;          // vertices per gain-kernel tile (one wave)
 constexpr int kNone = -1;
 
 // Integer arithmetic with the reference's i32 wrap-around semantics.
@@ -71,6 +71,19 @@ __device__ inline T group_sum(T x) {
 
 template <int G>
 __device__ inline int group_lane() { return __lane_id() & (G - 1); }
+
+// Compute units of the current device, queried once per device.
+inline int device_cus() {
+    static int cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (cache[dev] == 0) {
+        int c = 0;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+        cache[dev] = c;
+    }
+    return cache[dev];
+}
 
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");

@@ -100,7 +100,7 @@ void thresholds_from_targets(int32_t k, int32_t ncon, const double* tpw, const d
         }
 }
 
-Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon) {
+Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon, hipStream_t s) {
     Buffers B;
     const int64_t ntiles = ((int64_t)n + kTile - 1) / kTile;
     const int64_t kc = (int64_t)k * ncon;
@@ -108,6 +108,8 @@ Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon) {
     r.tile_rec = a.get<int4>(ntiles * kTile);
     r.tile_cnt = a.get<int32_t>(ntiles);
     r.tile_off = a.get<int32_t>(ntiles);
+    r.tile_bits = (uint32_t*)a.get<int32_t>((ntiles + 31) / 32);
+    RK_HIP(hipMemsetAsync(r.tile_bits, 0, (size_t)((ntiles + 31) / 32) * sizeof(uint32_t), s));
     r.mov = a.get<int4>(n);
     r.pool = a.get<int32_t>(m);
     r.heavy_rec = a.get<int4>(n);
@@ -278,7 +280,7 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
                         "(Greedy)");
     hipStream_t s = cx.stream;
     const int32_t ncon = g.ncon;
-    Buffers B = alloc_buffers(cx.arena, g.n, g.m, k, ncon);
+    Buffers B = alloc_buffers(cx.arena, g.n, g.m, k, ncon, cx.stream);
     const std::vector<int32_t> total = total_weight_checked(cx, g, B);
     int32_t* fin_heavy = cx.arena.get<int32_t>(g.n);
     const int32_t fin_nheavy = heavy_list_sync(g, fin_heavy, B.rb, s);

@@ -61,8 +61,8 @@ struct Buffers {
 };
 
 // Allocates the per-call work buffers for a finest level of n vertices,
-// m entries, k parts, ncon constraints.
-Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon);
+// m entries, k parts, ncon constraints (the tile bitmap is cleared on s).
+Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon, hipStream_t s);
 
 // Column sums of vwgt (or n per constraint when vwgt is absent), checked
 // against the int32 range the reference's sums live in.

@@ -28,7 +28,9 @@ constexpr int kGainHeavy = 64;   // vertices of higher degree get a block each i
 struct RefineBufs {
     int4* tile_rec = nullptr;      // [ntiles * kTile]
     int32_t* tile_cnt = nullptr;   // [ntiles]
-    int32_t* tile_off = nullptr;   // [ntiles]
+    int32_t* tile_off = nullptr;   // [ntiles] ordered ids of the tiles with movers (admission scratch)
+    uint32_t* tile_bits = nullptr; // [ceil(ntiles/32)] bit per tile with movers; set by the gain kernel,
+                                   // cleared by the admission kernel that reads it
     int4* mov = nullptr;           // [n] dense candidate records, vertex order
     int32_t* pool = nullptr;       // [m] candidate parts
     int4* heavy_rec = nullptr;     // [n]

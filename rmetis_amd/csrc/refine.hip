@@ -43,6 +43,7 @@ struct GainArgs {
     int32_t* pool;              // candidate parts, vertex v at pool[xadj[v]...]
     int4* tile_rec;
     int32_t* tile_cnt;
+    uint32_t* tile_bits;
     const int32_t* done;
     int4* vrec;                 // optional per-vertex record (gain-table export)
     int32_t* g0_out;            // optional zero-group gain per vertex
@@ -61,178 +62,180 @@ __device__ inline bool part_overweight(const GainArgs& a, int32_t p) {
     return false;
 }
 
-constexpr int kGT = 256;                 // vertices (= threads) per gain tile
-constexpr int kGItems = 16;              // staged edges per thread per batch
-constexpr int kGCap = kGT * kGItems;     // edges staged in LDS per batch
+constexpr int kGWaves = 4;               // waves (independent tiles) per gain block
+constexpr int kGCap = 1024;              // edges a wave stages in LDS per batch
+constexpr int kGItems = kGCap / 64;      // staged edges per lane per batch
+constexpr int kGainBlocksPerCU = 4;      // resident gain blocks per CU (LDS-limited)
+static_assert(kTile == 64, "a gain tile is one wave");
 
-__device__ inline int block_excl_scan_i(int x, int* s_w, int* total) {
-    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+__device__ inline int wave_excl_scan_i(int x, int* total) {
+    const int lane = __lane_id();
     int inc = x;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const int y = __shfl_up(inc, o, 64);
         if (lane >= o) inc += y;
     }
-    if (lane == 63) s_w[wave] = inc;
-    __syncthreads();
-    int off = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < kGT / 64; ++w) {
-        const int c = s_w[w];
-        if (w < wave) off += c;
-        tot += c;
-    }
-    __syncthreads();
-    *total = tot;
-    return off + inc - x;
+    *total = __shfl(inc, 63, 64);
+    return inc - x;
 }
 
-// One block per tile of 256 consecutive vertices.  The tile's edges (of
-// vertices with degree <= kGainHeavy) are streamed in batches of kGCap with
-// kGItems independent coalesced loads per thread (adjncy, adjwgt, then the
-// part[] gathers) into LDS; each thread then builds its own vertex's gain
-// table from LDS: ed of its own part, the boundary test, and one entry per
-// adjacent part with the weight sum (aggregated in place over its edge range).
-__global__ __launch_bounds__(kGT) void k_gain_tile(GainArgs a) {
-    __shared__ int32_t s_pu[kGCap];
-    __shared__ int32_t s_w[kGCap];
-    __shared__ int32_t s_loff[kGT + 1];
-    __shared__ int32_t s_beg[kGT];
-    __shared__ int32_t s_bstart[kGT + 2];
-    __shared__ int s_tmp[kGT / 64];
-    __shared__ int s_nb;
-    if (*a.done) return;
-    const int t = threadIdx.x;
-    const int64_t v0 = (int64_t)blockIdx.x * kGT;
-    const int nv = (int)min((int64_t)kGT, (int64_t)a.n - v0);
-    const bool own = t < nv;
-    const int64_t v = v0 + t;
-    int32_t beg = 0, end = 0, pv = -1;
-    if (own) {
-        beg = a.xadj[v];
-        end = a.xadj[v + 1];
-        pv = a.part[v];
-    }
-    const int32_t deg = end - beg;
-    const bool heavy = own && deg > kGainHeavy;
-    int total;
-    const int loff = block_excl_scan_i((own && !heavy) ? deg : 0, s_tmp, &total);
-    s_loff[t] = loff;
-    s_beg[t] = beg;
-    if (t == 0) s_loff[kGT] = total;
-    const int anyheavy = __syncthreads_or(heavy);
-    if (t == 0) {
-        // batches of whole vertices with at most kGCap staged edges each
-        int nb = 0, vs = 0;
-        while (vs < kGT) {
-            s_bstart[nb++] = vs;
-            int lo = vs + 1, hi = kGT;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_loff[mid] - s_loff[vs] <= kGCap) lo = mid;
-                else hi = mid - 1;
-            }
-            vs = lo;
+// Gain tables of one pass.  Every wave handles tiles of 64 consecutive
+// vertices (lane = vertex) on its own, with its own LDS staging area and no
+// block barriers, so the waves of a CU overlap their load and compute phases
+// freely.  Waves are persistent over tiles gw, gw + nwaves, ...; the next
+// tile's CSR offsets and own parts are loaded while the current one is
+// processed.  Per tile:
+//  * the edges of vertices with degree <= kGainHeavy are staged in batches
+//    of whole vertices of at most kGCap edges: kGItems independent coalesced
+//    loads per lane (adjncy, adjwgt), then the part[] gathers, into LDS;
+//  * each lane builds its vertex's gain table from LDS: the weight to its
+//    own part, the boundary test, and one entry per adjacent part with the
+//    weight sum (aggregated in place over its edge range), ranked by
+//    (gain desc, part asc) into pool[xadj[v]...];
+//  * the tile's mover records are compacted with a ballot into
+//    tile_rec[tile*64...], their count into tile_cnt[tile], and a tile with
+//    movers sets its bit in tile_bits.
+__global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntiles) {
+    __shared__ int32_t s_pu_all[kGWaves][kGCap];
+    __shared__ int32_t s_w_all[kGWaves][kGCap];
+    __shared__ int32_t s_loff_all[kGWaves][kTile + 1];
+    __shared__ int32_t s_beg_all[kGWaves][kTile];
+    const int wv = threadIdx.x >> 6, lane = __lane_id();
+    int32_t* s_pu = s_pu_all[wv];
+    int32_t* s_w = s_w_all[wv];
+    int32_t* s_loff = s_loff_all[wv];
+    int32_t* s_beg = s_beg_all[wv];
+    const int gw = blockIdx.x * kGWaves + wv, nwaves = gridDim.x * kGWaves;
+    int32_t hb = 0, he = 0, hp = -1;   // head of the next tile: xadj[v], xadj[v+1], part[v]
+    auto load_head = [&](int tl) {
+        const int64_t v = (int64_t)tl * kTile + lane;
+        hb = he = 0;
+        hp = -1;
+        if (v < a.n) {
+            hb = a.xadj[v];
+            he = a.xadj[v + 1];
+            hp = a.part[v];
         }
-        s_bstart[nb] = kGT;
-        s_nb = nb;
-    }
-    __syncthreads();
-    const int32_t x0 = s_beg[0];
-    int4 rec = make_int4(-1, 0, 0, 0);
-    const int nb = s_nb;
-    for (int b = 0; b < nb; ++b) {
-        const int vs = s_bstart[b], ve = s_bstart[b + 1];
-        const int P0 = s_loff[vs], P1 = s_loff[ve];
-        int32_t uu[kGItems], ww[kGItems];
+    };
+    if (gw < ntiles) load_head(gw);
+    if (*a.done) return;   // read together with the first head
+    for (int tile = gw; tile < ntiles; tile += nwaves) {
+        const int64_t v = (int64_t)tile * kTile + lane;
+        const bool own = v < a.n;
+        const int32_t beg = hb, end = he, pv = hp;
+        if (tile + nwaves < ntiles) load_head(tile + nwaves);
+        const int32_t deg = end - beg;
+        const bool heavy = own && deg > kGainHeavy;
+        const int sdeg = (own && !heavy) ? deg : 0;
+        int total;
+        const int loff = wave_excl_scan_i(sdeg, &total);
+        s_loff[lane] = loff;
+        s_beg[lane] = beg;
+        if (lane == 63) s_loff[kTile] = total;
+        const bool anyheavy = __ballot(heavy) != 0;
+        const int32_t x0 = __builtin_amdgcn_readfirstlane(beg);   // lane 0 owns v0 < n
+        wave_sync();
+        int4 rec = make_int4(-1, 0, 0, 0);
+        // batches of whole vertices [vs, ve) with at most kGCap staged edges
+        for (int vs = 0; vs < kTile;) {
+            const int P0 = __shfl(loff, vs, 64);
+            const uint64_t fit = __ballot(lane >= vs && loff + sdeg - P0 <= kGCap);
+            const int ve = 64 - __builtin_clzll(fit);            // fit is lanes [vs, ve), ve > vs
+            const int P1 = __shfl(loff + sdeg, ve - 1, 64);
+            int32_t uu[kGItems], ww[kGItems];
 #pragma unroll
-        for (int i = 0; i < kGItems; ++i) {
-            const int p = P0 + t + i * kGT;
-            uu[i] = -1;
-            ww[i] = 1;
-            if (p < P1) {
-                int32_t j;
-                if (!anyheavy) {
-                    j = x0 + p;
-                } else {
-                    int lo = vs, hi = ve - 1;   // vertex whose staged range holds p
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (s_loff[mid] <= p) lo = mid;
-                        else hi = mid - 1;
-                    }
-                    j = s_beg[lo] + (p - s_loff[lo]);
-                }
-                uu[i] = a.adjncy[j];
-                if (a.adjwgt) ww[i] = a.adjwgt[j];
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < kGItems; ++i) {
-            const int p = P0 + t + i * kGT;
-            if (p < P1) {
-                s_pu[p - P0] = a.part[uu[i]];
-                s_w[p - P0] = ww[i];
-            }
-        }
-        __syncthreads();
-        if (own && !heavy && t >= vs && t < ve && deg > 0) {
-            const int lb = loff - P0, le = lb + deg;
-            uint32_t internal = 0;
-            bool ext = false;
-            for (int e = lb; e < le; ++e) {
-                if (s_pu[e] == pv) internal += (uint32_t)s_w[e];
-                else ext = true;
-            }
-            if (ext && (a.mode == 1 || part_overweight(a, pv))) {
-                const int32_t ed_own = (int32_t)internal;
-                const int32_t g0 = wsub(0, ed_own);
-                // aggregate equal parts onto their first occurrence
-                for (int e = lb; e < le; ++e) {
-                    const int32_t q = s_pu[e];
-                    if (q < 0 || q == pv) continue;
-                    uint32_t sum = (uint32_t)s_w[e];
-                    for (int e2 = e + 1; e2 < le; ++e2)
-                        if (s_pu[e2] == q) {
-                            sum += (uint32_t)s_w[e2];
-                            s_pu[e2] = -1;
+            for (int i = 0; i < kGItems; ++i) {
+                const int p = P0 + lane + i * 64;
+                uu[i] = -1;
+                ww[i] = 1;
+                if (p < P1) {
+                    int32_t j;
+                    if (!anyheavy) {
+                        j = x0 + p;
+                    } else {
+                        int lo = vs, hi = ve - 1;   // last vertex whose staged range starts at or before p
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (s_loff[mid] <= p) lo = mid;
+                            else hi = mid - 1;
                         }
-                    s_w[e] = (int32_t)sum;
-                }
-                int ncand = 0, nbefore = 0;
-                for (int e = lb; e < le; ++e) {
-                    const int32_t q = s_pu[e];
-                    if (q < 0 || q == pv || s_w[e] == 0) continue;
-                    const int32_t g = wsub(s_w[e], ed_own);
-                    if (!gain_qualifies(a.mode, g)) continue;
-                    ++ncand;
-                    if (g > g0) ++nbefore;
-                    int rank = 0;
-                    for (int e2 = lb; e2 < le; ++e2) {
-                        const int32_t q2 = s_pu[e2];
-                        if (q2 < 0 || q2 == pv || s_w[e2] == 0) continue;
-                        const int32_t g2 = wsub(s_w[e2], ed_own);
-                        if (!gain_qualifies(a.mode, g2)) continue;
-                        if (g2 > g || (g2 == g && q2 < q)) ++rank;
+                        j = s_beg[lo] + (p - s_loff[lo]);
                     }
-                    a.pool[beg + rank] = q;
-                    if (a.gains) a.gains[beg + rank] = g;
+                    uu[i] = a.adjncy[j];
+                    if (a.adjwgt) ww[i] = a.adjwgt[j];
                 }
-                const bool zflag = gain_qualifies(a.mode, g0);
-                if (a.g0_out) a.g0_out[v] = g0;
-                if (ncand > 0 || zflag)
-                    rec = make_int4((int)v, pv, ncand, (int)((uint32_t)nbefore | (zflag ? 0x80000000u : 0u)));
             }
+#pragma unroll
+            for (int i = 0; i < kGItems; ++i) {
+                const int p = P0 + lane + i * 64;
+                if (p < P1) {
+                    s_pu[p - P0] = a.part[uu[i]];
+                    s_w[p - P0] = ww[i];
+                }
+            }
+            wave_sync();
+            if (lane >= vs && lane < ve && sdeg > 0) {
+                const int lb = loff - P0, le = lb + deg;
+                uint32_t internal = 0;
+                bool ext = false;
+                for (int e = lb; e < le; ++e) {
+                    if (s_pu[e] == pv) internal += (uint32_t)s_w[e];
+                    else ext = true;
+                }
+                if (ext && (a.mode == 1 || part_overweight(a, pv))) {
+                    const int32_t ed_own = (int32_t)internal;
+                    const int32_t g0 = wsub(0, ed_own);
+                    // aggregate equal parts onto their first occurrence
+                    for (int e = lb; e < le; ++e) {
+                        const int32_t q = s_pu[e];
+                        if (q < 0 || q == pv) continue;
+                        uint32_t sum = (uint32_t)s_w[e];
+                        for (int e2 = e + 1; e2 < le; ++e2)
+                            if (s_pu[e2] == q) {
+                                sum += (uint32_t)s_w[e2];
+                                s_pu[e2] = -1;
+                            }
+                        s_w[e] = (int32_t)sum;
+                    }
+                    int ncand = 0, nbefore = 0;
+                    for (int e = lb; e < le; ++e) {
+                        const int32_t q = s_pu[e];
+                        if (q < 0 || q == pv || s_w[e] == 0) continue;
+                        const int32_t g = wsub(s_w[e], ed_own);
+                        if (!gain_qualifies(a.mode, g)) continue;
+                        ++ncand;
+                        if (g > g0) ++nbefore;
+                        int rank = 0;
+                        for (int e2 = lb; e2 < le; ++e2) {
+                            const int32_t q2 = s_pu[e2];
+                            if (q2 < 0 || q2 == pv || s_w[e2] == 0) continue;
+                            const int32_t g2 = wsub(s_w[e2], ed_own);
+                            if (!gain_qualifies(a.mode, g2)) continue;
+                            if (g2 > g || (g2 == g && q2 < q)) ++rank;
+                        }
+                        a.pool[beg + rank] = q;
+                        if (a.gains) a.gains[beg + rank] = g;
+                    }
+                    const bool zflag = gain_qualifies(a.mode, g0);
+                    if (a.g0_out) a.g0_out[v] = g0;
+                    if (ncand > 0 || zflag)
+                        rec = make_int4((int)v, pv, ncand, (int)((uint32_t)nbefore | (zflag ? 0x80000000u : 0u)));
+                }
+            }
+            wave_sync();
+            vs = ve;
         }
-        __syncthreads();
+        if (heavy) rec = a.heavy_rec[v];
+        if (a.vrec && own) a.vrec[v] = rec;
+        const bool f = rec.x >= 0;
+        const uint64_t fm = __ballot(f);
+        if (f) a.tile_rec[(int64_t)tile * kTile + __popcll(fm & ((1ull << lane) - 1ull))] = rec;
+        if (lane == 0) {
+            a.tile_cnt[tile] = __popcll(fm);
+            if (fm) atomicOr(&a.tile_bits[tile >> 5], 1u << (tile & 31));
+        }
     }
-    if (heavy) rec = a.heavy_rec[v];
-    if (a.vrec && own) a.vrec[v] = rec;
-    const bool f = rec.x >= 0;
-    int tot;
-    const int pos = block_flag_scan(f, s_tmp, &tot);
-    if (f) a.tile_rec[(int64_t)blockIdx.x * kTile + pos] = rec;
-    if (t == 0) a.tile_cnt[blockIdx.x] = tot;
 }
 
 // One block per heavy vertex; dense neighbour-part histogram of k entries in
@@ -344,7 +347,9 @@ void launch_gain(const GainArgs& ga, int nheavy, const int32_t* heavy_list, Refi
         hipLaunchKernelGGL(k_gain_heavy, dim3(grid), dim3(kBlock), lds, s, ga, heavy_list, nheavy, b.heavy_rec,
                            b.heavy_scratch);
     }
-    hipLaunchKernelGGL(k_gain_tile, dim3(ntiles), dim3(kGT), 0, s, ga);
+    const int blocks = (ntiles + kGWaves - 1) / kGWaves, resident = device_cus() * kGainBlocksPerCU;
+    hipLaunchKernelGGL(k_gain_tile, dim3(blocks < resident ? blocks : resident), dim3(64 * kGWaves), 0, s, ga,
+                       ntiles);
 }
 
 }  // namespace
@@ -360,7 +365,7 @@ void gain_table(const DevGraph& g, int32_t k, int nheavy, const int32_t* heavy_l
     RK_HIP(hipMemcpyAsync(b.omax, om.data(), kc * sizeof(long long), hipMemcpyHostToDevice, s));
     RK_HIP(hipMemsetAsync(b.done, 0, sizeof(int32_t), s));
     GainArgs ga{g.n, g.ncon, k, 0, g.xadj, g.adjncy, g.adjwgt, g.vwgt, part, b.pw, b.omax,
-                b.heavy_rec, b.pool, b.tile_rec, b.tile_cnt, b.done, vrec, g0, gains};
+                b.heavy_rec, b.pool, b.tile_rec, b.tile_cnt, b.tile_bits, b.done, vrec, g0, gains};
     launch_gain(ga, nheavy, heavy_list, b, s);
     RK_HIP(hipStreamSynchronize(s));
     RK_HIP(hipGetLastError());
@@ -388,7 +393,7 @@ void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32
     if (g.n <= 0) return;
     const int ntiles = (g.n + kTile - 1) / kTile;
     GainArgs ga{g.n, g.ncon, k, mode, g.xadj, g.adjncy, g.adjwgt, g.vwgt, part, b.pw, b.omax,
-                b.heavy_rec, b.pool, b.tile_rec, b.tile_cnt, b.done, nullptr, nullptr, nullptr};
+                b.heavy_rec, b.pool, b.tile_rec, b.tile_cnt, b.tile_bits, b.done, nullptr, nullptr, nullptr};
     {
         // algorithmic bytes: xadj, adjncy, adjwgt, part[adjncy] gather, part[v]
         const double bytes = 4.0 * (g.n + 1) + 4.0 * g.n + (g.adjwgt ? 12.0 : 8.0) * g.m;
