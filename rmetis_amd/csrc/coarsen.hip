@@ -57,15 +57,42 @@ __device__ inline EdgeKey edge_key(int32_t v, int32_t u, int32_t w, int32_t dv, 
     return EdgeKey{((uint64_t)dpri << 32) | (uint64_t)((uint32_t)w ^ 0x80000000u), h};
 }
 
-__global__ __launch_bounds__(kBlock) void k_match_init(int32_t n, const int32_t* __restrict__ xadj,
-                                                       int32_t* __restrict__ match) {
-    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (v < n) match[v] = (xadj[v + 1] == xadj[v]) ? (int32_t)v : -1;
+// The rounds keep the unmatched vertices in per-block lists: block b of nseg
+// owns the vertices [b*S, (b+1)*S) (S = seg), proposes and accepts for them
+// only, and compacts its survivors into its own next list with a block scan,
+// so no global counter is shared.  The list order does not affect the
+// matching: every decision depends on the match state alone.
+__global__ __launch_bounds__(kBlock) void k_match_init(int32_t n, int32_t seg, const int32_t* __restrict__ xadj,
+                                                       int32_t* __restrict__ match, int32_t* __restrict__ cand,
+                                                       int32_t* __restrict__ list, int32_t* __restrict__ seg_cnt) {
+    __shared__ int s_tmp[kBlock / 64];
+    const int64_t lo = (int64_t)blockIdx.x * seg, hi = min(lo + seg, (int64_t)n);
+    int nout = 0;
+    for (int64_t vb = lo; vb < hi; vb += kBlock) {
+        const int64_t v = vb + threadIdx.x;
+        bool open = false;
+        if (v < hi) {
+            open = xadj[v + 1] != xadj[v];
+            match[v] = open ? -1 : (int32_t)v;
+            cand[v] = -1;
+        }
+        int tot;
+        const int pos = block_flag_scan(open, s_tmp, &tot);
+        if (open) list[lo + nout + pos] = (int32_t)v;
+        nout += tot;
+    }
+    if (threadIdx.x == 0) seg_cnt[blockIdx.x] = nout;
 }
 
-// Round r runs only if round r-1 matched or retired something (flags[r-1]).
+// Round r proposes for the unmatched vertices of each block's list and runs
+// only if round r-1 matched or retired something (prev_changed).  An open
+// vertex whose previous proposal is still unmatched proposes it again
+// without a rescan: its unmatched neighbours only ever lose members, so the
+// best of them stays the best while it is unmatched.
 template <int G>
-__global__ __launch_bounds__(kBlock) void k_match_propose(int32_t n, const int32_t* __restrict__ xadj,
+__global__ __launch_bounds__(kBlock) void k_match_propose(int32_t seg, const int32_t* __restrict__ list,
+                                                          const int32_t* __restrict__ seg_cnt,
+                                                          const int32_t* __restrict__ xadj,
                                                           const int32_t* __restrict__ adjncy,
                                                           const int32_t* __restrict__ adjwgt,
                                                           const int32_t* __restrict__ match,
@@ -74,51 +101,81 @@ __global__ __launch_bounds__(kBlock) void k_match_propose(int32_t n, const int32
     if (prev_changed && *prev_changed == 0) return;
     constexpr int NG = kBlock / G;
     const int gl = group_lane<G>();
-    const int64_t v = (int64_t)blockIdx.x * NG + threadIdx.x / G;
-    if (v >= n) return;
-    if (match[v] != -1) return;   // group-uniform
-    const int32_t beg = xadj[v], end = xadj[v + 1];
-    const int32_t dv = end - beg;
-    EdgeKey best{0ull, 0u};
-    int32_t bu = -1;
-    for (int32_t j = beg + gl; j < end; j += G) {
-        const int32_t u = adjncy[j];
-        if (match[u] != -1) continue;
-        const int32_t du = xadj[u + 1] - xadj[u];
-        const EdgeKey key = edge_key((int32_t)v, u, adjwgt ? adjwgt[j] : 1, dv, du, seed, shem);
-        if (bu < 0 || key.gt(best) || (key.eq(best) && u < bu)) {
-            best = key;
-            bu = u;
+    const int32_t nl = seg_cnt[blockIdx.x];
+    const int32_t* mine = list + (int64_t)blockIdx.x * seg;
+    for (int i = threadIdx.x / G; i < nl; i += NG) {
+        const int32_t v = mine[i];
+        const int32_t prev = cand[v];
+        if (prev >= 0 && match[prev] == -1) continue;   // group-uniform
+        const int32_t beg = xadj[v], end = xadj[v + 1];
+        const int32_t dv = end - beg;
+        EdgeKey best{0ull, 0u};
+        int32_t bu = -1;
+        for (int32_t j = beg + gl; j < end; j += G) {
+            const int32_t u = adjncy[j];
+            if (match[u] != -1) continue;
+            const int32_t du = xadj[u + 1] - xadj[u];
+            const EdgeKey key = edge_key(v, u, adjwgt ? adjwgt[j] : 1, dv, du, seed, shem);
+            if (bu < 0 || key.gt(best) || (key.eq(best) && u < bu)) {
+                best = key;
+                bu = u;
+            }
         }
-    }
 #pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) {
-        const EdgeKey ob{__shfl_xor(best.hi, o, G), __shfl_xor(best.lo, o, G)};
-        const int32_t ou = __shfl_xor(bu, o, G);
-        if (ou >= 0 && (bu < 0 || ob.gt(best) || (ob.eq(best) && ou < bu))) {
-            best = ob;
-            bu = ou;
+        for (int o = G / 2; o > 0; o >>= 1) {
+            const EdgeKey ob{__shfl_xor(best.hi, o, G), __shfl_xor(best.lo, o, G)};
+            const int32_t ou = __shfl_xor(bu, o, G);
+            if (ou >= 0 && (bu < 0 || ob.gt(best) || (ob.eq(best) && ou < bu))) {
+                best = ob;
+                bu = ou;
+            }
         }
+        if (gl == 0) cand[v] = bu >= 0 ? bu : -2;
     }
-    if (gl == 0) cand[v] = bu >= 0 ? bu : -2;
 }
 
-__global__ __launch_bounds__(kBlock) void k_match_accept(int32_t n, const int32_t* __restrict__ cand,
-                                                         int32_t* __restrict__ match, int32_t* __restrict__ changed,
+// Mutual proposals become pairs, vertices without an unmatched neighbour
+// (cand -2) retire; each block compacts the rest of its list into its next
+// list.
+__global__ __launch_bounds__(kBlock) void k_match_accept(int32_t seg, const int32_t* __restrict__ list,
+                                                         const int32_t* __restrict__ seg_cnt,
+                                                         const int32_t* __restrict__ cand,
+                                                         int32_t* __restrict__ match, int32_t* __restrict__ next,
+                                                         int32_t* __restrict__ next_cnt, int32_t* __restrict__ changed,
                                                          const int32_t* __restrict__ prev_changed) {
-    if (prev_changed && *prev_changed == 0) return;
-    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    bool ch = false;
-    if (v < n && match[v] == -1) {
-        const int32_t u = cand[v];
-        if (u == -2) {
-            match[v] = (int32_t)v;
-            ch = true;
-        } else if (u >= 0 && cand[u] == (int32_t)v) {
-            match[v] = u;
-            ch = true;
-        }
+    __shared__ int s_tmp[kBlock / 64];
+    const int32_t nl = seg_cnt[blockIdx.x];
+    if (prev_changed && *prev_changed == 0) {
+        if (threadIdx.x == 0) next_cnt[blockIdx.x] = nl;   // list carried over unchanged (never read)
+        return;
     }
+    const int32_t* mine = list + (int64_t)blockIdx.x * seg;
+    int32_t* out = next + (int64_t)blockIdx.x * seg;
+    bool ch = false;
+    int nout = 0;
+    for (int ib = 0; ib < nl; ib += kBlock) {
+        const int i = ib + threadIdx.x;
+        int32_t v = -1;
+        bool open = false;
+        if (i < nl) {
+            v = mine[i];
+            const int32_t u = cand[v];
+            if (u == -2) {
+                match[v] = v;
+                ch = true;
+            } else if (u >= 0 && cand[u] == v) {
+                match[v] = u;
+                ch = true;
+            } else {
+                open = true;
+            }
+        }
+        int tot;
+        const int pos = block_flag_scan(open, s_tmp, &tot);
+        if (open) out[nout + pos] = v;
+        nout += tot;
+    }
+    if (threadIdx.x == 0) next_cnt[blockIdx.x] = nout;
     // one plain store per block that changed something (every writer stores 1)
     if (__syncthreads_or(ch) && threadIdx.x == 0) *changed = 1;
 }
@@ -165,155 +222,144 @@ __device__ inline int32_t seq_edge(const int32_t* xadj, int32_t a, int32_t b, in
 }
 
 
-constexpr int kCT = 256;                 // coarse vertices (= threads) per contraction tile
-constexpr int kCItems = 16;              // staged positions per thread per batch
-constexpr int kCCap = kCT * kCItems;     // merged-sequence positions staged per batch
+constexpr int kCWaves = 4;               // waves (independent tiles) per contraction block
+constexpr int kCCap = 1024;              // merged-sequence positions a wave stages per batch
+constexpr int kCItems = kCCap / 64;      // staged positions per lane per batch
+constexpr int kContractBlocksPerCU = 3;  // resident contraction blocks per CU (LDS- and VGPR-limited)
 
-__device__ inline int block_excl_scan_c(int x, int* s_w, int* total) {
-    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+__device__ inline int wave_excl_scan_c(int x, int* total) {
+    const int lane = __lane_id();
     int inc = x;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const int y = __shfl_up(inc, o, 64);
         if (lane >= o) inc += y;
     }
-    if (lane == 63) s_w[wave] = inc;
-    __syncthreads();
-    int off = 0, tot = 0;
-#pragma unroll
-    for (int w = 0; w < kCT / 64; ++w) {
-        const int c = s_w[w];
-        if (w < wave) off += c;
-        tot += c;
-    }
-    __syncthreads();
-    *total = tot;
-    return off + inc - x;
+    *total = __shfl(inc, 63, 64);
+    return inc - x;
 }
 
-// One block per tile of 256 consecutive coarse vertices.  Their merged
-// adjacency sequences adj(a) ++ adj(b) are streamed in batches of kCCap
-// positions (independent coalesced loads of adjncy/adjwgt, then the cmap
-// gathers) into LDS; each thread then merges its own coarse vertex in LDS:
+// Every wave merges tiles of 64 consecutive coarse vertices (lane = coarse
+// vertex) on its own, with its own LDS area and no block barriers; waves are
+// persistent over tiles gw, gw + nwaves, ... while tile*64 < cn.  The merged
+// sequences adj(a) ++ adj(b) of the tile's ordinary coarse vertices are
+// staged in batches of whole vertices of at most kCCap positions: each lane
+// marks its own positions in a byte array of owners, then kCItems
+// independent coalesced loads per lane fetch adjncy/adjwgt and the cmap[]
+// gathers into LDS.  Each lane then merges its coarse vertex in LDS:
 // internal entries dropped, equal coarse neighbours summed onto their first
 // occurrence, and the survivors written in sequence order.
-__global__ __launch_bounds__(kCT) void k_contract_tile(const int32_t* __restrict__ cn_dev,
-                                                       const int32_t* __restrict__ xadj,
-                                                       const int32_t* __restrict__ adjncy,
-                                                       const int32_t* __restrict__ adjwgt,
-                                                       const int32_t* __restrict__ cmap,
-                                                       const int32_t* __restrict__ members,
-                                                       const int32_t* __restrict__ degsum,
-                                                       const int32_t* __restrict__ tmp_off,
-                                                       int32_t* __restrict__ tmp_adj, int32_t* __restrict__ tmp_w,
-                                                       int32_t* __restrict__ cdeg, int32_t* __restrict__ big_list,
-                                                       int32_t* __restrict__ counters) {
-    __shared__ int32_t s_cu[kCCap];
-    __shared__ int32_t s_w[kCCap];
-    __shared__ int32_t s_loff[kCT + 1];
-    __shared__ int32_t s_xa[kCT], s_xb[kCT], s_da[kCT];
-    __shared__ int32_t s_bstart[kCT + 2];
-    __shared__ int s_tmp[kCT / 64];
-    __shared__ int s_nb;
-    const int t = threadIdx.x;
+__global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* __restrict__ cn_dev,
+                                                               const int32_t* __restrict__ xadj,
+                                                               const int32_t* __restrict__ adjncy,
+                                                               const int32_t* __restrict__ adjwgt,
+                                                               const int32_t* __restrict__ cmap,
+                                                               const int32_t* __restrict__ members,
+                                                               const int32_t* __restrict__ degsum,
+                                                               const int32_t* __restrict__ tmp_off,
+                                                               int32_t* __restrict__ tmp_adj,
+                                                               int32_t* __restrict__ tmp_w, int32_t* __restrict__ cdeg,
+                                                               int32_t* __restrict__ big_list,
+                                                               int32_t* __restrict__ counters) {
+    __shared__ int32_t s_cu_all[kCWaves][kCCap];
+    __shared__ int32_t s_w_all[kCWaves][kCCap];
+    __shared__ uint8_t s_own_all[kCWaves][kCCap];
+    __shared__ int32_t s_loff_all[kCWaves][65], s_xa_all[kCWaves][64], s_xb_all[kCWaves][64], s_da_all[kCWaves][64];
+    const int wv = threadIdx.x >> 6, lane = __lane_id();
+    int32_t* s_cu = s_cu_all[wv];
+    int32_t* s_w = s_w_all[wv];
+    uint8_t* s_own = s_own_all[wv];
+    int32_t *s_loff = s_loff_all[wv], *s_xa = s_xa_all[wv], *s_xb = s_xb_all[wv], *s_da = s_da_all[wv];
     const int32_t cn = *cn_dev;
-    if ((int64_t)blockIdx.x * kCT >= cn) return;   // block-uniform
-    const int64_t cv = (int64_t)blockIdx.x * kCT + t;
-    const bool own = cv < cn;
-    int32_t ds = 0, xa = 0, xb = 0, da = 0;
-    bool big = false;
-    if (own) {
-        const int32_t a = members[2 * cv], b = members[2 * cv + 1];
-        ds = degsum[cv];
-        xa = xadj[a];
-        da = xadj[a + 1] - xa;
-        xb = b >= 0 ? xadj[b] : 0;
-        big = ds > kContractBig;
-        if (big) big_list[atomicAdd(&counters[2], 1)] = (int32_t)cv;
-    }
-    int total;
-    const int loff = block_excl_scan_c((own && !big) ? ds : 0, s_tmp, &total);
-    s_loff[t] = loff;
-    s_xa[t] = xa;
-    s_xb[t] = xb;
-    s_da[t] = da;
-    if (t == 0) s_loff[kCT] = total;
-    __syncthreads();
-    if (t == 0) {
-        int nb = 0, vs = 0;
-        while (vs < kCT) {
-            s_bstart[nb++] = vs;
-            int lo = vs + 1, hi = kCT;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_loff[mid] - s_loff[vs] <= kCCap) lo = mid;
-                else hi = mid - 1;
-            }
-            vs = lo;
-        }
-        s_bstart[nb] = kCT;
-        s_nb = nb;
-    }
-    __syncthreads();
+    const int ntiles = (cn + 63) / 64;
+    const int gw = blockIdx.x * kCWaves + wv, nwaves = gridDim.x * kCWaves;
     bool not_one = false;
-    const int nb = s_nb;
-    for (int bb = 0; bb < nb; ++bb) {
-        const int vs = s_bstart[bb], ve = s_bstart[bb + 1];
-        const int P0 = s_loff[vs], P1 = s_loff[ve];
-        int32_t uu[kCItems], ww[kCItems];
+    for (int tile = gw; tile < ntiles; tile += nwaves) {
+        const int64_t cv = (int64_t)tile * 64 + lane;
+        const bool own = cv < cn;
+        int32_t ds = 0, xa = 0, xb = 0, da = 0;
+        if (own) {
+            const int32_t a = members[2 * cv], b = members[2 * cv + 1];
+            ds = degsum[cv];
+            xa = xadj[a];
+            da = xadj[a + 1] - xa;
+            xb = b >= 0 ? xadj[b] : 0;
+        }
+        const bool big = own && ds > kContractBig;
+        const uint64_t bigm = __ballot(big);
+        if (bigm) {                                   // wave-aggregated append to the big list
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&counters[2], __popcll(bigm));
+            base = __shfl(base, 0, 64);
+            if (big) big_list[base + __popcll(bigm & ((1ull << lane) - 1ull))] = (int32_t)cv;
+        }
+        const int sds = (own && !big) ? ds : 0;
+        int total;
+        const int loff = wave_excl_scan_c(sds, &total);
+        s_loff[lane] = loff;
+        s_xa[lane] = xa;
+        s_xb[lane] = xb;
+        s_da[lane] = da;
+        if (lane == 63) s_loff[64] = total;
+        for (int vs = 0; vs < 64;) {
+            const int P0 = __shfl(loff, vs, 64);
+            const uint64_t fit = __ballot(lane >= vs && loff + sds - P0 <= kCCap);
+            const int ve = 64 - __builtin_clzll(fit);               // fit is lanes [vs, ve), ve > vs
+            const int P1 = __shfl(loff + sds, ve - 1, 64);
+            if (lane >= vs && lane < ve)
+                for (int e = 0; e < sds; ++e) s_own[loff - P0 + e] = (uint8_t)lane;
+            wave_sync();
+            int32_t uu[kCItems], ww[kCItems];
 #pragma unroll
-        for (int i = 0; i < kCItems; ++i) {
-            const int p = P0 + t + i * kCT;
-            uu[i] = 0;
-            ww[i] = 1;
-            if (p < P1) {
-                int lo = vs, hi = ve - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_loff[mid] <= p) lo = mid;
-                    else hi = mid - 1;
+            for (int i = 0; i < kCItems; ++i) {
+                const int p = P0 + lane + i * 64;
+                uu[i] = 0;
+                ww[i] = 1;
+                if (p < P1) {
+                    const int o = s_own[p - P0];
+                    const int32_t tl = p - s_loff[o];
+                    const int32_t j = tl < s_da[o] ? s_xa[o] + tl : s_xb[o] + (tl - s_da[o]);
+                    uu[i] = adjncy[j];
+                    if (adjwgt) ww[i] = adjwgt[j];
                 }
-                const int32_t tl = p - s_loff[lo];
-                const int32_t j = tl < s_da[lo] ? s_xa[lo] + tl : s_xb[lo] + (tl - s_da[lo]);
-                uu[i] = adjncy[j];
-                if (adjwgt) ww[i] = adjwgt[j];
             }
-        }
 #pragma unroll
-        for (int i = 0; i < kCItems; ++i) {
-            const int p = P0 + t + i * kCT;
-            if (p < P1) {
-                s_cu[p - P0] = cmap[uu[i]];
-                s_w[p - P0] = ww[i];
+            for (int i = 0; i < kCItems; ++i) {
+                const int p = P0 + lane + i * 64;
+                if (p < P1) {
+                    s_cu[p - P0] = cmap[uu[i]];
+                    s_w[p - P0] = ww[i];
+                }
             }
-        }
-        __syncthreads();
-        if (own && !big && t >= vs && t < ve) {
-            const int lb = loff - P0, le = lb + ds;
-            const int32_t me = (int32_t)cv;
-            for (int e = lb; e < le; ++e)
-                if (s_cu[e] == me) s_cu[e] = -1;                     // internal edge (graph/mod.rs:330-332)
-            const int32_t off = tmp_off[cv];
-            int32_t pos = 0;
-            for (int e = lb; e < le; ++e) {
-                const int32_t cu = s_cu[e];
-                if (cu < 0) continue;
-                uint32_t sum = (uint32_t)s_w[e];
-                for (int e2 = e + 1; e2 < le; ++e2)
-                    if (s_cu[e2] == cu) {                             // merge (graph/mod.rs:335-339)
-                        sum += (uint32_t)s_w[e2];
-                        s_cu[e2] = -1;
-                    }
-                tmp_adj[off + pos] = cu;
-                tmp_w[off + pos] = (int32_t)sum;
-                if ((int32_t)sum != 1) not_one = true;
-                ++pos;
+            wave_sync();
+            if (lane >= vs && lane < ve && own && !big) {
+                const int lb = loff - P0, le = lb + ds;
+                const int32_t me = (int32_t)cv;
+                for (int e = lb; e < le; ++e)
+                    if (s_cu[e] == me) s_cu[e] = -1;                     // internal edge (graph/mod.rs:330-332)
+                const int32_t off = tmp_off[cv];
+                int32_t pos = 0;
+                for (int e = lb; e < le; ++e) {
+                    const int32_t cu = s_cu[e];
+                    if (cu < 0) continue;
+                    uint32_t sum = (uint32_t)s_w[e];
+                    for (int e2 = e + 1; e2 < le; ++e2)
+                        if (s_cu[e2] == cu) {                             // merge (graph/mod.rs:335-339)
+                            sum += (uint32_t)s_w[e2];
+                            s_cu[e2] = -1;
+                        }
+                    tmp_adj[off + pos] = cu;
+                    tmp_w[off + pos] = (int32_t)sum;
+                    if ((int32_t)sum != 1) not_one = true;
+                    ++pos;
+                }
+                cdeg[cv] = pos;
             }
-            cdeg[cv] = pos;
+            wave_sync();
+            vs = ve;
         }
-        __syncthreads();
     }
-    if (__ballot(not_one) && __lane_id() == 0) atomicOr(&counters[4], 1);
+    if (__ballot(not_one) && lane == 0) atomicOr(&counters[4], 1);
 }
 
 __device__ inline int32_t ld_agent(const int32_t* p) {
@@ -433,10 +479,9 @@ inline int grid_cap(int64_t items, int per_block, int cap) {
 }
 
 template <int G>
-void launch_propose(const DevGraph& g, CoarsenBufs& b, uint64_t seed, int shem, const int32_t* prev,
-                    hipStream_t s) {
-    constexpr int NG = kBlock / G;
-    hipLaunchKernelGGL(k_match_propose<G>, dim3((g.n + NG - 1) / NG), dim3(kBlock), 0, s, g.n, g.xadj, g.adjncy,
+void launch_propose(const DevGraph& g, CoarsenBufs& b, int nseg, int seg, const int32_t* list, const int32_t* cnt,
+                    uint64_t seed, int shem, const int32_t* prev, hipStream_t s) {
+    hipLaunchKernelGGL(k_match_propose<G>, dim3(nseg), dim3(kBlock), 0, s, seg, list, cnt, g.xadj, g.adjncy,
                        g.adjwgt, b.match, b.cand, seed, shem, prev);
 }
 
@@ -446,20 +491,29 @@ void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, C
     if (g.n <= 0) return;
     if (rounds > kMaxMatchRounds) rounds = kMaxMatchRounds;
     const int nb = (g.n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_match_init, dim3(nb), dim3(kBlock), 0, s, g.n, g.xadj, b.match);
+    // one list segment per resident block (at most kMatchSegMax), each of seg vertices
+    int nseg = device_cus() * 8;
+    if (nseg > kMatchSegMax) nseg = kMatchSegMax;
+    if (nseg > nb) nseg = nb;
+    const int seg = (int)(((int64_t)g.n + nseg - 1) / nseg);
+    nseg = (int)(((int64_t)g.n + seg - 1) / seg);
     RK_HIP(hipMemsetAsync(b.round_flags, 0, sizeof(int32_t) * kMaxMatchRounds, s));
+    hipLaunchKernelGGL(k_match_init, dim3(nseg), dim3(kBlock), 0, s, g.n, seg, g.xadj, b.match, b.cand, b.mlist[0],
+                       b.seg_cnt);
     const int G = pick_group(g.n, g.m);
     for (int r = 0; r < rounds; ++r) {
         const int32_t* prev = r > 0 ? b.round_flags + (r - 1) : nullptr;
+        const int32_t* list = b.mlist[r & 1];
+        const int32_t* cnt = b.seg_cnt + (r & 1) * kMatchSegMax;
         switch (G) {
-            case 4: launch_propose<4>(g, b, seed, ctype_shem, prev, s); break;
-            case 8: launch_propose<8>(g, b, seed, ctype_shem, prev, s); break;
-            case 16: launch_propose<16>(g, b, seed, ctype_shem, prev, s); break;
-            case 32: launch_propose<32>(g, b, seed, ctype_shem, prev, s); break;
-            default: launch_propose<64>(g, b, seed, ctype_shem, prev, s); break;
+            case 4: launch_propose<4>(g, b, nseg, seg, list, cnt, seed, ctype_shem, prev, s); break;
+            case 8: launch_propose<8>(g, b, nseg, seg, list, cnt, seed, ctype_shem, prev, s); break;
+            case 16: launch_propose<16>(g, b, nseg, seg, list, cnt, seed, ctype_shem, prev, s); break;
+            case 32: launch_propose<32>(g, b, nseg, seg, list, cnt, seed, ctype_shem, prev, s); break;
+            default: launch_propose<64>(g, b, nseg, seg, list, cnt, seed, ctype_shem, prev, s); break;
         }
-        hipLaunchKernelGGL(k_match_accept, dim3(nb), dim3(kBlock), 0, s, g.n, b.cand, b.match,
-                           b.round_flags + r, prev);
+        hipLaunchKernelGGL(k_match_accept, dim3(nseg), dim3(kBlock), 0, s, seg, list, cnt, b.cand, b.match,
+                           b.mlist[(r + 1) & 1], b.seg_cnt + ((r + 1) & 1) * kMatchSegMax, b.round_flags + r, prev);
     }
     hipLaunchKernelGGL(k_match_finish, dim3(nb), dim3(kBlock), 0, s, g.n, b.match);
     RK_HIP(hipGetLastError());
@@ -487,9 +541,13 @@ ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, 
     hipLaunchKernelGGL(k_cmap, dim3(nb), dim3(kBlock), 0, s, n, g.ncon, g.xadj, g.vwgt, g.vsize, match, b.cid, cmap,
                        b.members, b.degsum, cg.vwgt, cg.vsize);
     exclusive_scan_i32(b.degsum, b.tmp_off, n, nullptr, b.scan_tmp, s);
-    hipLaunchKernelGGL(k_contract_tile, dim3((n + kCT - 1) / kCT), dim3(kCT), 0, s, cn_dev, g.xadj, g.adjncy,
-                       g.adjwgt, cmap, b.members, b.degsum, b.tmp_off, b.tmp_adj, b.tmp_w, b.cdeg, b.big_list,
-                       b.counters);
+    {
+        const int blocks = (n + 64 * kCWaves - 1) / (64 * kCWaves);
+        const int resident = device_cus() * kContractBlocksPerCU;
+        hipLaunchKernelGGL(k_contract_tile, dim3(blocks < resident ? blocks : resident), dim3(64 * kCWaves), 0, s,
+                           cn_dev, g.xadj, g.adjncy, g.adjwgt, cmap, b.members, b.degsum, b.tmp_off, b.tmp_adj,
+                           b.tmp_w, b.cdeg, b.big_list, b.counters);
+    }
     hipLaunchKernelGGL(k_contract_big, dim3(1024), dim3(kBlock), 0, s, g.xadj, g.adjncy, g.adjwgt, cmap,
                        b.members, b.degsum, b.tmp_off, b.tmp_adj, b.tmp_w, b.cdeg, b.big_list, b.counters,
                        b.hash_keys, b.hash_first, b.hash_sum);

@@ -143,6 +143,9 @@ Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon, h
     c.hash_sum = a.get<int32_t>(2 * (int64_t)m + n + 1);
     c.counters = a.get<int32_t>(16);
     c.round_flags = a.get<int32_t>(kMaxMatchRounds);
+    c.seg_cnt = a.get<int32_t>(2 * kMatchSegMax);
+    c.mlist[0] = a.get<int32_t>((int64_t)n + kMatchSegMax);
+    c.mlist[1] = a.get<int32_t>((int64_t)n + kMatchSegMax);
     c.scan_tmp = r.scan_tmp;
     B.cut_acc = a.get<unsigned long long>(1);
     B.cut_out = a.get<int32_t>(1);

@@ -91,10 +91,13 @@ struct CoarsenBufs {
     int32_t* hash_sum = nullptr;
     int32_t* counters = nullptr;   // [16]: 0 cn, 2 nbig, 4 weight-not-one flag, 6-7 hash bump (u64)
     int32_t* round_flags = nullptr;  // [kMaxMatchRounds] "round r changed the matching"
+    int32_t* seg_cnt = nullptr;      // [2 * kMatchSegMax] per-block list lengths, current / next round
+    int32_t* mlist[2] = {nullptr, nullptr};  // [n + kMatchSegMax] each: per-block lists of unmatched vertices
     void* scan_tmp = nullptr;
 };
 
 constexpr int kMaxMatchRounds = 64;
+constexpr int kMatchSegMax = 4096;   // matching blocks (list segments) at most
 constexpr int kContractBig = 64;   // coarse vertices of larger degree sum use the global-hash path
 
 // Deterministic seeded handshake matching (ctype 1 heavy edge, 0 random).
