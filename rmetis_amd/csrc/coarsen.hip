@@ -62,60 +62,61 @@ __device__ inline EdgeKey edge_key(int32_t v, int32_t u, int32_t w, int32_t dv, 
 // only, and compacts its survivors into its own next list with a block scan,
 // so no global counter is shared.  The list order does not affect the
 // matching: every decision depends on the match state alone.
-__global__ __launch_bounds__(kBlock) void k_match_init(int32_t n, int32_t seg, const int32_t* __restrict__ xadj,
-                                                       int32_t* __restrict__ match, int32_t* __restrict__ cand,
-                                                       int32_t* __restrict__ list, int32_t* __restrict__ seg_cnt) {
-    __shared__ int s_tmp[kBlock / 64];
-    const int64_t lo = (int64_t)blockIdx.x * seg, hi = min(lo + seg, (int64_t)n);
+struct MatchArgs {
+    int32_t n, seg, rounds, shem;
+    uint64_t seed;
+    const int32_t* xadj;
+    const int32_t* adjncy;
+    const int32_t* adjwgt;
+    int32_t* match;
+    int32_t* cand;
+    int32_t* mlist[2];
+    int32_t* seg_cnt;   // [2 * kMatchSegMax]
+    int32_t* flags;     // [kMaxMatchRounds] "round r changed the matching"
+};
+
+__device__ inline void dev_match_init(const MatchArgs& a, int blk, int* s_tmp) {
+    const int64_t lo = (int64_t)blk * a.seg, hi = min(lo + a.seg, (int64_t)a.n);
     int nout = 0;
     for (int64_t vb = lo; vb < hi; vb += kBlock) {
         const int64_t v = vb + threadIdx.x;
         bool open = false;
         if (v < hi) {
-            open = xadj[v + 1] != xadj[v];
-            match[v] = open ? -1 : (int32_t)v;
-            cand[v] = -1;
+            open = a.xadj[v + 1] != a.xadj[v];
+            a.match[v] = open ? -1 : (int32_t)v;
+            a.cand[v] = -1;
         }
         int tot;
         const int pos = block_flag_scan(open, s_tmp, &tot);
-        if (open) list[lo + nout + pos] = (int32_t)v;
+        if (open) a.mlist[0][lo + nout + pos] = (int32_t)v;
         nout += tot;
     }
-    if (threadIdx.x == 0) seg_cnt[blockIdx.x] = nout;
+    if (threadIdx.x == 0) a.seg_cnt[blk] = nout;
 }
 
-// Round r proposes for the unmatched vertices of each block's list and runs
-// only if round r-1 matched or retired something (prev_changed).  An open
-// vertex whose previous proposal is still unmatched proposes it again
-// without a rescan: its unmatched neighbours only ever lose members, so the
-// best of them stays the best while it is unmatched.
+// Proposals of round r for block blk's list.  An open vertex whose previous
+// proposal is still unmatched proposes it again without a rescan: its
+// unmatched neighbours only ever lose members, so the best of them stays the
+// best while it is unmatched.
 template <int G>
-__global__ __launch_bounds__(kBlock) void k_match_propose(int32_t seg, const int32_t* __restrict__ list,
-                                                          const int32_t* __restrict__ seg_cnt,
-                                                          const int32_t* __restrict__ xadj,
-                                                          const int32_t* __restrict__ adjncy,
-                                                          const int32_t* __restrict__ adjwgt,
-                                                          const int32_t* __restrict__ match,
-                                                          int32_t* __restrict__ cand, uint64_t seed, int shem,
-                                                          const int32_t* __restrict__ prev_changed) {
-    if (prev_changed && *prev_changed == 0) return;
+__device__ inline void dev_match_propose(const MatchArgs& a, int r, int blk) {
     constexpr int NG = kBlock / G;
     const int gl = group_lane<G>();
-    const int32_t nl = seg_cnt[blockIdx.x];
-    const int32_t* mine = list + (int64_t)blockIdx.x * seg;
+    const int32_t nl = a.seg_cnt[(r & 1) * kMatchSegMax + blk];
+    const int32_t* mine = a.mlist[r & 1] + (int64_t)blk * a.seg;
     for (int i = threadIdx.x / G; i < nl; i += NG) {
         const int32_t v = mine[i];
-        const int32_t prev = cand[v];
-        if (prev >= 0 && match[prev] == -1) continue;   // group-uniform
-        const int32_t beg = xadj[v], end = xadj[v + 1];
+        const int32_t prev = a.cand[v];
+        if (prev >= 0 && a.match[prev] == -1) continue;   // group-uniform
+        const int32_t beg = a.xadj[v], end = a.xadj[v + 1];
         const int32_t dv = end - beg;
         EdgeKey best{0ull, 0u};
         int32_t bu = -1;
         for (int32_t j = beg + gl; j < end; j += G) {
-            const int32_t u = adjncy[j];
-            if (match[u] != -1) continue;
-            const int32_t du = xadj[u + 1] - xadj[u];
-            const EdgeKey key = edge_key(v, u, adjwgt ? adjwgt[j] : 1, dv, du, seed, shem);
+            const int32_t u = a.adjncy[j];
+            if (a.match[u] != -1) continue;
+            const int32_t du = a.xadj[u + 1] - a.xadj[u];
+            const EdgeKey key = edge_key(v, u, a.adjwgt ? a.adjwgt[j] : 1, dv, du, a.seed, a.shem);
             if (bu < 0 || key.gt(best) || (key.eq(best) && u < bu)) {
                 best = key;
                 bu = u;
@@ -130,27 +131,17 @@ __global__ __launch_bounds__(kBlock) void k_match_propose(int32_t seg, const int
                 bu = ou;
             }
         }
-        if (gl == 0) cand[v] = bu >= 0 ? bu : -2;
+        if (gl == 0) a.cand[v] = bu >= 0 ? bu : -2;
     }
 }
 
-// Mutual proposals become pairs, vertices without an unmatched neighbour
-// (cand -2) retire; each block compacts the rest of its list into its next
-// list.
-__global__ __launch_bounds__(kBlock) void k_match_accept(int32_t seg, const int32_t* __restrict__ list,
-                                                         const int32_t* __restrict__ seg_cnt,
-                                                         const int32_t* __restrict__ cand,
-                                                         int32_t* __restrict__ match, int32_t* __restrict__ next,
-                                                         int32_t* __restrict__ next_cnt, int32_t* __restrict__ changed,
-                                                         const int32_t* __restrict__ prev_changed) {
-    __shared__ int s_tmp[kBlock / 64];
-    const int32_t nl = seg_cnt[blockIdx.x];
-    if (prev_changed && *prev_changed == 0) {
-        if (threadIdx.x == 0) next_cnt[blockIdx.x] = nl;   // list carried over unchanged (never read)
-        return;
-    }
-    const int32_t* mine = list + (int64_t)blockIdx.x * seg;
-    int32_t* out = next + (int64_t)blockIdx.x * seg;
+// Accepts of round r for block blk: mutual proposals become pairs, vertices
+// without an unmatched neighbour (cand -2) retire, the rest of the list is
+// compacted into the block's next list.  Sets flags[r] if anything changed.
+__device__ inline void dev_match_accept(const MatchArgs& a, int r, int blk, int* s_tmp) {
+    const int32_t nl = a.seg_cnt[(r & 1) * kMatchSegMax + blk];
+    const int32_t* mine = a.mlist[r & 1] + (int64_t)blk * a.seg;
+    int32_t* out = a.mlist[(r + 1) & 1] + (int64_t)blk * a.seg;
     bool ch = false;
     int nout = 0;
     for (int ib = 0; ib < nl; ib += kBlock) {
@@ -159,12 +150,12 @@ __global__ __launch_bounds__(kBlock) void k_match_accept(int32_t seg, const int3
         bool open = false;
         if (i < nl) {
             v = mine[i];
-            const int32_t u = cand[v];
+            const int32_t u = a.cand[v];
             if (u == -2) {
-                match[v] = v;
+                a.match[v] = v;
                 ch = true;
-            } else if (u >= 0 && cand[u] == v) {
-                match[v] = u;
+            } else if (u >= 0 && a.cand[u] == v) {
+                a.match[v] = u;
                 ch = true;
             } else {
                 open = true;
@@ -175,15 +166,37 @@ __global__ __launch_bounds__(kBlock) void k_match_accept(int32_t seg, const int3
         if (open) out[nout + pos] = v;
         nout += tot;
     }
-    if (threadIdx.x == 0) next_cnt[blockIdx.x] = nout;
+    if (threadIdx.x == 0) a.seg_cnt[((r + 1) & 1) * kMatchSegMax + blk] = nout;
     // one plain store per block that changed something (every writer stores 1)
-    if (__syncthreads_or(ch) && threadIdx.x == 0) *changed = 1;
+    if (__syncthreads_or(ch) && threadIdx.x == 0) a.flags[r] = 1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_match_finish(int32_t n, int32_t* __restrict__ match) {
-    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (v < n && match[v] == -1) match[v] = (int32_t)v;
+__device__ inline void dev_match_finish(const MatchArgs& a, int blk) {
+    const int64_t lo = (int64_t)blk * a.seg, hi = min(lo + a.seg, (int64_t)a.n);
+    for (int64_t v = lo + threadIdx.x; v < hi; v += kBlock)
+        if (a.match[v] == -1) a.match[v] = (int32_t)v;
 }
+
+// One launch per phase; round r runs only if round r-1 matched or retired
+// something.
+__global__ __launch_bounds__(kBlock) void k_match_init(MatchArgs a) {
+    __shared__ int s_tmp[kBlock / 64];
+    dev_match_init(a, blockIdx.x, s_tmp);
+}
+
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_match_propose(MatchArgs a, int r) {
+    if (r > 0 && a.flags[r - 1] == 0) return;
+    dev_match_propose<G>(a, r, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_match_accept(MatchArgs a, int r) {
+    __shared__ int s_tmp[kBlock / 64];
+    if (r > 0 && a.flags[r - 1] == 0) return;
+    dev_match_accept(a, r, blockIdx.x, s_tmp);
+}
+
+__global__ __launch_bounds__(kBlock) void k_match_finish_seg(MatchArgs a) { dev_match_finish(a, blockIdx.x); }
 
 __global__ __launch_bounds__(kBlock) void k_leader(int32_t n, const int32_t* __restrict__ match,
                                                    int32_t* __restrict__ leader) {
@@ -479,10 +492,13 @@ inline int grid_cap(int64_t items, int per_block, int cap) {
 }
 
 template <int G>
-void launch_propose(const DevGraph& g, CoarsenBufs& b, int nseg, int seg, const int32_t* list, const int32_t* cnt,
-                    uint64_t seed, int shem, const int32_t* prev, hipStream_t s) {
-    hipLaunchKernelGGL(k_match_propose<G>, dim3(nseg), dim3(kBlock), 0, s, seg, list, cnt, g.xadj, g.adjncy,
-                       g.adjwgt, b.match, b.cand, seed, shem, prev);
+void match_rounds(MatchArgs& a, int nseg, hipStream_t s) {
+    hipLaunchKernelGGL(k_match_init, dim3(nseg), dim3(kBlock), 0, s, a);
+    for (int r = 0; r < a.rounds; ++r) {
+        hipLaunchKernelGGL(k_match_propose<G>, dim3(nseg), dim3(kBlock), 0, s, a, r);
+        hipLaunchKernelGGL(k_match_accept, dim3(nseg), dim3(kBlock), 0, s, a, r);
+    }
+    hipLaunchKernelGGL(k_match_finish_seg, dim3(nseg), dim3(kBlock), 0, s, a);
 }
 
 }  // namespace
@@ -498,24 +514,15 @@ void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, C
     const int seg = (int)(((int64_t)g.n + nseg - 1) / nseg);
     nseg = (int)(((int64_t)g.n + seg - 1) / seg);
     RK_HIP(hipMemsetAsync(b.round_flags, 0, sizeof(int32_t) * kMaxMatchRounds, s));
-    hipLaunchKernelGGL(k_match_init, dim3(nseg), dim3(kBlock), 0, s, g.n, seg, g.xadj, b.match, b.cand, b.mlist[0],
-                       b.seg_cnt);
-    const int G = pick_group(g.n, g.m);
-    for (int r = 0; r < rounds; ++r) {
-        const int32_t* prev = r > 0 ? b.round_flags + (r - 1) : nullptr;
-        const int32_t* list = b.mlist[r & 1];
-        const int32_t* cnt = b.seg_cnt + (r & 1) * kMatchSegMax;
-        switch (G) {
-            case 4: launch_propose<4>(g, b, nseg, seg, list, cnt, seed, ctype_shem, prev, s); break;
-            case 8: launch_propose<8>(g, b, nseg, seg, list, cnt, seed, ctype_shem, prev, s); break;
-            case 16: launch_propose<16>(g, b, nseg, seg, list, cnt, seed, ctype_shem, prev, s); break;
-            case 32: launch_propose<32>(g, b, nseg, seg, list, cnt, seed, ctype_shem, prev, s); break;
-            default: launch_propose<64>(g, b, nseg, seg, list, cnt, seed, ctype_shem, prev, s); break;
-        }
-        hipLaunchKernelGGL(k_match_accept, dim3(nseg), dim3(kBlock), 0, s, seg, list, cnt, b.cand, b.match,
-                           b.mlist[(r + 1) & 1], b.seg_cnt + ((r + 1) & 1) * kMatchSegMax, b.round_flags + r, prev);
+    MatchArgs a{g.n, seg, rounds, ctype_shem, seed, g.xadj, g.adjncy, g.adjwgt, b.match, b.cand,
+                {b.mlist[0], b.mlist[1]}, b.seg_cnt, b.round_flags};
+    switch (pick_group(g.n, g.m)) {
+        case 4: match_rounds<4>(a, nseg, s); break;
+        case 8: match_rounds<8>(a, nseg, s); break;
+        case 16: match_rounds<16>(a, nseg, s); break;
+        case 32: match_rounds<32>(a, nseg, s); break;
+        default: match_rounds<64>(a, nseg, s); break;
     }
-    hipLaunchKernelGGL(k_match_finish, dim3(nb), dim3(kBlock), 0, s, g.n, b.match);
     RK_HIP(hipGetLastError());
 }
 
