@@ -324,10 +324,10 @@ struct RegSlack {
     }
 };
 
-// Walk window slots [s0, s0+cnt) (cnt <= 64) with wave 0: applies each move
-// to the running slack S and writes each slot's decision to w.dst.
+// Walk window slots [s0 + first, s0 + cnt) (cnt <= 64) with wave 0: applies
+// each move to the running slack S and writes each slot's decision to w.dst.
 template <class S>
-RK_INL void resolve_chunk(const Ctx& x, S& st, int s0, int cnt) {
+RK_INL void resolve_chunk(const Ctx& x, S& st, int s0, int cnt, int first = 0) {
     const int lane = __lane_id();
     const int s = s0 + lane;
     int32_t v = 0, pv = 0, nc = 0, nbz = 0, off = 0, w0 = 0, w1 = 0;
@@ -344,8 +344,8 @@ RK_INL void resolve_chunk(const Ctx& x, S& st, int s0, int cnt) {
     int32_t mine = kNoDest;
     // the staged candidates of the next mover are read one step ahead (they
     // do not depend on the slack); the pool is read at use, when nc > kCL
-    int32_t qs = x.w.cl[s0 * kCL + (lane & (kCL - 1))];
-    for (int i = 0; i < cnt; ++i) {
+    int32_t qs = x.w.cl[(s0 + first) * kCL + (lane & (kCL - 1))];
+    for (int i = first; i < cnt; ++i) {
         const int32_t vi = __builtin_amdgcn_readlane(v, i);
         const int32_t pvi = __builtin_amdgcn_readlane(pv, i);
         const int32_t w0i = __builtin_amdgcn_readlane(w0, i);
@@ -360,14 +360,17 @@ RK_INL void resolve_chunk(const Ctx& x, S& st, int s0, int cnt) {
         if (lane == i) mine = di;
         if (di >= 0) st.apply(x, vi, di, pvi, w0i, w1i);
     }
-    if (lane < cnt) x.w.dst[s] = mine;
+    if (lane >= first && lane < cnt) x.w.dst[s] = mine;
 }
 
 constexpr int kSpecB = 8;    // vertex weights below 2^kSpecB in a speculative chunk
+constexpr int kSpecRounds = 8;   // speculative rounds per chunk; the walk finishes the rest
 
 // Exact speculative resolution of window slots [s0, s0+cnt) (cnt <= 64) by
 // wave 0 in the cut-optimisation pass, for ncon == NC <= 2, lists of at most
-// kCL candidates and no zero group; returns false (nothing done) otherwise.
+// kCL candidates and no zero group.  Returns the number of leading slots it
+// resolved: 0 when the chunk does not qualify, fewer than cnt when
+// kSpecRounds rounds did not finish it (the walk takes over from there).
 //
 // Rounds start at the first unresolved lane f, whose prefix is exact.  Every
 // lane >= f decides against the slack at f; the lanes that move set their bit
@@ -379,9 +382,10 @@ constexpr int kSpecB = 8;    // vertex weights below 2^kSpecB in a speculative c
 // verdict under that slack, its guess is the sequential decision (given that
 // the lanes before it are).  The lanes before the first failing one u commit
 // their moves; the next round starts at u, so a round always commits at least
-// one lane and a chunk takes one round per wrong guess.
+// one lane and a chunk takes one round per wrong guess.  Committed moves are
+// in the LDS slack, so the walk can continue from the first unresolved slot.
 template <int NC>
-RK_INL bool spec_chunk(const Ctx& x, uint64_t* m_in, uint64_t* m_out, int s0, int cnt) {
+RK_INL int spec_chunk(const Ctx& x, uint64_t* m_in, uint64_t* m_out, int s0, int cnt) {
     const int lane = __lane_id();
     const int s = s0 + lane;
     const bool act = lane < cnt;
@@ -394,7 +398,7 @@ RK_INL bool spec_chunk(const Ctx& x, uint64_t* m_in, uint64_t* m_out, int s0, in
         w0 = x.w.w0[s];
         if (NC > 1) w1 = x.w.w1[s];
     }
-    if (__ballot(act && (nc > kCL || nbz < 0 || (uint32_t)(w0 | w1) >= (1u << kSpecB)))) return false;
+    if (__ballot(act && (nc > kCL || nbz < 0 || (uint32_t)(w0 | w1) >= (1u << kSpecB)))) return 0;
     // bit planes of the weights: P0[b] / P1[b] = lanes whose weight has bit b
     int B = 0;
     uint64_t P0[kSpecB], P1[kSpecB];
@@ -415,7 +419,8 @@ RK_INL bool spec_chunk(const Ctx& x, uint64_t* m_in, uint64_t* m_out, int s0, in
     int32_t* S = x.slack;
     int32_t d = kNoDest;
     const uint64_t bit = 1ull << lane, below = bit - 1ull;
-    for (int f = 0; f < cnt;) {
+    int f = 0;
+    for (int round = 0; f < cnt && round < kSpecRounds; ++round) {
         const bool live = act && lane >= f;
         // 1. guess against the slack at f
         int32_t sv0[kCL], sv1[kCL];
@@ -484,8 +489,8 @@ RK_INL bool spec_chunk(const Ctx& x, uint64_t* m_in, uint64_t* m_out, int s0, in
         }
         f = u;
     }
-    if (act) x.w.dst[s] = d;
-    return true;
+    if (act && lane < f) x.w.dst[s] = d;
+    return f;
 }
 
 // L: the k*ncon slack/threshold tables live in LDS (k*ncon <= kLdsKC);
@@ -617,11 +622,13 @@ __global__ __launch_bounds__(kAT) void k_admit(AdmArgs a) {
         if ((tid >> 6) == 0)
             for (int s0 = 0; s0 < wn; s0 += 64) {
                 const int cnt = min(64, wn - s0);
+                int first = 0;
                 if constexpr (NC > 0)
-                    if (a.mode == 1 && spec_chunk<NC>(x, m_in, m_out, s0, cnt)) continue;
+                    if (a.mode == 1) first = spec_chunk<NC>(x, m_in, m_out, s0, cnt);
+                if (first == cnt) continue;
                 Slack st;
                 st.load(x);
-                resolve_chunk(x, st, s0, cnt);
+                resolve_chunk(x, st, s0, cnt, first);
                 st.store(x);
             }
         __syncthreads();

@@ -269,7 +269,7 @@ int32_t rmetis_hip_gain_table(idx_t nvtxs, const idx_t* xadj, const idx_t* adjnc
         ContextGuard cx;
         DevGraph g = upload_graph(*cx.c, nvtxs, 1, xadj, adjncy, nullptr, adjwgt, nullptr);
         int32_t* dp = upload(*cx.c, part, nvtxs);
-        Buffers B = alloc_buffers(cx->arena, g.n, g.m, nparts, 1, cx->stream);
+        Buffers B = alloc_buffers(*cx.c, g.n, g.m, nparts, 1);
         int4* vrec = cx->arena.get<int4>(nvtxs);
         int32_t* dg0 = cx->arena.get<int32_t>(nvtxs);
         int32_t* gains = cx->arena.get<int32_t>(g.m);
@@ -304,7 +304,7 @@ int32_t rmetis_hip_greedy_pass(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const
         hipStream_t s = cx->stream;
         DevGraph g = upload_graph(*cx.c, nvtxs, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
         int32_t* dp = upload(*cx.c, part, nvtxs);
-        Buffers B = alloc_buffers(cx->arena, g.n, g.m, nparts, ncon, cx->stream);
+        Buffers B = alloc_buffers(*cx.c, g.n, g.m, nparts, ncon);
         const size_t kc = (size_t)nparts * ncon;
         std::vector<long long> fm(kc), om(kc);
         thresholds_from_targets(nparts, ncon, tpwgts, ubvec, fm.data(), om.data());
@@ -341,7 +341,7 @@ int32_t rmetis_hip_greedy_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, con
         hipStream_t s = cx->stream;
         DevGraph g = upload_graph(*cx.c, nvtxs, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
         int32_t* dp = upload(*cx.c, part, nvtxs);
-        Buffers B = alloc_buffers(cx->arena, g.n, g.m, nparts, ncon, cx->stream);
+        Buffers B = alloc_buffers(*cx.c, g.n, g.m, nparts, ncon);
         const size_t kc = (size_t)nparts * ncon;
         std::vector<long long> fm(kc), om(kc);
         thresholds_from_targets(nparts, ncon, tpwgts, ubvec, fm.data(), om.data());
@@ -367,7 +367,7 @@ int32_t rmetis_hip_match(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, co
         if (!validate_csr(nvtxs, 1, xadj, adjncy, nullptr)) return METIS_ERROR_INPUT;
         ContextGuard cx;
         DevGraph g = upload_graph(*cx.c, nvtxs, 1, xadj, adjncy, nullptr, adjwgt, nullptr);
-        Buffers B = alloc_buffers(cx->arena, g.n, g.m, 2, 1, cx->stream);
+        Buffers B = alloc_buffers(*cx.c, g.n, g.m, 2, 1);
         match_graph(g, ctype != 0, seed, 24, B.cb, cx->stream);
         download(*cx.c, matching, B.cb.match, nvtxs);
         RK_HIP(hipStreamSynchronize(cx->stream));
@@ -391,7 +391,7 @@ int32_t rmetis_hip_build_coarse_graph(idx_t nvtxs, idx_t ncon, const idx_t* xadj
         ContextGuard cx;
         hipStream_t s = cx->stream;
         DevGraph g = upload_graph(*cx.c, nvtxs, ncon, xadj, adjncy, vwgt, adjwgt, vsize);
-        Buffers B = alloc_buffers(cx->arena, g.n, g.m, 2, ncon, cx->stream);
+        Buffers B = alloc_buffers(*cx.c, g.n, g.m, 2, ncon);
         int32_t* dm = upload(*cx.c, matching, nvtxs);
         int32_t* dcmap = cx->arena.get<int32_t>(nvtxs);
         struct A : CoarseAlloc {

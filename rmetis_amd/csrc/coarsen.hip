@@ -469,10 +469,12 @@ __global__ __launch_bounds__(kBlock) void k_contract_copy(int32_t n_max, const i
                                                           const int32_t* __restrict__ tmp_adj,
                                                           const int32_t* __restrict__ tmp_w,
                                                           const int32_t* __restrict__ cxadj,
-                                                          int32_t* __restrict__ cadj, int32_t* __restrict__ cw) {
+                                                          int32_t* __restrict__ cadj, int32_t* __restrict__ cw,
+                                                          int32_t* __restrict__ counters) {
     constexpr int G = 16, NG = kBlock / G;
     const int gl = group_lane<G>();
     const int32_t cn = *cn_dev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[1] = cxadj[n_max];   // cm, read with the other counters
     for (int64_t cv = (int64_t)blockIdx.x * NG + threadIdx.x / G; cv < cn; cv += (int64_t)gridDim.x * NG) {
         const int32_t src = tmp_off[cv];
         const int32_t dst = cxadj[cv];
@@ -560,13 +562,11 @@ ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, 
                        b.hash_keys, b.hash_first, b.hash_sum);
     exclusive_scan_i32(b.cdeg, cg.xadj, n, cg.xadj + n, b.scan_tmp, s);   // cxadj[cn..n] = cm
     hipLaunchKernelGGL(k_contract_copy, dim3(grid_cap(n, kBlock / 16, 65536)), dim3(kBlock), 0, s, n, cn_dev,
-                       b.tmp_off, b.tmp_adj, b.tmp_w, cg.xadj, cg.adjncy, cg.adjwgt);
+                       b.tmp_off, b.tmp_adj, b.tmp_w, cg.xadj, cg.adjncy, cg.adjwgt, b.counters);
     if (heavy_list) heavy_list_build(n, cn_dev, cg.xadj, heavy_list, b.counters + 8, s);
-    int32_t hc[4] = {0, 0, 0, 0};
-    RK_HIP(hipMemcpyAsync(hc + 0, cn_dev, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    RK_HIP(hipMemcpyAsync(hc + 1, cg.xadj + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    RK_HIP(hipMemcpyAsync(hc + 2, b.counters + 4, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    RK_HIP(hipMemcpyAsync(hc + 3, b.counters + 8, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    // one read of counters[0..8] (cn, cm, ..., the weight flag, nheavy)
+    int32_t* hc = b.host_small;
+    RK_HIP(hipMemcpyAsync(hc, b.counters, 9 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     RK_HIP(hipStreamSynchronize(s));
     RK_HIP(hipGetLastError());
     cg.n = hc[0];
@@ -574,8 +574,8 @@ ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, 
     ContractResult r;
     r.cn = hc[0];
     r.cm = hc[1];
-    r.adjwgt_not_one = hc[2] != 0;
-    r.nheavy = heavy_list ? hc[3] : 0;
+    r.adjwgt_not_one = hc[4] != 0;
+    r.nheavy = heavy_list ? hc[8] : 0;
     return r;
 }
 

@@ -80,6 +80,13 @@ struct Context {
 
     void* pinned2 = nullptr;
     size_t pinned2_size = 0;
+    int32_t* small = nullptr;   // pinned scratch for device-to-host reads of a few counters
+
+    int32_t* host_small() {
+        if (!small) RK_HIP(hipHostMalloc((void**)&small, kSmallBytes, hipHostMallocDefault));
+        return small;
+    }
+    static constexpr size_t kSmallBytes = 4096;
 
     // Two independent pinned staging buffers; a buffer may be reused only
     // after the stream has been synchronised past its last copy.
@@ -98,6 +105,7 @@ struct Context {
     ~Context() {
         if (pinned) (void)hipHostFree(pinned);
         if (pinned2) (void)hipHostFree(pinned2);
+        if (small) (void)hipHostFree(small);
         arena.release();
         if (stream) (void)hipStreamDestroy(stream);
     }

@@ -100,7 +100,9 @@ void thresholds_from_targets(int32_t k, int32_t ncon, const double* tpw, const d
         }
 }
 
-Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon, hipStream_t s) {
+Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon) {
+    Arena& a = cx.arena;
+    const hipStream_t s = cx.stream;
     Buffers B;
     const int64_t ntiles = ((int64_t)n + kTile - 1) / kTile;
     const int64_t kc = (int64_t)k * ncon;
@@ -125,6 +127,7 @@ Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon, h
     r.nmoves = cnt + 2;
     r.done = cnt + 3;
     r.passes = cnt + 4;
+    r.host_small = cx.host_small();
     r.scan_tmp = a.alloc(scan_temp_bytes(std::max<int64_t>(ntiles, n)));
     CoarsenBufs& c = B.cb;
     c.match = a.get<int32_t>(n);
@@ -144,6 +147,7 @@ Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon, h
     c.counters = a.get<int32_t>(16);
     c.round_flags = a.get<int32_t>(kMaxMatchRounds);
     c.seg_cnt = a.get<int32_t>(2 * kMatchSegMax);
+    c.host_small = r.host_small;
     c.mlist[0] = a.get<int32_t>((int64_t)n + kMatchSegMax);
     c.mlist[1] = a.get<int32_t>((int64_t)n + kMatchSegMax);
     c.scan_tmp = r.scan_tmp;
@@ -283,7 +287,7 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
                         "(Greedy)");
     hipStream_t s = cx.stream;
     const int32_t ncon = g.ncon;
-    Buffers B = alloc_buffers(cx.arena, g.n, g.m, k, ncon, cx.stream);
+    Buffers B = alloc_buffers(cx, g.n, g.m, k, ncon);
     const std::vector<int32_t> total = total_weight_checked(cx, g, B);
     int32_t* fin_heavy = cx.arena.get<int32_t>(g.n);
     const int32_t fin_nheavy = heavy_list_sync(g, fin_heavy, B.rb, s);
@@ -367,14 +371,17 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
         RK_HIP(hipMemcpyAsync(d_thr, stage, 2 * kc * nl * sizeof(long long), hipMemcpyHostToDevice, s));
         // initial partition of the coarsest level on the host (kway.rs:66-70)
         auto t0 = std::chrono::steady_clock::now();
-        std::vector<int32_t> hx(cg.n + 1), ha(std::max(cg.m, 1)), hv, hp(cg.n);
-        RK_HIP(hipMemcpyAsync(hx.data(), cg.xadj, sizeof(int32_t) * (cg.n + 1), hipMemcpyDeviceToHost, s));
-        RK_HIP(hipMemcpyAsync(ha.data(), cg.adjncy, sizeof(int32_t) * cg.m, hipMemcpyDeviceToHost, s));
-        if (cg.vwgt) {
-            hv.resize((size_t)cg.n * ncon);
-            RK_HIP(hipMemcpyAsync(hv.data(), cg.vwgt, sizeof(int32_t) * hv.size(), hipMemcpyDeviceToHost, s));
-        }
+        // coarsest CSR (and weights) through pinned staging in one synchronisation
+        const size_t nvw = cg.vwgt ? (size_t)cg.n * ncon : 0;
+        int32_t* hb = (int32_t*)cx.host_stage2(sizeof(int32_t) * ((size_t)cg.n + 1 + (size_t)cg.m + nvw));
+        RK_HIP(hipMemcpyAsync(hb, cg.xadj, sizeof(int32_t) * (cg.n + 1), hipMemcpyDeviceToHost, s));
+        RK_HIP(hipMemcpyAsync(hb + cg.n + 1, cg.adjncy, sizeof(int32_t) * cg.m, hipMemcpyDeviceToHost, s));
+        if (nvw)
+            RK_HIP(hipMemcpyAsync(hb + cg.n + 1 + cg.m, cg.vwgt, sizeof(int32_t) * nvw, hipMemcpyDeviceToHost, s));
         RK_HIP(hipStreamSynchronize(s));
+        std::vector<int32_t> hx(hb, hb + cg.n + 1), ha(hb + cg.n + 1, hb + cg.n + 1 + cg.m), hv, hp(cg.n);
+        if (ha.empty()) ha.resize(1);
+        if (nvw) hv.assign(hb + cg.n + 1 + cg.m, hb + cg.n + 1 + cg.m + nvw);
         for (int c = 0; c < ncon; ++c) ltot[c] = cg.vwgt ? total[c] : cg.n;
         std::vector<double> tgt(kc);
         for (int32_t p = 0; p < k; ++p)
@@ -412,9 +419,9 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
             pc = pf;
         }
         edge_cut(g, pc, B.cut_acc, B.cut_out, pick_group(g.n, g.m), s);    // kway.rs:94
-        int32_t obj = 0;
-        RK_HIP(hipMemcpyAsync(&obj, B.cut_out, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        RK_HIP(hipMemcpyAsync(B.rb.host_small, B.cut_out, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         RK_HIP(hipStreamSynchronize(s));
+        const int32_t obj = B.rb.host_small[0];
         if (!have_best || obj < best) {                                 // kway.rs:96-98
             have_best = true;
             best = obj;

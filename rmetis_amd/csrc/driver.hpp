@@ -61,8 +61,9 @@ struct Buffers {
 };
 
 // Allocates the per-call work buffers for a finest level of n vertices,
-// m entries, k parts, ncon constraints (the tile bitmap is cleared on s).
-Buffers alloc_buffers(Arena& a, int32_t n, int32_t m, int32_t k, int32_t ncon, hipStream_t s);
+// m entries, k parts, ncon constraints (the tile bitmap is cleared on the
+// context's stream).
+Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon);
 
 // Column sums of vwgt (or n per constraint when vwgt is absent), checked
 // against the int32 range the reference's sums live in.

@@ -45,6 +45,7 @@ struct RefineBufs {
     int32_t* nmoves = nullptr;
     int32_t* done = nullptr;
     int32_t* passes = nullptr;     // count of executed passes (statistics)
+    int32_t* host_small = nullptr; // pinned host scratch for counter reads
     void* scan_tmp = nullptr;
 };
 
@@ -89,9 +90,10 @@ struct CoarsenBufs {
     int32_t* hash_keys = nullptr;  // [2m] global hash tables of the big path
     int32_t* hash_first = nullptr;
     int32_t* hash_sum = nullptr;
-    int32_t* counters = nullptr;   // [16]: 0 cn, 2 nbig, 4 weight-not-one flag, 6-7 hash bump (u64)
+    int32_t* counters = nullptr;   // [16]: 0 cn, 1 cm, 2 nbig, 4 weight-not-one flag, 6-7 hash bump (u64), 8 nheavy
     int32_t* round_flags = nullptr;  // [kMaxMatchRounds] "round r changed the matching"
     int32_t* seg_cnt = nullptr;      // [2 * kMatchSegMax] per-block list lengths, current / next round
+    int32_t* host_small = nullptr;   // pinned host scratch for the per-level counter read
     int32_t* mlist[2] = {nullptr, nullptr};  // [n + kMatchSegMax] each: per-block lists of unmatched vertices
     void* scan_tmp = nullptr;
 };

@@ -382,10 +382,9 @@ void heavy_list_build(int32_t n_max, const int32_t* n_dev, const int32_t* xadj, 
 
 int heavy_list_sync(const DevGraph& g, int32_t* list, RefineBufs& b, hipStream_t s) {
     heavy_list_build(g.n, nullptr, g.xadj, list, b.heavy_cnt, s);
-    int32_t nh = 0;
-    RK_HIP(hipMemcpyAsync(&nh, b.heavy_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    RK_HIP(hipMemcpyAsync(b.host_small, b.heavy_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     RK_HIP(hipStreamSynchronize(s));
-    return nh;
+    return b.host_small[0];
 }
 
 void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32_t* heavy_list, RefineBufs& b,
