@@ -44,26 +44,6 @@ constexpr int kCntWin = 2048;   // tiles with movers gathered per window
 constexpr int kNoDest = -1;
 constexpr size_t kLdsBytes = 64 * 1024;   // dynamic LDS of one admission workgroup
 
-struct AdmArgs {
-    int32_t mode, k, ncon, ntiles;
-    const int32_t* tile_cnt;
-    uint32_t* tile_bits;         // bit per tile with movers (cleared here)
-    int32_t* tile_list;          // [ntiles] ordered ids of those tiles
-    const int4* tile_rec;
-    int4* mov;
-    const int32_t* pool;
-    const int32_t* xadj;
-    const int32_t* vwgt;
-    const long long* fmax;
-    const long long* omax;
-    int32_t* pw;
-    int32_t* part;
-    int2* moves;
-    int32_t* nmoves;
-    int32_t* done;
-    int32_t* passes;
-    int32_t* gscratch;   // 3*k*ncon ints when k*ncon > kLdsKC
-};
 
 // LDS layout of the mover window.
 struct Win {
@@ -108,7 +88,7 @@ RK_INL int block_excl_scan_int(int x, int* s_w, int* total) {
 // LDS-resident instantiation, these pointers keep the LDS address space
 // (ds_* accesses waited on with lgkmcnt only).
 struct Ctx {
-    const AdmArgs* a;
+    const AdmitArgs* a;
     int32_t* slack;       // fmax - pw, running
     const int32_t* fm;    // clamped fmax
     const int32_t* om;    // clamped omax
@@ -116,7 +96,7 @@ struct Ctx {
 };
 
 // Weight of mover v in constraint c; constraints 0 and 1 come staged.
-RK_INL int32_t vw_c(const AdmArgs& a, int32_t v, int c, int32_t w0, int32_t w1) {
+RK_INL int32_t vw_c(const AdmitArgs& a, int32_t v, int c, int32_t w0, int32_t w1) {
     return c == 0 ? w0 : (c == 1 ? w1 : a.vwgt[(int64_t)v * a.ncon + c]);
 }
 
@@ -148,7 +128,7 @@ RK_INL uint64_t low_mask(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull);
 // q0 holds the mover's first 64 candidates, one per lane (-1 past the list).
 RK_INL int decide_table(const Ctx& x, int s, int32_t v, int32_t pv, int32_t nc, int32_t nbz, int32_t off,
                         int32_t w0, int32_t w1, int32_t q0) {
-    const AdmArgs& a = *x.a;
+    const AdmitArgs& a = *x.a;
     const int lane = __lane_id();
     const int ncon = a.ncon;
     if (a.mode == 0) {                                               // greedy.rs:69-74
@@ -257,7 +237,7 @@ struct RegSlack {
     // readlane and a bit test, and most movers stop at the first one.
     RK_INL int decide(const Ctx& x, int s, int32_t, int32_t pv, int32_t nc, int32_t nbz, int32_t off, int32_t w0,
                       int32_t w1, int32_t q0) const {
-        const AdmArgs& a = *x.a;
+        const AdmitArgs& a = *x.a;
         if (a.mode == 0) {                                           // greedy.rs:69-74
             bool over = (long long)x.fm[pv * NC] - at(pv, 0) > (long long)x.om[pv * NC];
             if (NC > 1) over = over || (long long)x.fm[pv * NC + 1] - at(pv, 1) > (long long)x.om[pv * NC + 1];
@@ -498,7 +478,13 @@ RK_INL int spec_chunk(const Ctx& x, uint64_t* m_in, uint64_t* m_out, int s0, int
 // registers (k <= 64*KP, ncon == NC <= 2).  NC > 0: cut-optimisation chunks
 // are resolved speculatively where spec_chunk applies.
 template <bool L, int KP, int NC>
-__global__ __launch_bounds__(kAT) void k_admit(AdmArgs a) {
+__global__ __launch_bounds__(kAT) void k_admit(AdmitArgs arg, const AdmitArgs* __restrict__ table) {
+    // a batched level passes one argument block per graph (workgroup)
+    AdmitArgs a = arg;
+    if (table) {
+        a = table[blockIdx.x];
+        a.mode = arg.mode;
+    }
     static_assert(KP == 0 || L, "register slack needs the LDS tables");
     using Slack = std::conditional_t<(KP > 0), RegSlack<(KP > 0 ? KP : 1), NC>, TableSlack<L>>;
     extern __shared__ int32_t s_dyn[];
@@ -654,34 +640,52 @@ __global__ __launch_bounds__(kAT) void k_admit(AdmArgs a) {
 
 }  // namespace
 
-void admit_pass(const DevGraph& g, int32_t k, int mode, int ntiles, RefineBufs& b, int32_t* part, hipStream_t s) {
-    AdmArgs aa{mode, k, g.ncon, ntiles, b.tile_cnt, b.tile_bits, b.tile_off, b.tile_rec, b.mov, b.pool, g.xadj, g.vwgt, b.fmax, b.omax,
-               b.pw, part, b.moves, b.nmoves, b.done, b.passes, b.adm_scratch};
-    const int kc = k * g.ncon;
+namespace {
+
+// Launches k_admit for graphs of k*ncon = kc at most: one workgroup with the
+// arguments aa, or one per graph of a batched level with arguments from table.
+void launch_admit(const AdmitArgs& aa, const AdmitArgs* table, int grid, int32_t k, int32_t ncon, hipStream_t s) {
+    const int kc = k * ncon;
     const size_t win = (size_t)kWinInts * sizeof(int32_t);
     const size_t cntb = 2 * (size_t)kCntWin * sizeof(int32_t);
     const size_t uni = win > cntb ? win : cntb;
     if (kc <= kLdsKC) {
         const size_t base = (size_t)((3 * kc + 3) & ~3) * sizeof(int32_t) + uni;
         // the speculative path's part masks ride along while LDS stays within 64 KiB
-        const int nc2 = g.ncon <= 2 && base + 16 * (size_t)k <= kLdsBytes ? g.ncon : 0;
+        const int nc2 = ncon <= 2 && base + 16 * (size_t)k <= kLdsBytes ? ncon : 0;
         const size_t lds = base + (nc2 ? 16 * (size_t)k : 0);
         const int kp = nc2 == 0 ? 0 : (k <= 64 ? 1 : (k <= 128 ? 2 : (k <= 256 ? 4 : 0)));
         switch (kp * 4 + nc2) {
-        case 4 + 1: hipLaunchKernelGGL((k_admit<true, 1, 1>), dim3(1), dim3(kAT), lds, s, aa); break;
-        case 4 + 2: hipLaunchKernelGGL((k_admit<true, 1, 2>), dim3(1), dim3(kAT), lds, s, aa); break;
-        case 8 + 1: hipLaunchKernelGGL((k_admit<true, 2, 1>), dim3(1), dim3(kAT), lds, s, aa); break;
-        case 8 + 2: hipLaunchKernelGGL((k_admit<true, 2, 2>), dim3(1), dim3(kAT), lds, s, aa); break;
-        case 16 + 1: hipLaunchKernelGGL((k_admit<true, 4, 1>), dim3(1), dim3(kAT), lds, s, aa); break;
-        case 16 + 2: hipLaunchKernelGGL((k_admit<true, 4, 2>), dim3(1), dim3(kAT), lds, s, aa); break;
-        case 1: hipLaunchKernelGGL((k_admit<true, 0, 1>), dim3(1), dim3(kAT), lds, s, aa); break;
-        case 2: hipLaunchKernelGGL((k_admit<true, 0, 2>), dim3(1), dim3(kAT), lds, s, aa); break;
-        default: hipLaunchKernelGGL((k_admit<true, 0, 0>), dim3(1), dim3(kAT), lds, s, aa); break;
+        case 4 + 1: hipLaunchKernelGGL((k_admit<true, 1, 1>), dim3(grid), dim3(kAT), lds, s, aa, table); break;
+        case 4 + 2: hipLaunchKernelGGL((k_admit<true, 1, 2>), dim3(grid), dim3(kAT), lds, s, aa, table); break;
+        case 8 + 1: hipLaunchKernelGGL((k_admit<true, 2, 1>), dim3(grid), dim3(kAT), lds, s, aa, table); break;
+        case 8 + 2: hipLaunchKernelGGL((k_admit<true, 2, 2>), dim3(grid), dim3(kAT), lds, s, aa, table); break;
+        case 16 + 1: hipLaunchKernelGGL((k_admit<true, 4, 1>), dim3(grid), dim3(kAT), lds, s, aa, table); break;
+        case 16 + 2: hipLaunchKernelGGL((k_admit<true, 4, 2>), dim3(grid), dim3(kAT), lds, s, aa, table); break;
+        case 1: hipLaunchKernelGGL((k_admit<true, 0, 1>), dim3(grid), dim3(kAT), lds, s, aa, table); break;
+        case 2: hipLaunchKernelGGL((k_admit<true, 0, 2>), dim3(grid), dim3(kAT), lds, s, aa, table); break;
+        default: hipLaunchKernelGGL((k_admit<true, 0, 0>), dim3(grid), dim3(kAT), lds, s, aa, table); break;
         }
     } else {
-        hipLaunchKernelGGL((k_admit<false, 0, 0>), dim3(1), dim3(kAT), uni, s, aa);
+        hipLaunchKernelGGL((k_admit<false, 0, 0>), dim3(grid), dim3(kAT), uni, s, aa, table);
     }
     RK_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void admit_pass(const DevGraph& g, int32_t k, int mode, int ntiles, RefineBufs& b, int32_t* part, hipStream_t s) {
+    const AdmitArgs aa{mode, k, g.ncon, ntiles, b.tile_cnt, b.tile_bits, b.tile_off, b.tile_rec, b.mov, b.pool,
+                       g.xadj, g.vwgt, b.fmax, b.omax, b.pw, part, b.moves, b.nmoves, b.done, b.passes,
+                       b.adm_scratch};
+    launch_admit(aa, nullptr, 1, k, g.ncon, s);
+}
+
+void admit_pass_batch(int mode, int ngraphs, const AdmitArgs* table, int32_t kmax, int32_t ncon, hipStream_t s) {
+    if (ngraphs <= 0) return;
+    AdmitArgs aa{};
+    aa.mode = mode;
+    launch_admit(aa, table, ngraphs, kmax, ncon, s);
 }
 
 }  // namespace rk

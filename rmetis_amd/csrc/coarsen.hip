@@ -73,6 +73,12 @@ struct MatchArgs {
     int32_t* mlist[2];
     int32_t* seg_cnt;   // [2 * kMatchSegMax]
     int32_t* flags;     // [kMaxMatchRounds] "round r changed the matching"
+    // batched levels (null for a single graph): graph of each vertex, and per
+    // graph its first vertex, matching seed and whether it is matched here
+    const int32_t* gseg;
+    const int32_t* gv0;
+    const uint64_t* gseed;
+    const uint8_t* gon;
 };
 
 __device__ inline void dev_match_init(const MatchArgs& a, int blk, int* s_tmp) {
@@ -82,7 +88,7 @@ __device__ inline void dev_match_init(const MatchArgs& a, int blk, int* s_tmp) {
         const int64_t v = vb + threadIdx.x;
         bool open = false;
         if (v < hi) {
-            open = a.xadj[v + 1] != a.xadj[v];
+            open = a.xadj[v + 1] != a.xadj[v] && (!a.gseg || a.gon[a.gseg[v]]);
             a.match[v] = open ? -1 : (int32_t)v;
             a.cand[v] = -1;
         }
@@ -110,13 +116,22 @@ __device__ inline void dev_match_propose(const MatchArgs& a, int r, int blk) {
         if (prev >= 0 && a.match[prev] == -1) continue;   // group-uniform
         const int32_t beg = a.xadj[v], end = a.xadj[v + 1];
         const int32_t dv = end - beg;
+        // keys use graph-local ids and the graph's seed, so a graph of a batch
+        // is matched exactly as it is alone
+        uint64_t seed = a.seed;
+        int32_t base = 0;
+        if (a.gseg) {
+            const int32_t gi = a.gseg[v];
+            seed = a.gseed[gi];
+            base = a.gv0[gi];
+        }
         EdgeKey best{0ull, 0u};
         int32_t bu = -1;
         for (int32_t j = beg + gl; j < end; j += G) {
             const int32_t u = a.adjncy[j];
             if (a.match[u] != -1) continue;
             const int32_t du = a.xadj[u + 1] - a.xadj[u];
-            const EdgeKey key = edge_key(v, u, a.adjwgt ? a.adjwgt[j] : 1, dv, du, a.seed, a.shem);
+            const EdgeKey key = edge_key(v - base, u - base, a.adjwgt ? a.adjwgt[j] : 1, dv, du, seed, a.shem);
             if (bu < 0 || key.gt(best) || (key.eq(best) && u < bu)) {
                 best = key;
                 bu = u;
@@ -198,10 +213,13 @@ __global__ __launch_bounds__(kBlock) void k_match_accept(MatchArgs a, int r) {
 
 __global__ __launch_bounds__(kBlock) void k_match_finish_seg(MatchArgs a) { dev_match_finish(a, blockIdx.x); }
 
+// Batched levels: vertices of graphs not matched at this level (gseg/gon)
+// lead no coarse vertex and get cmap -1; they leave the hierarchy here.
 __global__ __launch_bounds__(kBlock) void k_leader(int32_t n, const int32_t* __restrict__ match,
-                                                   int32_t* __restrict__ leader) {
+                                                   int32_t* __restrict__ leader, const int32_t* __restrict__ gseg,
+                                                   const uint8_t* __restrict__ gon) {
     const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (v < n) leader[v] = match[v] >= (int32_t)v ? 1 : 0;
+    if (v < n) leader[v] = (match[v] >= (int32_t)v && (!gseg || gon[gseg[v]])) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_cmap(int32_t n, int32_t ncon, const int32_t* __restrict__ xadj,
@@ -209,9 +227,14 @@ __global__ __launch_bounds__(kBlock) void k_cmap(int32_t n, int32_t ncon, const 
                                                  const int32_t* __restrict__ match, const int32_t* __restrict__ cid,
                                                  int32_t* __restrict__ cmap, int32_t* __restrict__ members,
                                                  int32_t* __restrict__ degsum, int32_t* __restrict__ cvwgt,
-                                                 int32_t* __restrict__ cvsize) {
+                                                 int32_t* __restrict__ cvsize, const int32_t* __restrict__ gseg,
+                                                 const uint8_t* __restrict__ gon, int32_t* __restrict__ cgseg) {
     const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v >= n) return;
+    if (gseg && !gon[gseg[v]]) {
+        cmap[v] = -1;
+        return;
+    }
     const int32_t u = match[v];
     const int32_t a = min((int32_t)v, u);
     const int32_t cv = cid[a];
@@ -226,6 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_cmap(int32_t n, int32_t ncon, const 
                 cvwgt[(int64_t)cv * ncon + c] =
                     wadd(vwgt[v * ncon + c], pair ? vwgt[(int64_t)u * ncon + c] : 0);
         if (cvsize) cvsize[cv] = wadd(vsize[v], pair ? vsize[u] : 0);
+        if (cgseg) cgseg[cv] = gseg[v];
     }
 }
 
@@ -505,7 +529,8 @@ void match_rounds(MatchArgs& a, int nseg, hipStream_t s) {
 
 }  // namespace
 
-void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, CoarsenBufs& b, hipStream_t s) {
+void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, CoarsenBufs& b, hipStream_t s,
+                 const GraphSegs* gs) {
     if (g.n <= 0) return;
     if (rounds > kMaxMatchRounds) rounds = kMaxMatchRounds;
     const int nb = (g.n + kBlock - 1) / kBlock;
@@ -517,7 +542,8 @@ void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, C
     nseg = (int)(((int64_t)g.n + seg - 1) / seg);
     RK_HIP(hipMemsetAsync(b.round_flags, 0, sizeof(int32_t) * kMaxMatchRounds, s));
     MatchArgs a{g.n, seg, rounds, ctype_shem, seed, g.xadj, g.adjncy, g.adjwgt, b.match, b.cand,
-                {b.mlist[0], b.mlist[1]}, b.seg_cnt, b.round_flags};
+                {b.mlist[0], b.mlist[1]}, b.seg_cnt, b.round_flags,
+                gs ? gs->gseg : nullptr, gs ? gs->gv0 : nullptr, gs ? gs->gseed : nullptr, gs ? gs->gon : nullptr};
     switch (pick_group(g.n, g.m)) {
         case 4: match_rounds<4>(a, nseg, s); break;
         case 8: match_rounds<8>(a, nseg, s); break;
@@ -529,7 +555,7 @@ void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, C
 }
 
 ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, DevGraph& cg, bool aggregate_vsize,
-                        CoarsenBufs& b, CoarseAlloc& al, int32_t* heavy_list, hipStream_t s) {
+                        CoarsenBufs& b, CoarseAlloc& al, int32_t* heavy_list, hipStream_t s, const GraphSegs* gs) {
     // Coarse sizes stay on the device until the single synchronisation at the
     // end; arrays are allocated with the upper bounds cn <= n, cm <= m and
     // every kernel reads cn = counters[0].
@@ -542,13 +568,15 @@ ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, 
     cg.xadj = al.alloc_i32((int64_t)n + 1);
     cg.adjncy = al.alloc_i32(g.m > 0 ? g.m : 1);
     cg.adjwgt = al.alloc_i32(g.m > 0 ? g.m : 1);
-    hipLaunchKernelGGL(k_leader, dim3(nb), dim3(kBlock), 0, s, n, match, b.leader);
+    const int32_t* gseg = gs ? gs->gseg : nullptr;
+    const uint8_t* gon = gs ? gs->gon : nullptr;
+    hipLaunchKernelGGL(k_leader, dim3(nb), dim3(kBlock), 0, s, n, match, b.leader, gseg, gon);
     exclusive_scan_i32(b.leader, b.cid, n, cn_dev, b.scan_tmp, s);      // coarse ids (graph/mod.rs:288-297)
     RK_HIP(hipMemsetAsync(b.degsum, 0, sizeof(int32_t) * n, s));
     RK_HIP(hipMemsetAsync(b.cdeg, 0, sizeof(int32_t) * n, s));
     RK_HIP(hipMemsetAsync(b.counters + 2, 0, 8 * sizeof(int32_t), s));
     hipLaunchKernelGGL(k_cmap, dim3(nb), dim3(kBlock), 0, s, n, g.ncon, g.xadj, g.vwgt, g.vsize, match, b.cid, cmap,
-                       b.members, b.degsum, cg.vwgt, cg.vsize);
+                       b.members, b.degsum, cg.vwgt, cg.vsize, gseg, gon, gs ? gs->cgseg : nullptr);
     exclusive_scan_i32(b.degsum, b.tmp_off, n, nullptr, b.scan_tmp, s);
     {
         const int blocks = (n + 64 * kCWaves - 1) / (64 * kCWaves);

@@ -11,6 +11,23 @@ void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* tot
                         hipStream_t s);
 
 // ---- reduce.hip ----
+// Batched levels (graphs laid out one after another; gseg[v] = graph of v,
+// per-graph k*ncon tables at offset gkc[g]):
+void part_weights_batch(const DevGraph& g, const int32_t* part, const int32_t* gseg, const int32_t* gkc,
+                        const int32_t* gk, int64_t kc_total, int32_t* pw, hipStream_t s);
+// cut_out[g] = edge cut of graph g (i32 wrap, halved); acc is ngraphs u64 scratch.
+void edge_cut_batch(const DevGraph& g, const int32_t* part, const int32_t* gseg, int32_t ngraphs,
+                    unsigned long long* acc, int32_t* cut_out, hipStream_t s);
+// part[v] = packed[goff[g] + v - gv0[g]] for the graphs g = gseg[v] with goff[g] >= 0.
+void scatter_parts(int32_t n, const int32_t* gseg, const int32_t* gv0, const int32_t* goff, const int32_t* packed,
+                   int32_t* part, hipStream_t s);
+// desc: 5 ints per listed graph {v0, n, e0, m, out}; packed[out...] = its
+// xadj (n+1, rebased), adjncy (m, graph-local ids), vwgt (n*ncon).
+void gather_csr(const DevGraph& g, const int32_t* desc, int32_t ndesc, int32_t* packed, hipStream_t s);
+void copy_improved(int32_t n, const int32_t* gseg, const uint8_t* improved, const int32_t* part, int32_t* best,
+                   hipStream_t s);
+void fill_gseg(int32_t ngraphs, const int32_t* gv0, const int32_t* gn, int32_t* gseg, hipStream_t s);
+
 // Lanes per vertex for a level with n vertices and m directed entries.
 int pick_group(int32_t n, int32_t m);
 void edge_cut(const DevGraph& g, const int32_t* part, unsigned long long* acc, int32_t* cut_out, int G,
@@ -49,6 +66,28 @@ struct RefineBufs {
     void* scan_tmp = nullptr;
 };
 
+// Arguments of one ordered admission pass (one graph).
+struct AdmitArgs {
+    int32_t mode, k, ncon, ntiles;
+    const int32_t* tile_cnt;
+    uint32_t* tile_bits;         // bit per tile with movers (cleared by the pass)
+    int32_t* tile_list;          // [ntiles] ordered ids of those tiles
+    const int4* tile_rec;
+    int4* mov;
+    const int32_t* pool;
+    const int32_t* xadj;
+    const int32_t* vwgt;
+    const long long* fmax;
+    const long long* omax;
+    int32_t* pw;
+    int32_t* part;
+    int2* moves;
+    int32_t* nmoves;
+    int32_t* done;
+    int32_t* passes;
+    int32_t* gscratch;   // 3*k*ncon ints when k*ncon exceeds the LDS tables
+};
+
 // Heavy-vertex list (degree > kGainHeavy) of a level with n = *n_dev (or n_max
 // when n_dev is null) vertices; the count lands in *count (no host sync).
 void heavy_list_build(int32_t n_max, const int32_t* n_dev, const int32_t* xadj, int32_t* list, int32_t* count,
@@ -66,6 +105,10 @@ void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32
 // Ordered admission of the candidates that the gain kernels left in
 // b.tile_rec / b.tile_cnt (one workgroup; updates b.pw, part, b.moves).
 void admit_pass(const DevGraph& g, int32_t k, int mode, int ntiles, RefineBufs& b, int32_t* part, hipStream_t s);
+// One admission pass for every graph of a batched level: workgroup i runs
+// the pass described by table[i] (device memory); kmax and ncon size the
+// LDS tables of all of them.
+void admit_pass_batch(int mode, int ngraphs, const AdmitArgs* table, int32_t kmax, int32_t ncon, hipStream_t s);
 
 // Gain table of every vertex (repair-mode candidate lists with every part
 // treated as overweight): per-vertex record (v or -1, pv, ncand, nbefore |
@@ -103,7 +146,18 @@ constexpr int kMatchSegMax = 4096;   // matching blocks (list segments) at most
 constexpr int kContractBig = 64;   // coarse vertices of larger degree sum use the global-hash path
 
 // Deterministic seeded handshake matching (ctype 1 heavy edge, 0 random).
-void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, CoarsenBufs& b, hipStream_t s);
+// The graphs of a batched level, laid out one after another in one CSR
+// (null for a single graph).
+struct GraphSegs {
+    const int32_t* gseg = nullptr;     // [n] graph of each vertex
+    const int32_t* gv0 = nullptr;      // [ngraphs] first vertex of each graph at this level
+    const uint64_t* gseed = nullptr;   // [ngraphs] matching seed of each graph at this level
+    const uint8_t* gon = nullptr;      // [ngraphs] graph is matched and contracted at this level
+    int32_t* cgseg = nullptr;          // [n] out: graph of each coarse vertex
+};
+
+void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, CoarsenBufs& b, hipStream_t s,
+                 const GraphSegs* gs = nullptr);
 struct CoarseAlloc {
     virtual int32_t* alloc_i32(int64_t count) = 0;
     virtual ~CoarseAlloc() = default;
@@ -117,6 +171,7 @@ struct ContractResult {
 // cmap[n]; arrays come from `al` sized by the fine graph.  When heavy_list is
 // given, the coarse heavy-vertex list is built as well.  One synchronisation.
 ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, DevGraph& cg, bool aggregate_vsize,
-                        CoarsenBufs& b, CoarseAlloc& al, int32_t* heavy_list, hipStream_t s);
+                        CoarsenBufs& b, CoarseAlloc& al, int32_t* heavy_list, hipStream_t s,
+                        const GraphSegs* gs = nullptr);
 
 }  // namespace rk

@@ -73,11 +73,133 @@ __global__ __launch_bounds__(kBlock) void k_part_weights_global(int32_t n, int32
     }
 }
 
+// Vertices with cmap -1 (graphs of a batch whose coarsest level this is)
+// keep the part already stored for them.
 __global__ __launch_bounds__(kBlock) void k_project(int32_t n, const int32_t* __restrict__ cmap,
                                                     const int32_t* __restrict__ part_c,
                                                     int32_t* __restrict__ part_f) {
     int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (v < n) part_f[v] = part_c[cmap[v]];
+    if (v < n) {
+        const int32_t c = cmap[v];
+        if (c >= 0) part_f[v] = part_c[c];
+    }
+}
+
+// ---- batched levels: several graphs laid out one after another ----
+
+// pw of every graph at its table offset gkc[g]; a block whose vertices all
+// belong to one graph sums in LDS first.
+__global__ __launch_bounds__(kBlock) void k_part_weights_batch(int32_t n, int32_t ncon,
+                                                               const int32_t* __restrict__ vwgt,
+                                                               const int32_t* __restrict__ part,
+                                                               const int32_t* __restrict__ gseg,
+                                                               const int32_t* __restrict__ gkc,
+                                                               const int32_t* __restrict__ gk,
+                                                               int32_t* __restrict__ pw) {
+    __shared__ int32_t s_h[kPwLds];
+    const int64_t beg = (int64_t)blockIdx.x * kPwVerts;
+    const int64_t end = min((int64_t)n, beg + kPwVerts);
+    const int32_t g0 = gseg[beg], g1 = gseg[end - 1];
+    const int kc = gk[g0] * ncon;
+    if (g0 == g1 && kc <= kPwLds) {
+        for (int i = threadIdx.x; i < kc; i += kBlock) s_h[i] = 0;
+        __syncthreads();
+        for (int64_t v = beg + threadIdx.x; v < end; v += kBlock) {
+            const int32_t p = part[v];
+            for (int c = 0; c < ncon; ++c) atomicAdd(&s_h[p * ncon + c], vwgt ? vwgt[v * ncon + c] : 1);
+        }
+        __syncthreads();
+        int32_t* out = pw + gkc[g0];
+        for (int i = threadIdx.x; i < kc; i += kBlock)
+            if (s_h[i] != 0) atomicAdd(&out[i], s_h[i]);
+    } else {
+        for (int64_t v = beg + threadIdx.x; v < end; v += kBlock) {
+            int32_t* out = pw + gkc[gseg[v]];
+            const int32_t p = part[v];
+            for (int c = 0; c < ncon; ++c) atomicAdd(&out[p * ncon + c], vwgt ? vwgt[v * ncon + c] : 1);
+        }
+    }
+}
+
+// Edge cut of every graph: each wave sums its vertices' cut weight, one
+// atomic per wave when its vertices belong to one graph.
+__global__ __launch_bounds__(kBlock) void k_edge_cut_batch(int32_t n, const int32_t* __restrict__ xadj,
+                                                           const int32_t* __restrict__ adjncy,
+                                                           const int32_t* __restrict__ adjwgt,
+                                                           const int32_t* __restrict__ part,
+                                                           const int32_t* __restrict__ gseg,
+                                                           unsigned long long* __restrict__ acc) {
+    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    long long sum = 0;
+    int32_t gi = -1;
+    if (v < n) {
+        gi = gseg[v];
+        const int32_t pv = part[v];
+        for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j)
+            if (part[adjncy[j]] != pv) sum += adjwgt ? adjwgt[j] : 1;
+    }
+    const int32_t gfirst = __shfl(gi, 0, 64);
+    if (__ballot(v < n && gi != gfirst) == 0) {
+        const long long tot = wave_incl_scan(sum);
+        if (__lane_id() == 63 && gfirst >= 0) atomicAdd(&acc[gfirst], (unsigned long long)__shfl(tot, 63, 64));
+    } else if (v < n && sum) {
+        atomicAdd(&acc[gi], (unsigned long long)sum);
+    }
+}
+
+// The reference sums in i32 with wrap-around and then halves (truncating).
+__global__ __launch_bounds__(kBlock) void k_finish_cut_batch(int32_t ng, const unsigned long long* __restrict__ acc,
+                                                             int32_t* __restrict__ out) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g < ng) out[g] = (int32_t)(uint32_t)acc[g] / 2;
+}
+
+// part[v] = packed[goff[gseg[v]] + v - gv0[gseg[v]]] for the graphs with goff >= 0.
+__global__ __launch_bounds__(kBlock) void k_scatter_parts(int32_t n, const int32_t* __restrict__ gseg,
+                                                          const int32_t* __restrict__ gv0,
+                                                          const int32_t* __restrict__ goff,
+                                                          const int32_t* __restrict__ packed,
+                                                          int32_t* __restrict__ part) {
+    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v >= n) return;
+    const int32_t gi = gseg[v];
+    const int32_t o = goff[gi];
+    if (o >= 0) part[v] = packed[o + (v - gv0[gi])];
+}
+
+// One block per listed graph: copies its CSR slice with graph-local ids into
+// packed (xadj n+1, adjncy m, vwgt n*ncon at the offsets of its descriptor
+// {v0, n, e0, m, out}).
+__global__ __launch_bounds__(kBlock) void k_gather_csr(const int32_t* __restrict__ xadj,
+                                                       const int32_t* __restrict__ adjncy,
+                                                       const int32_t* __restrict__ vwgt, int32_t ncon,
+                                                       const int32_t* __restrict__ desc,
+                                                       int32_t* __restrict__ packed) {
+    const int32_t* d = desc + 5 * blockIdx.x;
+    const int32_t v0 = d[0], nn = d[1], e0 = d[2], mm = d[3], o = d[4];
+    int32_t* px = packed + o;
+    int32_t* pa = px + nn + 1;
+    int32_t* pv = pa + mm;
+    for (int i = threadIdx.x; i <= nn; i += kBlock) px[i] = xadj[v0 + i] - e0;
+    for (int i = threadIdx.x; i < mm; i += kBlock) pa[i] = adjncy[e0 + i] - v0;
+    if (vwgt)
+        for (int i = threadIdx.x; i < nn * ncon; i += kBlock) pv[i] = vwgt[(int64_t)v0 * ncon + i];
+}
+
+// best[v] = part[v] for the vertices of graphs whose attempt improved.
+__global__ __launch_bounds__(kBlock) void k_copy_improved(int32_t n, const int32_t* __restrict__ gseg,
+                                                          const uint8_t* __restrict__ improved,
+                                                          const int32_t* __restrict__ part,
+                                                          int32_t* __restrict__ best) {
+    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v < n && improved[gseg[v]]) best[v] = part[v];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_gseg(int32_t ng, const int32_t* __restrict__ gv0,
+                                                      const int32_t* __restrict__ gn, int32_t* __restrict__ gseg) {
+    // one block per graph
+    const int32_t v0 = gv0[blockIdx.x], nn = gn[blockIdx.x];
+    for (int i = threadIdx.x; i < nn; i += kBlock) gseg[v0 + i] = blockIdx.x;
 }
 
 __global__ __launch_bounds__(kBlock) void k_column_sum(int32_t n, int32_t ncon, int32_t c,
@@ -152,6 +274,55 @@ void column_sums(const DevGraph& g, long long* out, hipStream_t s) {
     for (int c = 0; c < g.ncon; ++c)
         hipLaunchKernelGGL(k_column_sum, dim3(grid_for(g.n, kBlock, 2048)), dim3(kBlock), 0, s, g.n, g.ncon, c,
                            g.vwgt, (unsigned long long*)(out + c));
+    RK_HIP(hipGetLastError());
+}
+
+void part_weights_batch(const DevGraph& g, const int32_t* part, const int32_t* gseg, const int32_t* gkc,
+                        const int32_t* gk, int64_t kc_total, int32_t* pw, hipStream_t s) {
+    RK_HIP(hipMemsetAsync(pw, 0, kc_total * sizeof(int32_t), s));
+    if (g.n > 0)
+        hipLaunchKernelGGL(k_part_weights_batch, dim3((g.n + kPwVerts - 1) / kPwVerts), dim3(kBlock), 0, s, g.n,
+                           g.ncon, g.vwgt, part, gseg, gkc, gk, pw);
+    RK_HIP(hipGetLastError());
+}
+
+void edge_cut_batch(const DevGraph& g, const int32_t* part, const int32_t* gseg, int32_t ngraphs,
+                    unsigned long long* acc, int32_t* cut_out, hipStream_t s) {
+    RK_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * ngraphs, s));
+    if (g.n > 0)
+        hipLaunchKernelGGL(k_edge_cut_batch, dim3((g.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g.n, g.xadj,
+                           g.adjncy, g.adjwgt, part, gseg, acc);
+    hipLaunchKernelGGL(k_finish_cut_batch, dim3((ngraphs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ngraphs, acc,
+                       cut_out);
+    RK_HIP(hipGetLastError());
+}
+
+void scatter_parts(int32_t n, const int32_t* gseg, const int32_t* gv0, const int32_t* goff, const int32_t* packed,
+                   int32_t* part, hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_scatter_parts, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, gseg, gv0, goff,
+                           packed, part);
+    RK_HIP(hipGetLastError());
+}
+
+void gather_csr(const DevGraph& g, const int32_t* desc, int32_t ndesc, int32_t* packed, hipStream_t s) {
+    if (ndesc > 0)
+        hipLaunchKernelGGL(k_gather_csr, dim3(ndesc), dim3(kBlock), 0, s, g.xadj, g.adjncy, g.vwgt, g.ncon, desc,
+                           packed);
+    RK_HIP(hipGetLastError());
+}
+
+void copy_improved(int32_t n, const int32_t* gseg, const uint8_t* improved, const int32_t* part, int32_t* best,
+                   hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_copy_improved, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, gseg, improved,
+                           part, best);
+    RK_HIP(hipGetLastError());
+}
+
+void fill_gseg(int32_t ngraphs, const int32_t* gv0, const int32_t* gn, int32_t* gseg, hipStream_t s) {
+    if (ngraphs > 0)
+        hipLaunchKernelGGL(k_fill_gseg, dim3(ngraphs), dim3(kBlock), 0, s, ngraphs, gv0, gn, gseg);
     RK_HIP(hipGetLastError());
 }
 

@@ -48,6 +48,17 @@ struct GainArgs {
     int4* vrec;                 // optional per-vertex record (gain-table export)
     int32_t* g0_out;            // optional zero-group gain per vertex
     int32_t* gains;             // optional candidate gains beside pool
+    // batched levels (null for a single graph): per tile its first vertex
+    // (-1: padding), vertex count and graph; per vertex its graph; per graph
+    // the offset of its k*ncon tables (pw, omax), its part count and its flag
+    // "a cut pass moved nothing"
+    const int32_t* tile_v0;
+    const int32_t* tile_nv;
+    const int32_t* tile_g;
+    const int32_t* gseg;
+    const int32_t* gkc;
+    const int32_t* gk;
+    const int32_t* gdone;
 };
 
 __device__ inline bool gain_qualifies(int mode, int32_t g) {
@@ -56,9 +67,9 @@ __device__ inline bool gain_qualifies(int mode, int32_t g) {
     return mode == 1 ? g > 0 : g != INT32_MIN;
 }
 
-__device__ inline bool part_overweight(const GainArgs& a, int32_t p) {
-    for (int c = 0; c < a.ncon; ++c)
-        if ((long long)a.pw[(int64_t)p * a.ncon + c] > a.omax[(int64_t)p * a.ncon + c]) return true;
+__device__ inline bool part_overweight(const int32_t* pw, const long long* omax, int ncon, int32_t p) {
+    for (int c = 0; c < ncon; ++c)
+        if ((long long)pw[(int64_t)p * ncon + c] > omax[(int64_t)p * ncon + c]) return true;
     return false;
 }
 
@@ -107,23 +118,46 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
     int32_t* s_loff = s_loff_all[wv];
     int32_t* s_beg = s_beg_all[wv];
     const int gw = blockIdx.x * kGWaves + wv, nwaves = gridDim.x * kGWaves;
-    int32_t hb = 0, he = 0, hp = -1;   // head of the next tile: xadj[v], xadj[v+1], part[v]
+    // head of the next tile: its span and, per lane, xadj[v], xadj[v+1], part[v]
+    int64_t hv0 = 0;
+    int hnv = 0;
+    int32_t hb = 0, he = 0, hp = -1;
     auto load_head = [&](int tl) {
-        const int64_t v = (int64_t)tl * kTile + lane;
+        if (a.tile_v0) {
+            hv0 = a.tile_v0[tl];
+            hnv = a.tile_nv[tl];
+        } else {
+            hv0 = (int64_t)tl * kTile;
+            hnv = (int)min((int64_t)kTile, (int64_t)a.n - hv0);
+        }
         hb = he = 0;
         hp = -1;
-        if (v < a.n) {
+        if (lane < hnv) {
+            const int64_t v = hv0 + lane;
             hb = a.xadj[v];
             he = a.xadj[v + 1];
             hp = a.part[v];
         }
     };
     if (gw < ntiles) load_head(gw);
-    if (*a.done) return;   // read together with the first head
+    if (!a.tile_g && *a.done) return;   // read together with the first head
     for (int tile = gw; tile < ntiles; tile += nwaves) {
-        const int64_t v = (int64_t)tile * kTile + lane;
-        const bool own = v < a.n;
+        const int64_t v = hv0 + lane;
+        const bool own = lane < hnv;
         const int32_t beg = hb, end = he, pv = hp;
+        // this tile's graph tables (a single graph: the arguments themselves)
+        const int32_t* pwg = a.pw;
+        const long long* omg = a.omax;
+        if (a.tile_g) {
+            const int32_t gi = a.tile_g[tile];
+            if (gi < 0 || a.gdone[gi]) {       // padding tile, or its graph has converged
+                if (lane == 0) a.tile_cnt[tile] = 0;
+                if (tile + nwaves < ntiles) load_head(tile + nwaves);
+                continue;
+            }
+            pwg += a.gkc[gi];
+            omg += a.gkc[gi];
+        }
         if (tile + nwaves < ntiles) load_head(tile + nwaves);
         const int32_t deg = end - beg;
         const bool heavy = own && deg > kGainHeavy;
@@ -183,7 +217,7 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
                     if (s_pu[e] == pv) internal += (uint32_t)s_w[e];
                     else ext = true;
                 }
-                if (ext && (a.mode == 1 || part_overweight(a, pv))) {
+                if (ext && (a.mode == 1 || part_overweight(pwg, omg, a.ncon, pv))) {
                     const int32_t ed_own = (int32_t)internal;
                     const int32_t g0 = wsub(0, ed_own);
                     // aggregate equal parts onto their first occurrence
@@ -246,19 +280,32 @@ __global__ __launch_bounds__(kBlock) void k_gain_heavy(GainArgs a, const int32_t
     extern __shared__ int32_t s_dyn[];
     __shared__ uint32_t s_int;
     __shared__ int s_ext, s_nc, s_nb;
-    if (*a.done) return;
-    const int k = a.k;
+    if (!a.gseg && *a.done) return;
+    const int kmax = a.k;   // the largest part count of the level's graphs
     int32_t* hist;
     int2* cands;
-    if (k <= kHeavyLds) {
+    if (kmax <= kHeavyLds) {
         hist = s_dyn;
-        cands = (int2*)(s_dyn + k + (k & 1));
+        cands = (int2*)(s_dyn + kmax + (kmax & 1));
     } else {
-        hist = gscratch + (int64_t)blockIdx.x * 3 * k;
-        cands = (int2*)(hist + k + (k & 1));
+        hist = gscratch + (int64_t)blockIdx.x * 3 * kmax;
+        cands = (int2*)(hist + kmax + (kmax & 1));
     }
     for (int hi = blockIdx.x; hi < nheavy; hi += gridDim.x) {
         const int32_t v = heavy_list[hi];
+        int k = kmax;
+        const int32_t* pwg = a.pw;
+        const long long* omg = a.omax;
+        if (a.gseg) {
+            const int32_t gi = a.gseg[v];
+            if (a.gdone[gi]) {
+                if (threadIdx.x == 0) heavy_rec[v] = make_int4(-1, 0, 0, 0);
+                continue;                      // block-uniform
+            }
+            k = a.gk[gi];
+            pwg += a.gkc[gi];
+            omg += a.gkc[gi];
+        }
         const int32_t beg = a.xadj[v], end = a.xadj[v + 1];
         const int32_t pv = a.part[v];
         for (int p = threadIdx.x; p < k; p += kBlock) hist[p] = 0;
@@ -290,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_gain_heavy(GainArgs a, const int32_t
         __syncthreads();
         const int32_t ed_own = (int32_t)s_int;
         int4 rec = make_int4(-1, 0, 0, 0);
-        const bool gate = a.mode == 1 || part_overweight(a, pv);
+        const bool gate = a.mode == 1 || part_overweight(pwg, omg, a.ncon, pv);
         if (s_ext && gate) {
             const int32_t g0 = wsub(0, ed_own);
             for (int p = threadIdx.x; p < k; p += kBlock) {
