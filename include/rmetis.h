@@ -88,6 +88,33 @@ int32_t rmetis_hip_kway_device(idx_t nvtxs, idx_t ncon, const idx_t* d_xadj, con
                                const real_t* ubvec, const idx_t* options, idx_t* objval, idx_t* d_part,
                                rmetis_hip_stats* stats);
 
+/* Many graphs in one call (no single counterpart in the reference: each
+ * graph gets what RMETIS_PartGraphKway / partition_kway, kway.rs:20-141,
+ * returns for it alone).  The graphs are laid out one after another in one
+ * CSR with global vertex ids: graph i owns vertices [gvtx[i], gvtx[i+1]) and
+ * their rows, and its edges stay inside that range.  gvtx[ngraphs+1],
+ * nparts[ngraphs], seeds[ngraphs] and objval[ngraphs] are host arrays;
+ * d_xadj[n+1], d_adjncy, d_vwgt[n*ncon] (required), d_adjwgt (may be NULL)
+ * and d_part[n] are device pointers, n = gvtx[ngraphs].  options are shared
+ * (RTYPE must be Greedy; NCUTS attempts per graph); attempt a of graph i uses
+ * seed seeds[i] + a (options' SEED + a when seeds is NULL; a zero seed draws
+ * entropy) and graph i keeps its strictly best cut, so every graph's part
+ * and objval equal those of a separate call with that seed.  ubvec[ncon]
+ * (NULL = 1 + ufactor/1000) applies to every graph; target weights are
+ * uniform.  The CSR is not validated.  stats (may be NULL) describes the
+ * union levels of the last attempt. */
+int32_t rmetis_hip_kway_batch_device(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, const idx_t* d_xadj,
+                                     const idx_t* d_adjncy, const idx_t* d_vwgt, const idx_t* d_adjwgt,
+                                     const idx_t* nparts, const real_t* ubvec, const idx_t* options,
+                                     const uint64_t* seeds, idx_t* objval, idx_t* d_part, rmetis_hip_stats* stats);
+
+/* rmetis_hip_kway_batch_device on host arrays, validated per graph as
+ * src/graph/validate.rs:16-142 validates one graph (rows inside their graph,
+ * no self-loops, symmetric entry counts, positive vertex weights). */
+int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                              const idx_t* vwgt, const idx_t* adjwgt, const idx_t* nparts, const real_t* ubvec,
+                              const idx_t* options, const uint64_t* seeds, idx_t* objval, idx_t* part);
+
 /* Graph::edge_cut, src/graph/mod.rs:123-137 (host pointers; adjwgt may be NULL). */
 int32_t rmetis_hip_edge_cut(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* adjwgt,
                             const idx_t* part, idx_t* cut);

@@ -317,5 +317,56 @@ def part_graph_kway(graph, nparts, tpwgts=None, ubvec=None, options=None):
     return PartitionResult(part=part, objval=obj.value)
 
 
+def pack_batch(graphs):
+    """Lays graphs out one after another in one CSR with global vertex ids.
+    Returns (gvtx, xadj, adjncy, vwgt, adjwgt, ncon); missing vertex weights
+    become ones, missing edge weights ones when any graph has edge weights."""
+    if not graphs:
+        raise InputError("a batch needs at least one graph")
+    ncon = graphs[0].ncon
+    if any(g.ncon != ncon for g in graphs):
+        raise InputError("every graph of a batch needs the same ncon")
+    nv = [len(g.xadj) - 1 for g in graphs]
+    gvtx = np.zeros(len(graphs) + 1, np.int64)
+    np.cumsum(nv, out=gvtx[1:])
+    eoff = np.zeros(len(graphs) + 1, np.int64)
+    np.cumsum([len(g.adjncy) for g in graphs], out=eoff[1:])
+    if gvtx[-1] >= 2**31 or eoff[-1] >= 2**31:
+        raise InputError("batch too large for int32 CSR")
+    xadj = np.concatenate([g.xadj[:-1].astype(np.int64) + eoff[i] for i, g in enumerate(graphs)]
+                          + [np.array([eoff[-1]])]).astype(np.int32)
+    adjncy = np.concatenate([g.adjncy.astype(np.int64) + gvtx[i] for i, g in enumerate(graphs)]).astype(np.int32)
+    vwgt = np.concatenate([g.vwgt if g.vwgt is not None else np.ones(n * ncon, np.int32)
+                           for g, n in zip(graphs, nv)]).astype(np.int32)
+    adjwgt = None
+    if any(g.adjwgt is not None for g in graphs):
+        adjwgt = np.concatenate([g.adjwgt if g.adjwgt is not None else np.ones(len(g.adjncy), np.int32)
+                                 for g in graphs]).astype(np.int32)
+    return gvtx.astype(np.int32), xadj, adjncy, vwgt, adjwgt, ncon
+
+
+def part_graph_kway_batch(graphs, nparts, ubvec=None, options=None, seeds=None):
+    """Partitions many graphs in one call (rmetis_hip_kway_batch): graph i with
+    nparts[i] parts gets the PartitionResult part_graph_kway(graphs[i],
+    nparts[i], None, ubvec, options) gives it when options' seed is seeds[i]
+    (the options' seed for every graph when seeds is None)."""
+    gvtx, xadj, adjncy, vwgt, adjwgt, ncon = pack_batch(graphs)
+    G = len(graphs)
+    kk = np.ascontiguousarray(nparts, np.int32)
+    if kk.shape != (G,):
+        raise InputError("nparts needs one entry per graph")
+    options = options if options is not None else Options()
+    opts = options.to_c_array()
+    ub = None if ubvec is None else np.ascontiguousarray(ubvec, np.float32)
+    sd = None if seeds is None else np.ascontiguousarray(seeds, np.uint64)
+    obj = np.zeros(G, np.int32)
+    part = np.zeros(int(gvtx[-1]), np.int32)
+    rc = _lib.lib().rmetis_hip_kway_batch(G, _p(gvtx), ncon, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), _p(kk),
+                                          _p(ub, _lib.f32p), _p(opts), _p(sd, C.POINTER(C.c_uint64)), _p(obj),
+                                          _p(part))
+    _raise(rc)
+    return [PartitionResult(part=part[gvtx[i]:gvtx[i + 1]].copy(), objval=int(obj[i])) for i in range(G)]
+
+
 def device_count():
     return _lib.lib().rmetis_hip_device_count()

@@ -1,0 +1,445 @@
+// batch.cpp -- many k-way partitions in one pass over a segmented union graph.
+//
+// Every graph of the batch is partitioned exactly as kway_device partitions
+// it alone (partition/kway.rs:20-141): the same matching keys and seeds
+// (graph-local ids, the graph's own seed stream), the same coarsening stop
+// rule (coarsen/mod.rs:49-69), the same host growing of its coarsest graph
+// (initial/grow.rs:12-103), the same greedy passes (greedy.rs:36-193) and
+// the same best-of-ncuts choice (kway.rs:96-98).  What changes is the
+// schedule: one attempt index of all graphs runs level by level in lockstep,
+// so each kernel launch covers every graph still present at that level.
+//
+// Level l holds the graphs that reached it laid out one after another.  A
+// graph whose coarsening stops at level l is present at l but not matched
+// there (its vertices get cmap -1 and leave the hierarchy); its grown
+// partition is scattered into level l's part array before refinement, while
+// the other graphs of level l receive theirs by projection from level l+1.
+// Tiles of 64 vertices never straddle graphs and start at a multiple of 32
+// per graph, so every graph owns whole words of the tile bitmap and is
+// admitted by its own workgroup.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <random>
+#include <thread>
+
+#include "driver.hpp"
+#include "profiler.hpp"
+
+namespace rk {
+
+namespace {
+
+// Asynchronous upload of v through the pinned bump area (into d, or into a
+// new arena array).
+template <typename T>
+T* put(Context& cx, const std::vector<T>& v, T* d = nullptr) {
+    if (!d) d = cx.arena.get<T>((int64_t)v.size());
+    if (!v.empty()) {
+        T* h = (T*)cx.upload_stage(sizeof(T) * v.size());
+        std::copy(v.begin(), v.end(), h);
+        RK_HIP(hipMemcpyAsync(d, h, sizeof(T) * v.size(), hipMemcpyHostToDevice, cx.stream));
+    }
+    return d;
+}
+
+// Per-graph extent of one level (zero n for graphs not present).
+struct Extent {
+    int32_t v0 = 0, n = 0, e0 = 0, m = 0;
+};
+
+struct BLevel {
+    DevGraph g;
+    int32_t* gseg = nullptr;      // [g.n] graph of each vertex
+    int32_t* cmap = nullptr;      // previous level's vertex -> this level (null at level 0)
+    int32_t* heavy = nullptr;
+    int32_t nheavy = 0;
+    int32_t* d_gv0 = nullptr;     // [G] first vertex of each graph here
+    std::vector<Extent> ext;
+    std::vector<uint8_t> on;      // matched and contracted at this level
+    std::vector<uint8_t> last;    // this level is the graph's coarsest
+};
+
+std::vector<Extent> read_extents(Context& cx, const DevGraph& g, const int32_t* gseg, int32_t G, int32_t* d_tmp) {
+    graph_ranges(g.n, gseg, g.xadj, G, d_tmp, cx.stream);
+    int32_t* h = (int32_t*)cx.host_stage2(sizeof(int32_t) * 4 * (size_t)G);
+    RK_HIP(hipMemcpyAsync(h, d_tmp, sizeof(int32_t) * 4 * (size_t)G, hipMemcpyDeviceToHost, cx.stream));
+    RK_HIP(hipStreamSynchronize(cx.stream));
+    std::vector<Extent> e(G);
+    for (int32_t i = 0; i < G; ++i) {
+        e[i].v0 = h[4 * i];
+        e[i].n = h[4 * i + 1] - h[4 * i];
+        e[i].e0 = h[4 * i + 2];
+        e[i].m = h[4 * i + 3] - h[4 * i + 2];
+    }
+    return e;
+}
+
+inline int64_t padded_tiles(int32_t n) { return ((((int64_t)n + kTile - 1) / kTile) + 31) & ~(int64_t)31; }
+
+}  // namespace
+
+void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvtx, const int32_t* nparts,
+                       const float* ubvec, const Options& o, const uint64_t* seeds, int32_t* d_part, int32_t* objval,
+                       KwayStats* st) {
+    if (G <= 0) throw Error(-2, "a batch needs at least one graph");
+    if (!o.rtype_greedy)
+        throw Error(-4, "rtype Fm is not provided by the GPU k-way path; set options[METIS_OPTION_RTYPE]=1 "
+                        "(Greedy)");
+    if (!g.vwgt) throw Error(-2, "a batch needs explicit vertex weights");
+    if (gvtx[0] != 0 || gvtx[G] != g.n) throw Error(-2, "graph offsets must start at 0 and end at nvtxs");
+    const int32_t ncon = g.ncon;
+    std::vector<int32_t> gk(G);
+    std::vector<int32_t> gkc(G);
+    int64_t kc_total = 0;
+    int32_t kmax = 0;
+    int64_t tiles0 = 0;
+    for (int32_t i = 0; i < G; ++i) {
+        const int32_t n = gvtx[i + 1] - gvtx[i];
+        if (n <= 0) throw Error(-2, "graph " + std::to_string(i) + " has no vertices");
+        if (nparts[i] < 2) throw Error(-2, "graph " + std::to_string(i) + ": nparts must be >= 2");   // kway.rs:31-33
+        if (nparts[i] > n) throw Error(-2, "graph " + std::to_string(i) + ": nparts > nvtxs");        // kway.rs:34-38
+        gk[i] = nparts[i];
+        gkc[i] = (int32_t)kc_total;
+        kc_total += (int64_t)nparts[i] * ncon;
+        if (kc_total > INT32_MAX) throw Error(-2, "sum of nparts*ncon over the batch exceeds the int32 range");
+        kmax = std::max(kmax, nparts[i]);
+        tiles0 += padded_tiles(n);
+    }
+    if (tiles0 > INT32_MAX / kTile) throw Error(-2, "batch too large for the tile tables");
+    hipStream_t s = cx.stream;
+    ArenaAlloc alloc(cx.arena);
+    Buffers B = alloc_buffers(cx, g.n, g.m, kmax, ncon, tiles0, kc_total);
+    // per-graph device tables that live for the whole call
+    int32_t* d_gk = put(cx, gk);
+    int32_t* d_gkc = put(cx, gkc);
+    int32_t* d_gvtx = put(cx, std::vector<int32_t>(gvtx, gvtx + G + 1));
+    int32_t* d_gdone = cx.arena.get<int32_t>(3 * (int64_t)G);
+    int32_t* d_gnmoves = d_gdone + G;
+    int32_t* d_gpasses = d_gdone + 2 * (int64_t)G;
+    RK_HIP(hipMemsetAsync(d_gdone, 0, sizeof(int32_t) * 3 * (size_t)G, s));
+    int32_t* d_ranges = cx.arena.get<int32_t>(4 * (int64_t)G);
+    unsigned long long* d_cutacc = cx.arena.get<unsigned long long>(G);
+    int32_t* d_cut = cx.arena.get<int32_t>(G);
+    int32_t* gseg0 = cx.arena.get<int32_t>(g.n);
+    {
+        std::vector<int32_t> gn(G);
+        for (int32_t i = 0; i < G; ++i) gn[i] = gvtx[i + 1] - gvtx[i];
+        fill_gseg(G, d_gvtx, put(cx, gn), gseg0, s);
+    }
+    // per-graph totals (graph/mod.rs:392-410) and the thresholds of every
+    // level: with explicit vertex weights they are the same at each level
+    std::vector<std::vector<int32_t>> total(G, std::vector<int32_t>(ncon));
+    {
+        long long* d_sum = cx.arena.get<long long>((int64_t)G * ncon);
+        graph_column_sums(g, d_gvtx, G, d_sum, s);
+        long long* h = (long long*)cx.host_stage2(sizeof(long long) * (size_t)G * ncon);
+        RK_HIP(hipMemcpyAsync(h, d_sum, sizeof(long long) * (size_t)G * ncon, hipMemcpyDeviceToHost, s));
+        RK_HIP(hipStreamSynchronize(s));
+        for (int32_t i = 0; i < G; ++i)
+            for (int c = 0; c < ncon; ++c) {
+                const long long v = h[(size_t)i * ncon + c];
+                if (v > INT32_MAX)
+                    throw Error(-4, "graph " + std::to_string(i) +
+                                        ": total vertex weight of a constraint exceeds the int32 range");
+                total[i][c] = (int32_t)v;
+            }
+    }
+    std::vector<double> ub(ncon);
+    for (int32_t c = 0; c < ncon; ++c) ub[c] = ubvec ? (double)ubvec[c] : 1.0 + (double)o.ufactor / 1000.0;
+    {
+        std::vector<long long> fm(kc_total), om(kc_total);
+        for (int32_t i = 0; i < G; ++i)
+            level_thresholds(gk[i], ncon, nullptr, total[i].data(), ub.data(), fm.data() + gkc[i],
+                             om.data() + gkc[i]);
+        put(cx, fm, B.rb.fmax);
+        put(cx, om, B.rb.omax);
+    }
+    int32_t* fin_heavy = cx.arena.get<int32_t>(g.n);
+    const int32_t fin_nheavy = heavy_list_sync(g, fin_heavy, B.rb, s);
+    const std::vector<Extent> ext0 = read_extents(cx, g, gseg0, G, d_ranges);
+    int32_t* d_gv00 = nullptr;
+    {
+        std::vector<int32_t> v0(G);
+        for (int32_t i = 0; i < G; ++i) v0[i] = gvtx[i];
+        d_gv00 = put(cx, v0);
+    }
+    std::vector<int32_t> best(G, 0);
+    std::vector<uint8_t> have(G, 0);
+    if (st) *st = KwayStats{};
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::random_device rd;
+    for (int attempt = 0; attempt < o.ncuts; ++attempt) {
+        const Arena::Mark mark = cx.arena.mark();
+        std::vector<Rng> rng;
+        rng.reserve(G);
+        for (int32_t i = 0; i < G; ++i) {
+            uint64_t sd = seeds ? seeds[i] : o.seed;
+            if (sd == 0) sd = ((uint64_t)rd() << 32) ^ rd();
+            else sd += (uint64_t)attempt;                                 // kway.rs:55-61
+            rng.emplace_back(sd);
+        }
+        std::vector<BLevel> lv(1);
+        lv[0].g = g;
+        lv[0].gseg = gseg0;
+        lv[0].heavy = fin_heavy;
+        lv[0].nheavy = fin_nheavy;
+        lv[0].d_gv0 = d_gv00;
+        lv[0].ext = ext0;
+        lv[0].on.assign(G, 1);
+        lv[0].last.assign(G, 0);
+        std::vector<LevelStat> lst;
+        // ---- coarsening (coarsen/mod.rs:49-69 per graph)
+        for (size_t l = 0;; ++l) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (st) {
+                RK_HIP(hipEventCreate(&e0));
+                RK_HIP(hipEventCreate(&e1));
+                RK_HIP(hipEventRecord(e0, s));
+            }
+            BLevel& L = lv[l];
+            std::vector<uint64_t> gseed(G, 0);
+            for (int32_t i = 0; i < G; ++i)
+                if (L.on[i]) gseed[i] = rng[i].next_u64();
+            GraphSegs gs;
+            gs.gseg = L.gseg;
+            gs.gv0 = L.d_gv0;
+            gs.gseed = put(cx, gseed);
+            gs.gon = put(cx, L.on);
+            gs.cgseg = cx.arena.get<int32_t>(L.g.n);
+            {
+                ProfScope ps(kProfMatch, 0.0, L.g.n, s);
+                match_graph(L.g, o.ctype_shem, 0, kMatchRounds, B.cb, s, &gs);
+            }
+            BLevel C;
+            C.cmap = cx.arena.get<int32_t>(L.g.n);
+            C.heavy = cx.arena.get<int32_t>(L.g.n);
+            ContractResult cr;
+            {
+                ProfScope ps(kProfContract, 0.0, L.g.n, s);
+                cr = contract(L.g, B.cb.match, C.cmap, C.g, false, B.cb, alloc, C.heavy, s, &gs);
+            }
+            C.nheavy = cr.nheavy;
+            C.gseg = gs.cgseg;
+            C.ext = read_extents(cx, C.g, C.gseg, G, d_ranges);
+            C.on.assign(G, 0);
+            C.last.assign(G, 0);
+            bool any = false;
+            std::vector<int32_t> v0(G, 0);
+            for (int32_t i = 0; i < G; ++i) {
+                v0[i] = C.ext[i].v0;
+                if (!L.on[i]) continue;
+                const int32_t cn = C.ext[i].n;
+                const int32_t limit = std::max(20 * gk[i], 20);                 // coarsen/mod.rs:43
+                const float ratio = (float)cn / (float)L.ext[i].n;              // coarsen/mod.rs:61
+                if (cn <= limit || ratio > 0.95f) {                              // coarsen/mod.rs:65
+                    if (cn < gk[i])
+                        throw Error(-4, "graph " + std::to_string(i) +
+                                            ": coarsest graph has fewer vertices than nparts");
+                    C.last[i] = 1;
+                } else {
+                    C.on[i] = 1;
+                    any = true;
+                }
+            }
+            C.d_gv0 = put(cx, v0);
+            if (st) {
+                LevelStat ls;
+                ls.n = L.g.n;
+                ls.m = L.g.m;
+                RK_HIP(hipEventRecord(e1, s));
+                RK_HIP(hipEventSynchronize(e1));
+                RK_HIP(hipEventElapsedTime(&ls.coarsen_ms, e0, e1));
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                lst.push_back(ls);
+            }
+            lv.push_back(std::move(C));
+            if (!any) break;
+        }
+        const size_t nl = lv.size();
+        // ---- coarsest graphs to the host, grown there (kway.rs:66-70)
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<int32_t> lvl_of(G, -1);
+        std::vector<int64_t> csr_off(G, 0), part_off(G, 0);
+        int64_t csr_tot = 0, part_tot = 0;
+        for (size_t l = 1; l < nl; ++l)
+            for (int32_t i = 0; i < G; ++i) {
+                if (!lv[l].last[i]) continue;
+                const Extent& e = lv[l].ext[i];
+                lvl_of[i] = (int32_t)l;
+                csr_off[i] = csr_tot;
+                part_off[i] = part_tot;
+                csr_tot += (int64_t)e.n + 1 + e.m + (int64_t)e.n * ncon;
+                part_tot += e.n;
+            }
+        if (csr_tot > INT32_MAX) throw Error(-4, "coarsest graphs of the batch exceed the staging range");
+        int32_t* d_packed = cx.arena.get<int32_t>(csr_tot);
+        for (size_t l = 1; l < nl; ++l) {
+            std::vector<int32_t> desc;
+            for (int32_t i = 0; i < G; ++i)
+                if (lvl_of[i] == (int32_t)l) {
+                    const Extent& e = lv[l].ext[i];
+                    desc.insert(desc.end(), {e.v0, e.n, e.e0, e.m, (int32_t)csr_off[i]});
+                }
+            if (!desc.empty()) gather_csr(lv[l].g, put(cx, desc), (int32_t)(desc.size() / 5), d_packed, s);
+        }
+        int32_t* hcsr = (int32_t*)cx.host_stage2(sizeof(int32_t) * (size_t)csr_tot);
+        RK_HIP(hipMemcpyAsync(hcsr, d_packed, sizeof(int32_t) * (size_t)csr_tot, hipMemcpyDeviceToHost, s));
+        RK_HIP(hipStreamSynchronize(s));
+        int32_t* hpart = (int32_t*)cx.upload_stage(sizeof(int32_t) * (size_t)std::max<int64_t>(part_tot, 1));
+        {
+            std::atomic<int32_t> next{0};
+            std::atomic<int32_t> failed{-1};
+            auto work = [&]() {
+                std::vector<double> tgt;
+                for (;;) {
+                    const int32_t i = next.fetch_add(1);
+                    if (i >= G) break;
+                    const Extent& e = lv[lvl_of[i]].ext[i];
+                    const int32_t* x = hcsr + csr_off[i];
+                    const int32_t* a = x + e.n + 1;
+                    const int32_t* vw = a + e.m;
+                    int32_t* hp = hpart + part_off[i];
+                    const int32_t k = gk[i];
+                    if (o.iptype == 1) {
+                        for (int32_t v = 0; v < e.n; ++v) hp[v] = (int32_t)rng[i].below((uint64_t)k);  // random.rs:8-10
+                        continue;
+                    }
+                    tgt.assign((size_t)k * ncon, 0.0);
+                    for (int32_t p = 0; p < k; ++p)
+                        for (int32_t c = 0; c < ncon; ++c)
+                            tgt[(size_t)p * ncon + c] = (1.0 / (double)k) * (double)total[i][c];
+                    if (!grow_host(e.n, ncon, x, a, vw, k, tgt.data(), rng[i], hp)) failed.store(i);
+                }
+            };
+            const unsigned nt = std::min<unsigned>(hw, (unsigned)G);
+            std::vector<std::thread> th;
+            for (unsigned t = 1; t < nt; ++t) th.emplace_back(work);
+            work();
+            for (auto& t : th) t.join();
+            if (failed.load() >= 0)
+                throw Error(-4, "graph " + std::to_string(failed.load()) +
+                                    ": coarsest graph has fewer vertices than nparts");
+        }
+        int32_t* d_gparts = cx.arena.get<int32_t>(std::max<int64_t>(part_tot, 1));
+        RK_HIP(hipMemcpyAsync(d_gparts, hpart, sizeof(int32_t) * (size_t)part_tot, hipMemcpyHostToDevice, s));
+        auto t1 = std::chrono::steady_clock::now();
+        // ---- refinement from the deepest level up (kway.rs:73-92)
+        int32_t* pc = nullptr;
+        std::vector<LevelStat> rst(nl);
+        for (int64_t l = (int64_t)nl - 1; l >= 0; --l) {
+            BLevel& L = lv[l];
+            int32_t* pf = cx.arena.get<int32_t>(L.g.n);
+            if (l + 1 < (int64_t)nl) project(L.g.n, lv[l + 1].cmap, pc, pf, s);
+            std::vector<int32_t> goff(G, -1);
+            bool anylast = false;
+            for (int32_t i = 0; i < G; ++i)
+                if (lvl_of[i] == l) {
+                    goff[i] = (int32_t)part_off[i];
+                    anylast = true;
+                }
+            if (anylast) scatter_parts(L.g.n, L.gseg, L.d_gv0, put(cx, goff), d_gparts, pf, s);
+            // tile and admission tables of the graphs present
+            std::vector<int4> tdesc;
+            std::vector<AdmitArgs> adm;
+            int64_t tb = 0;
+            for (int32_t i = 0; i < G; ++i) {
+                const Extent& e = L.ext[i];
+                if (e.n == 0) continue;
+                tdesc.push_back(make_int4(i, e.v0, e.n, (int32_t)tb));
+                AdmitArgs aa{};
+                aa.k = gk[i];
+                aa.ncon = ncon;
+                aa.ntiles = (int32_t)padded_tiles(e.n);
+                aa.tile_cnt = B.rb.tile_cnt + tb;
+                aa.tile_bits = B.rb.tile_bits + tb / 32;
+                aa.tile_list = B.rb.tile_off + tb;
+                aa.tile_rec = B.rb.tile_rec + tb * kTile;
+                aa.mov = B.rb.mov + e.v0;
+                aa.pool = B.rb.pool;
+                aa.xadj = L.g.xadj;
+                aa.vwgt = L.g.vwgt;
+                aa.fmax = B.rb.fmax + gkc[i];
+                aa.omax = B.rb.omax + gkc[i];
+                aa.pw = B.rb.pw + gkc[i];
+                aa.part = pf;
+                aa.moves = B.rb.moves + e.v0;
+                aa.nmoves = d_gnmoves + i;
+                aa.done = d_gdone + i;
+                aa.passes = d_gpasses + i;
+                aa.gscratch = B.rb.adm_scratch + 3 * (int64_t)gkc[i];
+                adm.push_back(aa);
+                tb += padded_tiles(e.n);
+            }
+            BatchLevel bl;
+            bl.ntiles = (int32_t)tb;
+            int32_t* tt = cx.arena.get<int32_t>(3 * tb);
+            bl.tile_v0 = tt;
+            bl.tile_nv = tt + tb;
+            bl.tile_g = tt + 2 * tb;
+            fill_tiles(put(cx, tdesc), (int32_t)tdesc.size(), tt, tt + tb, tt + 2 * tb, s);
+            bl.gseg = L.gseg;
+            bl.gkc = d_gkc;
+            bl.gk = d_gk;
+            bl.gdone = d_gdone;
+            bl.admit = put(cx, adm);
+            bl.ngraphs = (int32_t)adm.size();
+            bl.kmax = kmax;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (st) {
+                RK_HIP(hipEventCreate(&e0));
+                RK_HIP(hipEventCreate(&e1));
+                RK_HIP(hipEventRecord(e0, s));
+            }
+            part_weights_batch(L.g, pf, L.gseg, d_gkc, d_gk, kc_total, B.rb.pw, s);   // greedy.rs:50
+            RK_HIP(hipMemsetAsync(d_gdone, 0, sizeof(int32_t) * (size_t)G, s));
+            refine_pass_batch(L.g, bl, 0, L.nheavy, L.heavy, B.rb, pf, s);                  // greedy.rs:60-123
+            for (int it = 0; it < o.niter; ++it)
+                refine_pass_batch(L.g, bl, 1, L.nheavy, L.heavy, B.rb, pf, s);              // greedy.rs:126-190
+            if (st) {
+                RK_HIP(hipEventRecord(e1, s));
+                RK_HIP(hipEventSynchronize(e1));
+                rst[l].n = L.g.n;
+                rst[l].m = L.g.m;
+                RK_HIP(hipEventElapsedTime(&rst[l].refine_ms, e0, e1));
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+            }
+            pc = pf;
+        }
+        // ---- per-graph cut (kway.rs:94) and the strictly better attempts (kway.rs:96-98)
+        edge_cut_batch(g, pc, gseg0, G, d_cutacc, d_cut, s);
+        int32_t* hc = (int32_t*)cx.host_stage2(sizeof(int32_t) * (size_t)G);
+        RK_HIP(hipMemcpyAsync(hc, d_cut, sizeof(int32_t) * (size_t)G, hipMemcpyDeviceToHost, s));
+        RK_HIP(hipStreamSynchronize(s));
+        std::vector<uint8_t> improved(G, 0);
+        for (int32_t i = 0; i < G; ++i)
+            if (!have[i] || hc[i] < best[i]) {
+                have[i] = 1;
+                best[i] = hc[i];
+                improved[i] = 1;
+            }
+        copy_improved(g.n, gseg0, put(cx, improved), pc, d_part, s);
+        RK_HIP(hipStreamSynchronize(s));
+        if (st) {
+            st->levels.clear();
+            for (size_t l = 0; l < nl; ++l) {
+                LevelStat ls = rst[l];
+                if (l < lst.size()) ls.coarsen_ms = lst[l].coarsen_ms;
+                ls.n = lv[l].g.n;
+                ls.m = lv[l].g.m;
+                st->levels.push_back(ls);
+            }
+            st->coarsest_n = (int32_t)part_tot;
+            st->initial_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+            st->attempts = attempt + 1;
+        }
+        cx.arena.rewind(mark);
+        cx.upload_reset();
+    }
+    for (int32_t i = 0; i < G; ++i) objval[i] = best[i];
+}
+
+}  // namespace rk

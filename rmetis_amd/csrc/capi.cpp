@@ -222,6 +222,70 @@ int32_t rmetis_hip_kway_device(idx_t nvtxs, idx_t ncon, const idx_t* d_xadj, con
     });
 }
 
+int32_t rmetis_hip_kway_batch_device(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, const idx_t* d_xadj,
+                                     const idx_t* d_adjncy, const idx_t* d_vwgt, const idx_t* d_adjwgt,
+                                     const idx_t* nparts, const real_t* ubvec, const idx_t* options,
+                                     const uint64_t* seeds, idx_t* objval, idx_t* d_part, rmetis_hip_stats* stats) {
+    return guarded([&]() -> int32_t {
+        if (!gvtx || !d_xadj || !d_adjncy || !d_vwgt || !nparts || !objval || !d_part) return METIS_ERROR_INPUT;
+        if (ngraphs <= 0 || ncon <= 0 || gvtx[ngraphs] <= 0) return METIS_ERROR_INPUT;
+        const Options o = parse_options(options);
+        ContextGuard cx;
+        DevGraph g;
+        g.n = gvtx[ngraphs];
+        g.ncon = ncon;
+        RK_HIP(hipMemcpyAsync(&g.m, d_xadj + g.n, sizeof(int32_t), hipMemcpyDeviceToHost, cx->stream));
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        g.xadj = (int32_t*)d_xadj;
+        g.adjncy = (int32_t*)d_adjncy;
+        g.vwgt = (int32_t*)d_vwgt;
+        g.adjwgt = (int32_t*)d_adjwgt;
+        KwayStats ks;
+        kway_batch_device(*cx.c, g, ngraphs, gvtx, nparts, ubvec, o, seeds, d_part, objval, stats ? &ks : nullptr);
+        if (stats) fill_stats(ks, stats);
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                              const idx_t* vwgt, const idx_t* adjwgt, const idx_t* nparts, const real_t* ubvec,
+                              const idx_t* options, const uint64_t* seeds, idx_t* objval, idx_t* part) {
+    return guarded([&]() -> int32_t {
+        if (!gvtx || !xadj || !adjncy || !vwgt || !nparts || !objval || !part) return METIS_ERROR_INPUT;
+        if (ngraphs <= 0 || ncon <= 0) return METIS_ERROR_INPUT;
+        // graph/validate.rs:16-142 per graph: every row stays inside its graph
+        if (gvtx[0] != 0) return METIS_ERROR_INPUT;
+        for (int32_t i = 0; i < ngraphs; ++i) {
+            const int32_t v0 = gvtx[i], v1 = gvtx[i + 1];
+            if (v1 <= v0) return METIS_ERROR_INPUT;
+            if (nparts[i] < 2 || nparts[i] > v1 - v0) return METIS_ERROR_INPUT;
+            for (int32_t v = v0; v < v1; ++v) {
+                if (xadj[v] > xadj[v + 1]) return METIS_ERROR_INPUT;
+                for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j)
+                    if (adjncy[j] < v0 || adjncy[j] >= v1 || adjncy[j] == v) return METIS_ERROR_INPUT;
+            }
+        }
+        const int32_t n = gvtx[ngraphs];
+        if (xadj[0] != 0) return METIS_ERROR_INPUT;
+        std::vector<int32_t> cnt(n, 0);
+        for (int64_t j = 0; j < xadj[n]; ++j) cnt[adjncy[j]]++;
+        for (int32_t v = 0; v < n; ++v)
+            if (cnt[v] != xadj[v + 1] - xadj[v]) return METIS_ERROR_INPUT;
+        for (int64_t i = 0; i < (int64_t)n * ncon; ++i)
+            if (vwgt[i] <= 0) return METIS_ERROR_INPUT;
+        const Options o = parse_options(options);
+        ContextGuard cx;
+        DevGraph g = upload_graph(*cx.c, n, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
+        int32_t* dp = cx->arena.get<int32_t>(n);
+        std::vector<int32_t> obj(ngraphs);
+        kway_batch_device(*cx.c, g, ngraphs, gvtx, nparts, ubvec, o, seeds, dp, obj.data(), nullptr);
+        download(*cx.c, part, dp, n);
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        std::copy(obj.begin(), obj.end(), objval);
+        return METIS_OK;
+    });
+}
+
 int32_t rmetis_hip_edge_cut(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* adjwgt,
                             const idx_t* part, idx_t* cut) {
     return guarded([&]() -> int32_t {

@@ -92,6 +92,37 @@ struct Context {
     // after the stream has been synchronised past its last copy.
     void* host_stage(size_t bytes) { return grow(pinned, pinned_size, bytes); }
     void* host_stage2(size_t bytes) { return grow(pinned2, pinned2_size, bytes); }
+    // Pinned bump area for many small uploads within one call: nothing is
+    // reused before upload_reset(), which the caller issues only after the
+    // stream has been synchronised past the last copy.  Chunks are kept for
+    // later calls.
+    void* upload_stage(size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        while (up_cur < up_chunks.size()) {
+            if (up_off + bytes <= up_chunks[up_cur].second) {
+                char* p = up_chunks[up_cur].first + up_off;
+                up_off += bytes;
+                return p;
+            }
+            ++up_cur;
+            up_off = 0;
+        }
+        const size_t sz = bytes > kUpChunk ? bytes : kUpChunk;
+        void* p = nullptr;
+        RK_HIP(hipHostMalloc(&p, sz, hipHostMallocDefault));
+        up_chunks.push_back({(char*)p, sz});
+        up_cur = up_chunks.size() - 1;
+        up_off = bytes;
+        return p;
+    }
+    void upload_reset() {
+        up_cur = 0;
+        up_off = 0;
+    }
+    static constexpr size_t kUpChunk = 4u << 20;
+    std::vector<std::pair<char*, size_t>> up_chunks;
+    size_t up_cur = 0, up_off = 0;
+
     static void* grow(void*& p, size_t& cap, size_t bytes) {
         if (bytes > cap) {
             if (p) (void)hipHostFree(p);
@@ -106,6 +137,7 @@ struct Context {
         if (pinned) (void)hipHostFree(pinned);
         if (pinned2) (void)hipHostFree(pinned2);
         if (small) (void)hipHostFree(small);
+        for (auto& c : up_chunks) (void)hipHostFree(c.first);
         arena.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -121,6 +153,7 @@ struct ContextGuard {
     ContextGuard() : c(acquire_context()) {}
     ~ContextGuard() {
         c->arena.reset();
+        c->upload_reset();
         release_context(c);
     }
     Context* operator->() { return c; }

@@ -66,14 +66,6 @@ long long floor_threshold(double t, bool feasibility) {
     return (long long)std::floor(t);
 }
 
-struct ArenaAlloc : CoarseAlloc {
-    Arena& a;
-    explicit ArenaAlloc(Arena& ar) : a(ar) {}
-    int32_t* alloc_i32(int64_t count) override { return a.get<int32_t>(count); }
-};
-
-constexpr int kMatchRounds = 24;
-
 }  // namespace
 
 void level_thresholds(int32_t k, int32_t ncon, const float* tpwgts, const int32_t* total, const double* ub,
@@ -100,12 +92,12 @@ void thresholds_from_targets(int32_t k, int32_t ncon, const double* tpw, const d
         }
 }
 
-Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon) {
+Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon, int64_t tiles, int64_t kc_total) {
     Arena& a = cx.arena;
     const hipStream_t s = cx.stream;
     Buffers B;
-    const int64_t ntiles = ((int64_t)n + kTile - 1) / kTile;
-    const int64_t kc = (int64_t)k * ncon;
+    const int64_t ntiles = tiles >= 0 ? tiles : ((int64_t)n + kTile - 1) / kTile;
+    const int64_t kc = kc_total >= 0 ? kc_total : (int64_t)k * ncon;
     RefineBufs& r = B.rb;
     r.tile_rec = a.get<int4>(ntiles * kTile);
     r.tile_cnt = a.get<int32_t>(ntiles);

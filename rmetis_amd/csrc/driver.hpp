@@ -26,6 +26,13 @@ struct Options {
 };
 
 constexpr int kNOptions = 40;  // types.rs:170
+constexpr int kMatchRounds = 24;   // handshake rounds per level at most
+
+struct ArenaAlloc : CoarseAlloc {
+    Arena& a;
+    explicit ArenaAlloc(Arena& ar) : a(ar) {}
+    int32_t* alloc_i32(int64_t count) override { return a.get<int32_t>(count); }
+};
 
 // c_api.rs:211-222 + types.rs:190-224 (NULL or options[0] == -1 => defaults).
 Options parse_options(const int32_t* options);
@@ -62,8 +69,10 @@ struct Buffers {
 
 // Allocates the per-call work buffers for a finest level of n vertices,
 // m entries, k parts, ncon constraints (the tile bitmap is cleared on the
-// context's stream).
-Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon);
+// context's stream).  A batch passes its tile count (padding included) and
+// the total of its graphs' k*ncon tables; k is then the largest part count.
+Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon, int64_t tiles = -1,
+                      int64_t kc_total = -1);
 
 // Column sums of vwgt (or n per constraint when vwgt is absent), checked
 // against the int32 range the reference's sums live in.
@@ -77,6 +86,17 @@ void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* d_
 // writes d_part and returns objval.
 int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwgts, const float* ubvec,
                     const Options& o, int32_t* d_part, KwayStats* st);
+
+// Many graphs in one call (partition/kway.rs:20-141 per graph).  g is the
+// union of the graphs laid out one after another (global vertex ids; graph i
+// owns vertices [gvtx[i], gvtx[i+1]) and their adjacency rows, with no edge
+// between graphs; vwgt required).  Attempt a of graph i uses the seed
+// seeds[i] + a (o.seed + a when seeds is null; 0 draws entropy), exactly as
+// a separate kway_device call with those options would, and the graph keeps
+// its strictly best cut.  Writes d_part[g.n] and objval[ngraphs] (host).
+void kway_batch_device(Context& cx, const DevGraph& g, int32_t ngraphs, const int32_t* gvtx, const int32_t* nparts,
+                       const float* ubvec, const Options& o, const uint64_t* seeds, int32_t* d_part, int32_t* objval,
+                       KwayStats* st);
 
 // initial/grow.rs:12-103 on the host; false when n < k.
 bool grow_host(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,

@@ -27,6 +27,15 @@ void gather_csr(const DevGraph& g, const int32_t* desc, int32_t ndesc, int32_t* 
 void copy_improved(int32_t n, const int32_t* gseg, const uint8_t* improved, const int32_t* part, int32_t* best,
                    hipStream_t s);
 void fill_gseg(int32_t ngraphs, const int32_t* gv0, const int32_t* gn, int32_t* gseg, hipStream_t s);
+// out[g*ncon + c] = int64 column sums of vwgt over graph g's vertices (g.vwgt required).
+void graph_column_sums(const DevGraph& g, const int32_t* gvtx, int32_t ngraphs, long long* out, hipStream_t s);
+// out[4g..4g+3] = (first vertex, end vertex, first entry, end entry) of each
+// graph g < ngraphs_total in the level (zeros for graphs not present).
+void graph_ranges(int32_t n, const int32_t* gseg, const int32_t* xadj, int32_t ngraphs_total, int32_t* out,
+                  hipStream_t s);
+// Tile tables of a batched level; desc[i] = {graph, first vertex, n, first tile}.
+void fill_tiles(const int4* desc, int32_t ndesc, int32_t* tile_v0, int32_t* tile_nv, int32_t* tile_g,
+                hipStream_t s);
 
 // Lanes per vertex for a level with n vertices and m directed entries.
 int pick_group(int32_t n, int32_t m);
@@ -109,6 +118,27 @@ void admit_pass(const DevGraph& g, int32_t k, int mode, int ntiles, RefineBufs& 
 // the pass described by table[i] (device memory); kmax and ncon size the
 // LDS tables of all of them.
 void admit_pass_batch(int mode, int ngraphs, const AdmitArgs* table, int32_t kmax, int32_t ncon, hipStream_t s);
+
+// Device tables of one batched level (graphs laid out one after another):
+// tiles never straddle graphs and each graph's tiles start at a multiple of
+// 32, so that its words of tile_bits are its own.
+struct BatchLevel {
+    int32_t ntiles = 0;               // tiles of the level, padding included
+    const int32_t* tile_v0 = nullptr; // [ntiles] first vertex (-1: padding)
+    const int32_t* tile_nv = nullptr; // [ntiles] vertices in the tile
+    const int32_t* tile_g = nullptr;  // [ntiles] graph (-1: padding)
+    const int32_t* gseg = nullptr;    // [n] graph of each vertex
+    const int32_t* gkc = nullptr;     // [ngraphs_total] offset of the graph's k*ncon tables
+    const int32_t* gk = nullptr;      // [ngraphs_total] part count of each graph
+    int32_t* gdone = nullptr;         // [ngraphs_total] "a cut pass of this level moved nothing"
+    const AdmitArgs* admit = nullptr; // [ngraphs] admission arguments of the graphs present
+    int32_t ngraphs = 0;              // graphs present at this level
+    int32_t kmax = 0;                 // largest part count among them
+};
+// refine_pass over every graph of a batched level (b.pw / b.omax hold the
+// per-graph tables at the offsets gkc).
+void refine_pass_batch(const DevGraph& g, const BatchLevel& bl, int mode, int nheavy, const int32_t* heavy_list,
+                       RefineBufs& b, int32_t* part, hipStream_t s);
 
 // Gain table of every vertex (repair-mode candidate lists with every part
 // treated as overweight): per-vertex record (v or -1, pv, ncand, nbefore |

@@ -195,6 +195,60 @@ __global__ __launch_bounds__(kBlock) void k_copy_improved(int32_t n, const int32
     if (v < n && improved[gseg[v]]) best[v] = part[v];
 }
 
+// out[4g..4g+3] = (first vertex, end vertex, first entry, end entry) of
+// every graph g with vertices in this level (out cleared beforehand).
+__global__ __launch_bounds__(kBlock) void k_graph_ranges(int32_t n, const int32_t* __restrict__ gseg,
+                                                         const int32_t* __restrict__ xadj, int32_t* __restrict__ out) {
+    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v >= n) return;
+    const int32_t gi = gseg[v];
+    if (v == 0 || gseg[v - 1] != gi) {
+        out[4 * (int64_t)gi] = (int32_t)v;
+        out[4 * (int64_t)gi + 2] = xadj[v];
+    }
+    if (v == n - 1 || gseg[v + 1] != gi) {
+        out[4 * (int64_t)gi + 1] = (int32_t)v + 1;
+        out[4 * (int64_t)gi + 3] = xadj[v + 1];
+    }
+}
+
+// One block per graph present: desc {graph, first vertex, n, first tile};
+// its tiles of kTile vertices, then padding up to a multiple of 32 tiles.
+__global__ __launch_bounds__(kBlock) void k_fill_tiles(const int4* __restrict__ desc, int32_t* __restrict__ tile_v0,
+                                                       int32_t* __restrict__ tile_nv, int32_t* __restrict__ tile_g) {
+    const int4 d = desc[blockIdx.x];
+    const int32_t nt = (d.z + kTile - 1) / kTile;
+    const int32_t np = (nt + 31) & ~31;
+    for (int i = threadIdx.x; i < np; i += kBlock) {
+        const int64_t t = (int64_t)d.w + i;
+        const bool real = i < nt;
+        tile_v0[t] = real ? d.y + i * kTile : -1;
+        tile_nv[t] = real ? min(kTile, d.z - i * kTile) : 0;
+        tile_g[t] = real ? d.x : -1;
+    }
+}
+
+// One block per graph: out[g*ncon + c] = sum of vwgt[v*ncon + c] over its
+// vertices [gvtx[g], gvtx[g+1]), as int64.
+__global__ __launch_bounds__(kBlock) void k_graph_colsum(const int32_t* __restrict__ gvtx, int32_t ncon,
+                                                         const int32_t* __restrict__ vwgt, long long* __restrict__ out) {
+    __shared__ long long s_part[kBlock / 64];
+    const int64_t v0 = gvtx[blockIdx.x], v1 = gvtx[blockIdx.x + 1];
+    for (int c = 0; c < ncon; ++c) {
+        long long sum = 0;
+        for (int64_t v = v0 + threadIdx.x; v < v1; v += kBlock) sum += vwgt[v * ncon + c];
+        sum = wave_incl_scan(sum);
+        if (__lane_id() == 63) s_part[threadIdx.x >> 6] = sum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long t = 0;
+            for (int w = 0; w < kBlock / 64; ++w) t += s_part[w];
+            out[(int64_t)blockIdx.x * ncon + c] = t;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_gseg(int32_t ng, const int32_t* __restrict__ gv0,
                                                       const int32_t* __restrict__ gn, int32_t* __restrict__ gseg) {
     // one block per graph
@@ -317,6 +371,27 @@ void copy_improved(int32_t n, const int32_t* gseg, const uint8_t* improved, cons
     if (n > 0)
         hipLaunchKernelGGL(k_copy_improved, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, gseg, improved,
                            part, best);
+    RK_HIP(hipGetLastError());
+}
+
+void graph_ranges(int32_t n, const int32_t* gseg, const int32_t* xadj, int32_t ngraphs_total, int32_t* out,
+                  hipStream_t s) {
+    RK_HIP(hipMemsetAsync(out, 0, sizeof(int32_t) * 4 * (size_t)ngraphs_total, s));
+    if (n > 0)
+        hipLaunchKernelGGL(k_graph_ranges, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, gseg, xadj, out);
+    RK_HIP(hipGetLastError());
+}
+
+void fill_tiles(const int4* desc, int32_t ndesc, int32_t* tile_v0, int32_t* tile_nv, int32_t* tile_g,
+                hipStream_t s) {
+    if (ndesc > 0)
+        hipLaunchKernelGGL(k_fill_tiles, dim3(ndesc), dim3(kBlock), 0, s, desc, tile_v0, tile_nv, tile_g);
+    RK_HIP(hipGetLastError());
+}
+
+void graph_column_sums(const DevGraph& g, const int32_t* gvtx, int32_t ngraphs, long long* out, hipStream_t s) {
+    if (ngraphs > 0)
+        hipLaunchKernelGGL(k_graph_colsum, dim3(ngraphs), dim3(kBlock), 0, s, gvtx, g.ncon, g.vwgt, out);
     RK_HIP(hipGetLastError());
 }
 

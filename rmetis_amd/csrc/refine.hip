@@ -384,8 +384,9 @@ __global__ __launch_bounds__(kBlock) void k_heavy_list(int32_t n_max, const int3
     if (h) list[base + __popcll(b & ((1ull << __lane_id()) - 1ull))] = (int32_t)v;
 }
 
-void launch_gain(const GainArgs& ga, int nheavy, const int32_t* heavy_list, RefineBufs& b, hipStream_t s) {
-    const int ntiles = (ga.n + kTile - 1) / kTile;
+void launch_gain(const GainArgs& ga, int nheavy, const int32_t* heavy_list, RefineBufs& b, hipStream_t s,
+                 int ntiles = -1) {
+    if (ntiles < 0) ntiles = (ga.n + kTile - 1) / kTile;
     const int k = ga.k;
     if (nheavy > 0) {
         const size_t lds = k <= kHeavyLds ? (size_t)(k + (k & 1) + 2 * k) * sizeof(int32_t) : 0;
@@ -449,6 +450,24 @@ void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32
     {
         ProfScope ps(kProfAdmit, 0.0, g.n, s);
         admit_pass(g, k, mode, ntiles, b, part, s);
+    }
+    RK_HIP(hipGetLastError());
+}
+
+void refine_pass_batch(const DevGraph& g, const BatchLevel& bl, int mode, int nheavy, const int32_t* heavy_list,
+                       RefineBufs& b, int32_t* part, hipStream_t s) {
+    if (g.n <= 0 || bl.ngraphs <= 0) return;
+    GainArgs ga{g.n, g.ncon, bl.kmax, mode, g.xadj, g.adjncy, g.adjwgt, g.vwgt, part, b.pw, b.omax,
+                b.heavy_rec, b.pool, b.tile_rec, b.tile_cnt, b.tile_bits, b.done, nullptr, nullptr, nullptr,
+                bl.tile_v0, bl.tile_nv, bl.tile_g, bl.gseg, bl.gkc, bl.gk, bl.gdone};
+    {
+        const double bytes = 4.0 * (g.n + 1) + 4.0 * g.n + (g.adjwgt ? 12.0 : 8.0) * g.m;
+        ProfScope ps(kProfGain, bytes, g.n, s);
+        launch_gain(ga, nheavy, heavy_list, b, s, bl.ntiles);
+    }
+    {
+        ProfScope ps(kProfAdmit, 0.0, g.n, s);
+        admit_pass_batch(mode, bl.ngraphs, bl.admit, bl.kmax, g.ncon, s);
     }
     RK_HIP(hipGetLastError());
 }

@@ -177,39 +177,56 @@ def rmat(scale, ef, seed, a=0.57, b=0.19, c=0.19, name=None, chunk=1 << 23):
     return CSR(xadj, adj, name=name or f"rmat{scale}")
 
 
-def uneven_batch(count, seed, nmin_log2=10, nmax_log2=18):
+def _batch_member(args):
+    """Graph i of the uneven batch from its drawn (i, n, k, gseed)."""
+    i, n, k, gseed = args
+    grng = np.random.default_rng(gseed)
+    if i % 2 == 0:
+        aspect = 2.0 ** grng.uniform(-2, 2)
+        rows = max(2, int(round(math.sqrt(n * aspect))))
+        cols = max(2, int(round(n / rows)))
+        g = grid(rows, cols, name=f"b{i}_grid{rows}x{cols}")
+        m2 = g.m // 2
+        # symmetric U{1..10} weights: draw per undirected edge, mirror.
+        src = np.repeat(np.arange(g.n), np.diff(g.xadj))
+        lo = np.minimum(src, g.adjncy).astype(np.int64)
+        hi = np.maximum(src, g.adjncy).astype(np.int64)
+        ukey, inv = np.unique(lo * g.n + hi, return_inverse=True)
+        w = grng.integers(1, 11, size=len(ukey), dtype=np.int64).astype(np.int32)
+        g.adjwgt = np.ascontiguousarray(w[inv])
+        g.vwgt = grng.integers(1, 5, size=g.n, dtype=np.int64).astype(np.int32)
+        assert len(ukey) == m2
+    else:
+        deg = grng.uniform(8, 16)
+        r = math.sqrt(deg / (math.pi * n))
+        g = rgg(n, gseed, r=r, wmax=10, vwgt_max=4, name=f"b{i}_rgg{n}", rng=grng)
+    return g, min(k, g.n)
+
+
+def uneven_batch(count, seed, nmin_log2=10, nmax_log2=18, workers=1):
     """The mixed batch: n log-uniform in [2^nmin_log2, 2^nmax_log2]; even
     indices are grids with aspect 2^U(-2,2), odd ones RGGs of mean degree
     U(8,16); vwgt U{1..4}, adjwgt U{1..10}; k from {2,4,8,16,32,64}.
+    Every graph's (n, k, seed) is drawn first from the batch seed, so the
+    result does not depend on `workers` (processes building the graphs).
     Returns a list of (CSR, k)."""
     rng = np.random.default_rng(seed)
-    out = []
+    params = []
     for i in range(count):
         n = int(round(2.0 ** rng.uniform(nmin_log2, nmax_log2)))
         k = int(rng.choice([2, 4, 8, 16, 32, 64]))
         gseed = int(rng.integers(0, 2**63 - 1))
-        grng = np.random.default_rng(gseed)
-        if i % 2 == 0:
-            aspect = 2.0 ** grng.uniform(-2, 2)
-            rows = max(2, int(round(math.sqrt(n * aspect))))
-            cols = max(2, int(round(n / rows)))
-            g = grid(rows, cols, name=f"b{i}_grid{rows}x{cols}")
-            m2 = g.m // 2
-            # symmetric U{1..10} weights: draw per undirected edge, mirror.
-            src = np.repeat(np.arange(g.n), np.diff(g.xadj))
-            lo = np.minimum(src, g.adjncy).astype(np.int64)
-            hi = np.maximum(src, g.adjncy).astype(np.int64)
-            ukey, inv = np.unique(lo * g.n + hi, return_inverse=True)
-            w = grng.integers(1, 11, size=len(ukey), dtype=np.int64).astype(np.int32)
-            g.adjwgt = np.ascontiguousarray(w[inv])
-            g.vwgt = grng.integers(1, 5, size=g.n, dtype=np.int64).astype(np.int32)
-            assert len(ukey) == m2
-        else:
-            deg = grng.uniform(8, 16)
-            r = math.sqrt(deg / (math.pi * n))
-            g = rgg(n, gseed, r=r, wmax=10, vwgt_max=4, name=f"b{i}_rgg{n}", rng=grng)
-        k = min(k, g.n)
-        out.append((g, k))
+        params.append((i, n, k, gseed))
+    if workers <= 1:
+        return [_batch_member(p) for p in params]
+    import multiprocessing as mp
+    # largest graphs first so the pool stays busy to the end
+    order = sorted(range(count), key=lambda i: -params[i][1])
+    with mp.get_context("fork").Pool(workers) as pool:
+        built = pool.map(_batch_member, [params[i] for i in order], chunksize=1)
+    out = [None] * count
+    for j, i in enumerate(order):
+        out[i] = built[j]
     return out
 
 

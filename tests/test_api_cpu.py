@@ -150,3 +150,48 @@ def test_generators_shape():
     for gg, k in b:
         R.Graph(gg.n, 1, gg.xadj, gg.adjncy, gg.vwgt, gg.adjwgt)
         assert k <= gg.n
+
+
+def test_pack_batch_layout():
+    from graphs import path
+    g1 = R.Graph(4, 1, *cycle4())
+    x, a = path(5)
+    g2 = R.Graph(5, 1, x, a, vwgt=np.arange(1, 6), adjwgt=np.full(len(a), 3))
+    gvtx, xadj, adjncy, vwgt, adjwgt, ncon = R.pack_batch([g1, g2])
+    assert list(gvtx) == [0, 4, 9] and ncon == 1
+    assert xadj[-1] == len(g1.adjncy) + len(g2.adjncy)
+    assert np.array_equal(xadj[:5], g1.xadj)
+    assert np.array_equal(xadj[4:] - xadj[4], g2.xadj)
+    assert np.array_equal(adjncy[:8], g1.adjncy) and np.array_equal(adjncy[8:] - 4, g2.adjncy)
+    assert np.array_equal(vwgt, [1, 1, 1, 1, 1, 2, 3, 4, 5])
+    assert np.array_equal(adjwgt, [1] * 8 + [3] * len(a))
+    # the packed union is itself a valid CSR with no edge between graphs
+    R.Graph(9, 1, xadj, adjncy, vwgt, adjwgt)
+
+
+def _batch_raw(gvtx, xadj, adjncy, vwgt, nparts):
+    arr = [None if v is None else np.ascontiguousarray(v, np.int32) for v in (gvtx, xadj, adjncy, vwgt, nparts)]
+    ptr = [None if v is None else v.ctypes.data_as(_lib.i32p) for v in arr]
+    G = len(nparts)
+    obj = np.full(G, -99, np.int32)
+    part = np.zeros(int(gvtx[-1]), np.int32)
+    rc = _lib.lib().rmetis_hip_kway_batch(G, ptr[0], 1, ptr[1], ptr[2], ptr[3], None, ptr[4], None, None, None,
+                                          obj.ctypes.data_as(_lib.i32p), part.ctypes.data_as(_lib.i32p))
+    return rc, obj
+
+
+def test_batch_abi_input_errors():
+    # per-graph graph/validate.rs:16-142 and kway.rs:31-38 checks, before any GPU work
+    x, a = cycle4()
+    xx = np.concatenate([x, x[1:] + x[-1]])
+    aa = np.concatenate([a, a + 4])
+    ones = np.ones(8, np.int32)
+    assert _batch_raw([0, 4, 8], xx, aa, None, [2, 2])[0] == -2            # vwgt required
+    assert _batch_raw([0, 4, 8], xx, aa, ones, [2, 5])[0] == -2            # nparts > nvtxs
+    assert _batch_raw([0, 4, 8], xx, aa, ones, [1, 2])[0] == -2            # nparts < 2
+    assert _batch_raw([0, 4, 4, 8], np.concatenate([x, x[1:] + x[-1]]), aa, ones, [2, 2, 2])[0] == -2  # empty graph
+    cross = aa.copy()
+    cross[0] = 5                                                          # an edge into the other graph
+    assert _batch_raw([0, 4, 8], xx, cross, ones, [2, 2])[0] == -2
+    rc, obj = _batch_raw([0, 4, 8], xx, aa, np.zeros(8, np.int32), [2, 2])
+    assert rc == -2 and (obj == -99).all()                                # outputs untouched on error
