@@ -289,11 +289,16 @@ def bench_batch(args):
     part = d_part.numpy()
     bobj = obj.copy()
     # balance: every part of every graph within floor(total/k * 1.03)
-    violations = 0
+    violations, violators = 0, []
     for i, (g, k) in enumerate(batch):
         p = part[gvtx[i]:gvtx[i + 1]]
         pw = np.bincount(p, weights=g.vwgt, minlength=k)
-        violations += int((pw > math.floor(float(g.vwgt.sum()) / k * 1.03)).sum())
+        lim = math.floor((1.0 / k) * float(g.vwgt.sum()) * (1.0 + 30 / 1000.0))
+        bad = int((pw > lim).sum())
+        if bad:
+            violations += bad
+            violators.append({"graph": i, "n": g.n, "k": int(k), "parts_over": bad, "max_pw": int(pw.max()),
+                              "limit": lim})
     levels = [{"n": stats.level_n[i], "m": stats.level_m[i], "coarsen_ms": round(stats.coarsen_ms[i], 3),
                "refine_ms": round(stats.refine_ms[i], 3)} for i in range(stats.nlevels)]
     # the plain path on a sample of the same graphs, same options
@@ -367,6 +372,7 @@ def bench_batch(args):
         "batched_vs_plain": round(value / plain_value, 2),
         "edge_cut_sum": int(bobj.sum()),
         "balance_violations": violations,
+        "balance_violators": violators[:16],
         "edges_per_s": round(m * ncuts * value / count, 1),
         "kernel_ms_per_step": kernel_ms,
         "levels": levels,

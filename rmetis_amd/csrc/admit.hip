@@ -37,8 +37,8 @@ namespace rk {
 namespace {
 
 constexpr int kAT = 256;        // threads per admission workgroup
-constexpr int kLdsKC = 4096;    // k*ncon entries of slack/thresholds kept in LDS
-constexpr int kCL = 8;          // candidates per mover staged in LDS
+constexpr int kLdsKC = 2048;    // k*ncon entries of slack/thresholds kept in LDS
+constexpr int kCL = 32;         // candidates per mover staged in LDS
 constexpr int kWin = 256;       // movers staged per window
 constexpr int kCntWin = 2048;   // tiles with movers gathered per window
 constexpr int kNoDest = -1;
@@ -89,10 +89,13 @@ RK_INL int block_excl_scan_int(int x, int* s_w, int* total) {
 // (ds_* accesses waited on with lgkmcnt only).
 struct Ctx {
     const AdmitArgs* a;
+    int mode;             // 0 balance repair, 1 cut optimisation
     int32_t* slack;       // fmax - pw, running
     const int32_t* fm;    // clamped fmax
     const int32_t* om;    // clamped omax
     Win w;
+    int32_t* wv;          // [128] LDS: speculating lanes' weights (constraint 0, then 1)
+    uint64_t* pl;         // [64] LDS: their bit planes (constraint 0, then 1)
 };
 
 // Weight of mover v in constraint c; constraints 0 and 1 come staged.
@@ -131,7 +134,7 @@ RK_INL int decide_table(const Ctx& x, int s, int32_t v, int32_t pv, int32_t nc, 
     const AdmitArgs& a = *x.a;
     const int lane = __lane_id();
     const int ncon = a.ncon;
-    if (a.mode == 0) {                                               // greedy.rs:69-74
+    if (x.mode == 0) {                                               // greedy.rs:69-74
         bool over = false;
         for (int c = lane; c < ncon; c += 64)
             over = over ||
@@ -151,15 +154,23 @@ RK_INL int decide_table(const Ctx& x, int s, int32_t v, int32_t pv, int32_t nc, 
             return __builtin_amdgcn_readlane(q, __builtin_ctzll(m));
     }
     if (!zflag) return kNoDest;
-    // zero group: parts other than pv that are not in the explicit list
+    // zero group: the first feasible part other than pv outside the explicit
+    // list; the candidates ahead of it were all infeasible above, so only
+    // those after it can hide a feasible listed part
+    auto listed_after = [&](int32_t p) __attribute__((always_inline)) -> bool {
+        for (int tb = (nbefore / 64) * 64; tb < nc; tb += 64) {
+            const int t = tb + lane;
+            const int32_t q = t < nc ? (tb == 0 ? q0 : cand_at(x, s, off, t)) : -1;
+            if (__ballot(t >= nbefore && q == p)) return true;
+        }
+        return false;
+    };
     for (int qb = 0; qb < a.k; qb += 64) {
         const int32_t q = qb + lane;
-        bool listed = q == pv;
-        for (int t = 0; t < nc; ++t) {
-            const int32_t ct = t < 64 ? __builtin_amdgcn_readlane(q0, t) : cand_at(x, s, off, t);
-            listed = listed || ct == q;
+        for (uint64_t m = __ballot(q < a.k && q != pv && fits(x, q, v, w0, w1)); m; m &= m - 1) {
+            const int32_t p = qb + __builtin_ctzll(m);
+            if (!listed_after(p)) return p;
         }
-        if (const uint64_t m = __ballot(q < a.k && !listed && fits(x, q, v, w0, w1))) return qb + __builtin_ctzll(m);
     }
     // explicit candidates [nbefore, nc)
     if (const uint64_t m = f0 & ~low_mask(nbefore)) return __builtin_amdgcn_readlane(q0, __builtin_ctzll(m));
@@ -196,21 +207,41 @@ struct TableSlack {
     RK_INL void store(const Ctx&) const {}
 };
 
+// Up to four values of one kind held in named registers.  Element access
+// by index is a chain of selects on values (an array indexed at run time
+// would be placed in scratch memory).
+template <typename T, int N = 4>
+struct Quad {
+    T e0{}, e1{}, e2{}, e3{};
+    RK_INL T get(int j) const {
+        if constexpr (N == 1) return e0;
+        else if constexpr (N == 2) return j == 0 ? e0 : e1;
+        else return j == 0 ? e0 : (j == 1 ? e1 : (j == 2 ? e2 : e3));
+    }
+    RK_INL void set(int j, T v) {
+        e0 = j == 0 ? v : e0;
+        e1 = j == 1 ? v : e1;
+        e2 = j == 2 ? v : e2;
+        e3 = j == 3 ? v : e3;
+    }
+};
+
 // Running slack in wave 0's registers for k <= 64*KP and ncon == NC <= 2:
-// lane l of s0[j] / s1[j] holds part 64*j + l (INT32_MIN past k, never
+// lane l of block j of s0 / s1 holds part 64*j + l (INT32_MIN past k, never
 // feasible).  A mover's feasible parts are one compare per register block
 // and constraint; no memory access sits on the mover-to-mover chain.
 template <int KP, int NC>
 struct RegSlack {
-    int32_t s0[KP], s1[KP];
+    static_assert(KP >= 1 && KP <= 4, "at most four register blocks");
+    Quad<int32_t, KP> s0, s1;
 
     RK_INL void load(const Ctx& x) {
         const int lane = __lane_id(), k = x.a->k;
 #pragma unroll
         for (int j = 0; j < KP; ++j) {
             const int p = 64 * j + lane;
-            s0[j] = p < k ? x.slack[p * NC] : INT32_MIN;
-            s1[j] = p < k && NC > 1 ? x.slack[p * NC + 1] : INT32_MIN;
+            s0.set(j, p < k ? x.slack[p * NC] : INT32_MIN);
+            s1.set(j, p < k && NC > 1 ? x.slack[p * NC + 1] : INT32_MIN);
         }
     }
     RK_INL void store(const Ctx& x) const {
@@ -219,53 +250,58 @@ struct RegSlack {
         for (int j = 0; j < KP; ++j) {
             const int p = 64 * j + lane;
             if (p < k) {
-                x.slack[p * NC] = s0[j];
-                if (NC > 1) x.slack[p * NC + 1] = s1[j];
+                x.slack[p * NC] = s0.get(j);
+                if (NC > 1) x.slack[p * NC + 1] = s1.get(j);
             }
         }
     }
     // uniform part p: its slack in constraint c
     RK_INL int32_t at(int32_t p, int c) const {
-        int32_t r = c == 0 ? s0[0] : s1[0];
-#pragma unroll
-        for (int j = 1; j < KP; ++j)
-            if ((p >> 6) == j) r = c == 0 ? s0[j] : s1[j];
+        const int32_t r = c == 0 ? s0.get(p >> 6) : s1.get(p >> 6);
         return __builtin_amdgcn_readlane(r, p & 63);
     }
-    // The decision walks the candidate list in admission order with scalar
-    // code: F holds one feasibility bit per part, so each candidate costs a
-    // readlane and a bit test, and most movers stop at the first one.
+    // F holds one feasibility bit per part; the candidates are tested 64 at a
+    // time (lane t: candidate t, the first 64 already in q0, the rest loaded
+    // from the pool in coalesced blocks) and a ballot picks the first one in
+    // admission order.
     RK_INL int decide(const Ctx& x, int s, int32_t, int32_t pv, int32_t nc, int32_t nbz, int32_t off, int32_t w0,
                       int32_t w1, int32_t q0) const {
         const AdmitArgs& a = *x.a;
-        if (a.mode == 0) {                                           // greedy.rs:69-74
+        const int lane = __lane_id();
+        if (x.mode == 0) {                                           // greedy.rs:69-74
             bool over = (long long)x.fm[pv * NC] - at(pv, 0) > (long long)x.om[pv * NC];
             if (NC > 1) over = over || (long long)x.fm[pv * NC + 1] - at(pv, 1) > (long long)x.om[pv * NC + 1];
             if (!over) return kNoDest;
         }
-        uint64_t F[KP];
+        Quad<uint64_t, KP> F;
 #pragma unroll
         for (int j = 0; j < KP; ++j) {
-            F[j] = __ballot(w0 <= s0[j]);
-            if (NC > 1) F[j] &= __ballot(w1 <= s1[j]);
+            uint64_t m = __ballot(w0 <= s0.get(j));
+            if (NC > 1) m &= __ballot(w1 <= s1.get(j));
+            F.set(j, m);
         }
-        auto feasible = [&](int32_t q) -> bool {                     // q uniform, 0 <= q < k
-            uint64_t m = F[0];
-#pragma unroll
-            for (int j = 1; j < KP; ++j)
-                if ((q >> 6) == j) m = F[j];
-            return (m >> (q & 63)) & 1ull;
+        auto feasible = [&](int32_t q) __attribute__((always_inline)) -> bool {   // per lane; q < 0: false
+            return q >= 0 && ((F.get(q >> 6) >> (q & 63)) & 1ull);
         };
-        // first feasible explicit candidate with index in [t0, t1)
-        auto scan = [&](int t0, int t1) -> int32_t {
-            const int e = t1 < 64 ? t1 : 64;
-            for (int t = t0; t < e; ++t) {
-                const int32_t q = __builtin_amdgcn_readlane(q0, t);
-                if (feasible(q)) return q;
+        auto cand = [&](int tb) __attribute__((always_inline)) -> int32_t {       // lane's candidate tb + lane, or -1
+            const int t = tb + lane;
+            if (t >= nc) return -1;
+            return tb == 0 ? q0 : a.pool[off + t];
+        };
+        // first feasible explicit candidate with index in [t0, t1): a few
+        // candidates one at a time in scalar code, longer ranges 64 at a time
+        auto scan = [&](int t0, int t1) __attribute__((always_inline)) -> int32_t {
+            if (t1 <= 64 && t1 - t0 <= 4) {
+                for (int t = t0; t < t1; ++t) {
+                    const int32_t q = __builtin_amdgcn_readlane(q0, t);
+                    if ((F.get(q >> 6) >> (q & 63)) & 1ull) return q;
+                }
+                return kNoDest;
             }
-            for (int t = t0 > 64 ? t0 : 64; t < t1; ++t) {
-                const int32_t q = cand_at(x, s, off, t);
-                if (feasible(q)) return q;
+            for (int tb = (t0 / 64) * 64; tb < t1; tb += 64) {
+                const int t = tb + lane;
+                const int32_t q = t >= t0 && t < t1 ? cand(tb) : -1;
+                if (const uint64_t m = __ballot(feasible(q))) return __builtin_amdgcn_readlane(q, __builtin_ctzll(m));
             }
             return kNoDest;
         };
@@ -273,33 +309,42 @@ struct RegSlack {
         if (!zflag) return scan(0, nc);
         const int nbefore = nbz & 0x7fffffff;
         if (const int32_t q = scan(0, nbefore); q >= 0) return q;
-        // zero group: feasible parts other than pv outside the explicit list
-        uint64_t listed[KP];
-#pragma unroll
-        for (int j = 0; j < KP; ++j) listed[j] = (pv >> 6) == j ? 1ull << (pv & 63) : 0ull;
-        for (int t = 0; t < nc; ++t) {
-            const int32_t q = t < 64 ? __builtin_amdgcn_readlane(q0, t) : cand_at(x, s, off, t);
-#pragma unroll
-            for (int j = 0; j < KP; ++j)
-                if ((q >> 6) == j) listed[j] |= 1ull << (q & 63);
+        // zero group: the first feasible part other than pv outside the
+        // explicit list.  The candidates ahead of the zero group were all
+        // infeasible just above, so only those after it can hide a feasible
+        // listed part.
+        auto listed_after = [&](int32_t p) __attribute__((always_inline)) -> bool {
+            for (int tb = (nbefore / 64) * 64; tb < nc; tb += 64) {
+                const int t = tb + lane;
+                if (__ballot(t >= nbefore && cand(tb) == p)) return true;
+            }
+            return false;
+        };
+        for (int j = 0; j < KP; ++j) {
+            uint64_t m = F.get(j);
+            if ((pv >> 6) == j) m &= ~(1ull << (pv & 63));
+            for (; m; m &= m - 1) {
+                const int32_t p = 64 * j + __builtin_ctzll(m);
+                if (!listed_after(p)) return p;
+            }
         }
-#pragma unroll
-        for (int j = 0; j < KP; ++j)
-            if (const uint64_t m = F[j] & ~listed[j]) return 64 * j + __builtin_ctzll(m);
         return scan(nbefore, nc);
     }
     RK_INL void apply(const Ctx&, int32_t, int32_t d, int32_t pv, int32_t w0, int32_t w1) {
         const int lane = __lane_id();
 #pragma unroll
         for (int j = 0; j < KP; ++j) {
+            int32_t a0 = s0.get(j), a1 = s1.get(j);
             if ((d >> 6) == j && lane == (d & 63)) {
-                s0[j] -= w0;
-                if (NC > 1) s1[j] -= w1;
+                a0 -= w0;
+                a1 -= NC > 1 ? w1 : 0;
             }
             if ((pv >> 6) == j && lane == (pv & 63)) {
-                s0[j] += w0;
-                if (NC > 1) s1[j] += w1;
+                a0 += w0;
+                a1 += NC > 1 ? w1 : 0;
             }
+            s0.set(j, a0);
+            s1.set(j, a1);
         }
     }
 };
@@ -322,9 +367,15 @@ RK_INL void resolve_chunk(const Ctx& x, S& st, int s0, int cnt, int first = 0) {
         w1 = x.w.w1[s];
     }
     int32_t mine = kNoDest;
-    // the staged candidates of the next mover are read one step ahead (they
-    // do not depend on the slack); the pool is read at use, when nc > kCL
-    int32_t qs = x.w.cl[(s0 + first) * kCL + (lane & (kCL - 1))];
+    // the first 64 candidates of the next mover are read one step ahead (they
+    // do not depend on the slack): the staged ones from LDS into qs, the rest
+    // from the pool into qg.  The two stay separate loads, selected by value
+    // at use (a select of the two addresses would make one flat load).
+    int32_t qs = x.w.cl[(s0 + first) * kCL + (lane & (kCL - 1))], qg = -1;
+    {
+        const int32_t nc0 = __builtin_amdgcn_readlane(nc, first);
+        if (nc0 > kCL && lane >= kCL && lane < nc0) qg = x.a->pool[__builtin_amdgcn_readlane(off, first) + lane];
+    }
     for (int i = first; i < cnt; ++i) {
         const int32_t vi = __builtin_amdgcn_readlane(v, i);
         const int32_t pvi = __builtin_amdgcn_readlane(pv, i);
@@ -333,9 +384,13 @@ RK_INL void resolve_chunk(const Ctx& x, S& st, int s0, int cnt, int first = 0) {
         const int32_t nbzi = __builtin_amdgcn_readlane(nbz, i);
         const int32_t nci = __builtin_amdgcn_readlane(nc, i);
         const int32_t offi = __builtin_amdgcn_readlane(off, i);
-        int32_t q0 = lane < nci ? qs : -1;
-        if (nci > kCL && lane >= kCL && lane < nci) q0 = x.a->pool[offi + lane];
-        if (i + 1 < cnt) qs = x.w.cl[(s0 + i + 1) * kCL + (lane & (kCL - 1))];
+        const int32_t q0 = lane < nci ? (lane < kCL ? qs : qg) : -1;
+        if (i + 1 < cnt) {
+            const int32_t ncn = __builtin_amdgcn_readlane(nc, i + 1);
+            qs = x.w.cl[(s0 + i + 1) * kCL + (lane & (kCL - 1))];
+            qg = -1;
+            if (ncn > kCL && lane >= kCL && lane < ncn) qg = x.a->pool[__builtin_amdgcn_readlane(off, i + 1) + lane];
+        }
         const int di = st.decide(x, s0 + i, vi, pvi, nci, nbzi, offi, w0i, w1i, q0);
         if (lane == i) mine = di;
         if (di >= 0) st.apply(x, vi, di, pvi, w0i, w1i);
@@ -343,115 +398,106 @@ RK_INL void resolve_chunk(const Ctx& x, S& st, int s0, int cnt, int first = 0) {
     if (lane >= first && lane < cnt) x.w.dst[s] = mine;
 }
 
-constexpr int kSpecB = 8;    // vertex weights below 2^kSpecB in a speculative chunk
 constexpr int kSpecRounds = 8;   // speculative rounds per chunk; the walk finishes the rest
+constexpr int kSpecMinRun = 4;   // shortest run of slots between barriers worth a speculative call
 
-// Exact speculative resolution of window slots [s0, s0+cnt) (cnt <= 64) by
-// wave 0 in the cut-optimisation pass, for ncon == NC <= 2, lists of at most
-// kCL candidates and no zero group.  Returns the number of leading slots it
-// resolved: 0 when the chunk does not qualify, fewer than cnt when
-// kSpecRounds rounds did not finish it (the walk takes over from there).
-//
-// Rounds start at the first unresolved lane f, whose prefix is exact.  Every
-// lane >= f decides against the slack at f; the lanes that move set their bit
-// in the per-part masks m_in[dest] and m_out[source].  Each lane then rebuilds
-// the exact slack it would see after the lanes before it (within the round)
-// applied their guesses: the weight moved in or out of a part by lanes < i is
-// a popcount of that part's mask restricted to lanes < i, summed over the
-// bit planes of the weights.  If every candidate the lane looked at keeps its
-// verdict under that slack, its guess is the sequential decision (given that
-// the lanes before it are).  The lanes before the first failing one u commit
-// their moves; the next round starts at u, so a round always commits at least
-// one lane and a chunk takes one round per wrong guess.  Committed moves are
-// in the LDS slack, so the walk can continue from the first unresolved slot.
+// Net weight moved into a part by the lanes of masks mi (in) and mo (out):
+// one lane at a time when they are few, else by the bit planes of the lanes'
+// weights (weighted popcounts; __popcll is unsigned, so cast first).
 template <int NC>
-RK_INL int spec_chunk(const Ctx& x, uint64_t* m_in, uint64_t* m_out, int s0, int cnt) {
-    const int lane = __lane_id();
-    const int s = s0 + lane;
-    const bool act = lane < cnt;
-    int32_t pv = 0, nc = 0, nbz = 0, w0 = 0, w1 = 0;
-    if (act) {
-        const int4 r = x.w.rec[s];
-        pv = r.y;
-        nc = r.z;
-        nbz = r.w;
-        w0 = x.w.w0[s];
-        if (NC > 1) w1 = x.w.w1[s];
-    }
-    if (__ballot(act && (nc > kCL || nbz < 0 || (uint32_t)(w0 | w1) >= (1u << kSpecB)))) return 0;
-    // bit planes of the weights: P0[b] / P1[b] = lanes whose weight has bit b
-    int B = 0;
-    uint64_t P0[kSpecB], P1[kSpecB];
-#pragma unroll
-    for (int b = 0; b < kSpecB; ++b) {
-        P0[b] = __ballot((w0 >> b) & 1);
-        P1[b] = NC > 1 ? __ballot((w1 >> b) & 1) : 0ull;
-        if (P0[b] | P1[b]) B = b + 1;
-    }
-    int32_t c[kCL];
-    {
-        const int4 ca = *(const int4*)&x.w.cl[s * kCL];
-        const int4 cb = *(const int4*)&x.w.cl[s * kCL + 4];
-        const int32_t cc[kCL] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
-#pragma unroll
-        for (int t = 0; t < kCL; ++t) c[t] = t < nc ? cc[t] : 0;
-    }
-    int32_t* S = x.slack;
-    int32_t d = kNoDest;
-    const uint64_t bit = 1ull << lane, below = bit - 1ull;
-    int f = 0;
-    for (int round = 0; f < cnt && round < kSpecRounds; ++round) {
-        const bool live = act && lane >= f;
-        // 1. guess against the slack at f
-        int32_t sv0[kCL], sv1[kCL];
-#pragma unroll
-        for (int t = 0; t < kCL; ++t) {
-            sv0[t] = sv1[t] = 0;
-            if (live && t < nc) {
-                sv0[t] = S[c[t] * NC];
-                if (NC > 1) sv1[t] = S[c[t] * NC + 1];
+RK_INL void net_inflow(const Ctx& x, uint64_t mi, uint64_t mo, int B, long long& n0, long long& n1) {
+    n0 = 0;
+    n1 = 0;
+    if (B == 1) {                                              // every weight is 1
+        n0 = (long long)((int)__popcll(mi) - (int)__popcll(mo));
+        n1 = n0;
+    } else if ((int)(__popcll(mi) + __popcll(mo)) <= B) {
+        for (uint64_t m = mi; m; m &= m - 1) {
+            const int j = __builtin_ctzll(m);
+            n0 += x.wv[j];
+            if (NC > 1) n1 += x.wv[64 + j];
+        }
+        for (uint64_t m = mo; m; m &= m - 1) {
+            const int j = __builtin_ctzll(m);
+            n0 -= x.wv[j];
+            if (NC > 1) n1 -= x.wv[64 + j];
+        }
+    } else {
+        for (int b = 0; b < B; ++b) {
+            const uint64_t p0 = x.pl[b];
+            n0 += (long long)((int)__popcll(mi & p0) - (int)__popcll(mo & p0)) * (1ll << b);
+            if (NC > 1) {
+                const uint64_t p1 = x.pl[32 + b];
+                n1 += (long long)((int)__popcll(mi & p1) - (int)__popcll(mo & p1)) * (1ll << b);
             }
         }
+    }
+}
+
+// Speculative rounds over slots [from, lim) of a chunk whose lists hold at
+// most CL candidates (CL <= kCL, a multiple of 4); writes the decisions of
+// the resolved slots and returns the first unresolved one.
+template <int NC, int CL>
+RK_INL int spec_rounds(const Ctx& x, uint64_t* m_in, uint64_t* m_out, int s0, bool act, int32_t pv, int32_t nc,
+                       int32_t w0, int32_t w1, int from, int lim, int B) {
+    const int lane = __lane_id();
+    const int s = s0 + lane;
+    // the staged candidate list, one register per position
+    int32_t c[CL];
+    {
+        const int4* cl4 = (const int4*)&x.w.cl[(act ? s : s0) * kCL];
+#pragma unroll
+        for (int t4 = 0; t4 < CL / 4; ++t4) {
+            const int4 v = 4 * t4 < nc ? cl4[t4] : make_int4(0, 0, 0, 0);
+            c[4 * t4] = 4 * t4 < nc ? v.x : 0;            // entries past the list are not staged
+            c[4 * t4 + 1] = 4 * t4 + 1 < nc ? v.y : 0;
+            c[4 * t4 + 2] = 4 * t4 + 2 < nc ? v.z : 0;
+            c[4 * t4 + 3] = 4 * t4 + 3 < nc ? v.w : 0;
+        }
+    }
+    int32_t* S = x.slack;
+    auto fits_at = [&](int32_t q, long long n0, long long n1) __attribute__((always_inline)) -> bool {   // q's slack less the net inflow
+        return w0 <= (long long)S[q * NC] - n0 && (NC < 2 || w1 <= (long long)S[q * NC + 1] - n1);
+    };
+    int32_t d = kNoDest;
+    const uint64_t bit = 1ull << lane, below = bit - 1ull;
+    int f = from;
+    for (int round = 0; f < lim && round < kSpecRounds; ++round) {
+        const bool live = act && lane >= f;
+        // 1. guess against the slack at f: the first candidate that fits
         int T = -1;
 #pragma unroll
-        for (int t = kCL - 1; t >= 0; --t)
-            if (live && t < nc && w0 <= sv0[t] && (NC < 2 || w1 <= sv1[t])) T = t;
+        for (int t = CL - 1; t >= 0; --t)
+            if (live && t < nc && fits_at(c[t], 0, 0)) T = t;
         int32_t g = kNoDest;
 #pragma unroll
-        for (int t = 0; t < kCL; ++t)
+        for (int t = 0; t < CL; ++t)
             if (t == T) g = c[t];
         // 2. announce the guessed moves
         if (live && g >= 0) {
             atomicOr((unsigned long long*)&m_in[g], (unsigned long long)bit);
             atomicOr((unsigned long long*)&m_out[pv], (unsigned long long)bit);
         }
-        // 3. re-check every candidate up to the guess under the exact prefix
+        // 3. re-check, under the exact prefix, every candidate up to the guess
+        //    that a lane before this one moved into or out of (the others keep
+        //    the verdict they had at f): first mark them, then check them one
+        //    at a time (usually none or one per lane)
         const int last = T >= 0 ? T : nc - 1;
+        uint32_t touched = 0;
+#pragma unroll
+        for (int t = 0; t < CL; ++t)
+            if (live && t <= last && ((m_in[c[t]] | m_out[c[t]]) & below)) touched |= 1u << t;
         bool ok = true;
-#pragma unroll
-        for (int t = 0; t < kCL; ++t) {
-            const bool q = live && t <= last;
-            if (__ballot(q) == 0) break;
-            uint64_t mi = 0, mo = 0;
-            if (q) {
-                mi = m_in[c[t]] & below;
-                mo = m_out[c[t]] & below;
-            }
-            long long n0 = 0, n1 = 0;                      // net weight moved in
-#pragma unroll
-            for (int b = 0; b < kSpecB; ++b) {             // (__popcll is unsigned: cast before subtracting)
-                if (b >= B) break;
-                n0 += (long long)((int)__popcll(mi & P0[b]) - (int)__popcll(mo & P0[b])) * (1ll << b);
-                if (NC > 1) n1 += (long long)((int)__popcll(mi & P1[b]) - (int)__popcll(mo & P1[b])) * (1ll << b);
-            }
-            if (q) {
-                const bool fits_t = w0 <= (long long)sv0[t] - n0 && (NC < 2 || w1 <= (long long)sv1[t] - n1);
-                if (fits_t != (t == T)) ok = false;
-            }
+        for (; touched; touched &= touched - 1) {
+            const int t = __builtin_ctz(touched);
+            const int32_t q = x.w.cl[s * kCL + t];
+            long long n0, n1;
+            net_inflow<NC>(x, m_in[q] & below, m_out[q] & below, B, n0, n1);
+            if (fits_at(q, n0, n1) != (t == T)) ok = false;
         }
         // 4. commit the lanes before the first failure, clear the masks
         const uint64_t bad = __ballot(live && !ok);
-        const int u = bad ? __builtin_ctzll(bad) : cnt;
+        const int u = bad ? __builtin_ctzll(bad) : lim;
         if (live && lane < u) {
             d = g;
             if (g >= 0) {
@@ -473,22 +519,96 @@ RK_INL int spec_chunk(const Ctx& x, uint64_t* m_in, uint64_t* m_out, int s0, int
     return f;
 }
 
+// Exact speculative resolution of window slots [s0+from, s0+cnt) (cnt <= 64)
+// by wave 0 in the cut-optimisation pass, for ncon == NC <= 2.  A slot whose
+// mover has more than kCL candidates or a zero group is a barrier: the slots before the first barrier are speculated, and
+// the function returns the first slot f it did not resolve.  *walk_end is
+// where the walk that takes over from f should stop: after the run of
+// barriers at f and any short gaps between barriers (fewer than
+// kSpecMinRun slots, not worth a speculative call), or, as -cnt, at cnt
+// when the rounds ran out before a barrier.
+//
+// Rounds start at the first unresolved lane f, whose prefix is exact.  Every
+// lane >= f decides against the slack at f; the lanes that move set their bit
+// in the per-part masks m_in[dest] and m_out[source].  Each lane then rebuilds
+// the exact slack it would see after the lanes before it (within the round)
+// applied their guesses: the weight moved in or out of a part by lanes < i is
+// a popcount of that part's mask restricted to lanes < i, summed over the
+// bit planes of the weights.  If every candidate the lane looked at keeps its
+// verdict under that slack, its guess is the sequential decision (given that
+// the lanes before it are).  The lanes before the first failing one u commit
+// their moves; the next round starts at u, so a round always commits at least
+// one lane and a chunk takes one round per wrong guess.  Committed moves are
+// in the LDS slack, so the walk can continue from the first unresolved slot.
+template <int NC>
+RK_INL int spec_chunk(const Ctx& x, uint64_t* m_in, uint64_t* m_out, int s0, int cnt, int from, int* walk_end) {
+    const int lane = __lane_id();
+    const int s = s0 + lane;
+    int32_t pv = 0, nc = 0, nbz = 0, w0 = 0, w1 = 0;
+    if (lane < cnt) {
+        const int4 r = x.w.rec[s];
+        pv = r.y;
+        nc = r.z;
+        nbz = r.w;
+        w0 = x.w.w0[s];
+        if (NC > 1) w1 = x.w.w1[s];
+    }
+    const uint64_t dq =
+        __ballot(lane >= from && lane < cnt && (nc > kCL || nbz < 0));
+    const int lim = dq ? __builtin_ctzll(dq) : cnt;
+    {
+        int e = lim;   // barriers and short gaps from lim on
+        while (e < cnt) {
+            if ((dq >> e) & 1ull) {
+                ++e;
+                continue;
+            }
+            const uint64_t rest = dq >> e;
+            const int run = rest ? __builtin_ctzll(rest) : cnt - e;
+            if (run >= kSpecMinRun) break;
+            e += run;
+        }
+        *walk_end = e;
+    }
+    if (lim == from) return from;
+    const bool act = lane >= from && lane < lim;
+    if (!act) nc = 0;
+    // every lane's weights, and their bit planes pl[b] (pl[32 + b] for the
+    // second constraint) = the lanes whose weight has bit b, for the net
+    // inflow sums of the verification; B = planes in use
+    x.wv[lane] = w0;
+    if (NC > 1) x.wv[64 + lane] = w1;
+    int wor = act ? (w0 | w1) : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wor |= __shfl_xor(wor, o, 64);
+    const int B = 32 - __builtin_clz((uint32_t)__builtin_amdgcn_readfirstlane(wor) | 1u);
+    for (int b = 0; b < B; ++b) {
+        const uint64_t m0 = __ballot(act && ((w0 >> b) & 1));
+        const uint64_t m1 = NC > 1 ? __ballot(act && ((w1 >> b) & 1)) : 0ull;
+        if (lane == 0) {
+            x.pl[b] = m0;
+            if (NC > 1) x.pl[32 + b] = m1;
+        }
+    }
+    const int f = __ballot(act && nc > 8)
+                      ? spec_rounds<NC, kCL>(x, m_in, m_out, s0, act, pv, nc, w0, w1, from, lim, B)
+                      : spec_rounds<NC, 8>(x, m_in, m_out, s0, act, pv, nc, w0, w1, from, lim, B);
+    if (f < lim) *walk_end = -cnt;   // rounds ran out: the walk takes the rest
+    return f;
+}
+
 // L: the k*ncon slack/threshold tables live in LDS (k*ncon <= kLdsKC);
 // otherwise they use a.gscratch.  KP > 0: the walk keeps the slack in
 // registers (k <= 64*KP, ncon == NC <= 2).  NC > 0: cut-optimisation chunks
 // are resolved speculatively where spec_chunk applies.
 template <bool L, int KP, int NC>
-__global__ __launch_bounds__(kAT) void k_admit(AdmitArgs arg, const AdmitArgs* __restrict__ table) {
-    // a batched level passes one argument block per graph (workgroup)
-    AdmitArgs a = arg;
-    if (table) {
-        a = table[blockIdx.x];
-        a.mode = arg.mode;
-    }
+RK_INL void admit_body(const AdmitArgs& a, const int mode) {
     static_assert(KP == 0 || L, "register slack needs the LDS tables");
     using Slack = std::conditional_t<(KP > 0), RegSlack<(KP > 0 ? KP : 1), NC>, TableSlack<L>>;
     extern __shared__ int32_t s_dyn[];
     __shared__ int s_w[kAT / 64];
+    __shared__ int32_t s_wv[128];
+    __shared__ uint64_t s_pl[64];
     if (*a.done) return;
     const int tid = threadIdx.x;
     const int kc = a.k * a.ncon;
@@ -589,7 +709,7 @@ __global__ __launch_bounds__(kAT) void k_admit(AdmitArgs arg, const AdmitArgs* _
     w.w1 = w.w0 + kWin;
     w.dst = w.w1 + kWin;
     w.cl = w.dst + kWin;
-    const Ctx x{&a, slack, fm, om, w};
+    const Ctx x{&a, mode, slack, fm, om, w, s_wv, s_pl};
     int nout = 0;
     for (int W0 = 0; W0 < nmov; W0 += kWin) {
         const int wn = min(kWin, nmov - W0);
@@ -601,21 +721,43 @@ __global__ __launch_bounds__(kAT) void k_admit(AdmitArgs arg, const AdmitArgs* _
             w.off[tid] = off;
             w.w0[tid] = a.vwgt ? a.vwgt[(int64_t)r.x * a.ncon] : 1;
             w.w1[tid] = a.ncon > 1 ? (a.vwgt ? a.vwgt[(int64_t)r.x * a.ncon + 1] : 1) : 0;
+            // the first kCL candidates: every load issued before the first
+            // store, then written in 16-byte blocks
             const int lim = r.z < kCL ? r.z : kCL;
-            for (int t = 0; t < lim; ++t) w.cl[tid * kCL + t] = a.pool[off + t];
+            const int32_t* src = a.pool + off;
+            int32_t cv[kCL];
+#pragma unroll
+            for (int t = 0; t < kCL; ++t) cv[t] = t < lim ? src[t] : 0;
+            int4* dst4 = (int4*)&w.cl[tid * kCL];
+#pragma unroll
+            for (int t4 = 0; t4 < kCL / 4; ++t4)
+                if (4 * t4 < lim) dst4[t4] = make_int4(cv[4 * t4], cv[4 * t4 + 1], cv[4 * t4 + 2], cv[4 * t4 + 3]);
         }
         __syncthreads();
+        // a speculative call that runs out of rounds (many moves conflict)
+        // turns speculation off for the rest of the window
+        bool spec_on = mode == 1;
         if ((tid >> 6) == 0)
             for (int s0 = 0; s0 < wn; s0 += 64) {
                 const int cnt = min(64, wn - s0);
-                int first = 0;
-                if constexpr (NC > 0)
-                    if (a.mode == 1) first = spec_chunk<NC>(x, m_in, m_out, s0, cnt);
-                if (first == cnt) continue;
-                Slack st;
-                st.load(x);
-                resolve_chunk(x, st, s0, cnt, first);
-                st.store(x);
+                int i = 0;
+                while (i < cnt) {
+                    int e = cnt;   // the walk takes [i, e)
+                    if constexpr (NC > 0)
+                        if (spec_on) {
+                            i = spec_chunk<NC>(x, m_in, m_out, s0, cnt, i, &e);
+                            if (i == cnt) break;
+                            if (e < 0) {
+                                e = -e;
+                                spec_on = false;
+                            }
+                        }
+                    Slack st;
+                    st.load(x);
+                    resolve_chunk(x, st, s0, e, i);
+                    st.store(x);
+                    i = e;
+                }
             }
         __syncthreads();
         const int d = tid < wn ? w.dst[tid] : kNoDest;
@@ -634,8 +776,17 @@ __global__ __launch_bounds__(kAT) void k_admit(AdmitArgs arg, const AdmitArgs* _
     if (tid == 0) {
         *a.nmoves = nout;
         *a.passes += 1;
-        if (a.mode == 1 && nout == 0) *a.done = 1;
+        if (mode == 1 && nout == 0) *a.done = 1;
     }
+}
+
+// One pass over the graph described by arg, or, for a batched level, one per
+// workgroup over the graph described by table[blockIdx.x] (mode from arg).
+template <bool L, int KP, int NC>
+__global__ __launch_bounds__(kAT) void k_admit(AdmitArgs arg, const AdmitArgs* __restrict__ table) {
+    AdmitArgs a = arg;
+    if (table) a = table[blockIdx.x];
+    admit_body<L, KP, NC>(a, arg.mode);
 }
 
 }  // namespace
