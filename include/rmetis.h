@@ -94,8 +94,10 @@ int32_t rmetis_hip_kway_device(idx_t nvtxs, idx_t ncon, const idx_t* d_xadj, con
  * CSR with global vertex ids: graph i owns vertices [gvtx[i], gvtx[i+1]) and
  * their rows, and its edges stay inside that range.  gvtx[ngraphs+1],
  * nparts[ngraphs], seeds[ngraphs] and objval[ngraphs] are host arrays;
- * d_xadj[n+1], d_adjncy, d_vwgt[n*ncon] (required), d_adjwgt (may be NULL)
- * and d_part[n] are device pointers, n = gvtx[ngraphs].  options are shared
+ * d_xadj[n+1], d_adjncy, d_vwgt[n*ncon], d_adjwgt and d_part[n] are device
+ * pointers, n = gvtx[ngraphs]; d_vwgt and d_adjwgt may be NULL (for every
+ * graph: absent vertex weights are not the same as ones, since the coarse
+ * vertices of an unweighted graph keep weight 1, graph/mod.rs:300-314).  options are shared
  * (RTYPE must be Greedy; NCUTS attempts per graph); attempt a of graph i uses
  * seed seeds[i] + a (options' SEED + a when seeds is NULL; a zero seed draws
  * entropy) and graph i keeps its strictly best cut, so every graph's part
@@ -114,6 +116,39 @@ int32_t rmetis_hip_kway_batch_device(idx_t ngraphs, const idx_t* gvtx, idx_t nco
 int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
                               const idx_t* vwgt, const idx_t* adjwgt, const idx_t* nparts, const real_t* ubvec,
                               const idx_t* options, const uint64_t* seeds, idx_t* objval, idx_t* part);
+
+/* Serving (no counterpart in the reference): a request loop with continuous
+ * batching.  Each request is one graph with its own nparts and seed; a
+ * server runs up to `slots` batched calls at once (one HIP stream each),
+ * every idle slot taking the oldest pending request plus the pending
+ * requests that can share its batch (same ncon, ubvec and options but the
+ * seed) within the device-memory budget (<= 0: 80 % of the free memory at
+ * creation) and max_batch_vertices (<= 0: 2^25).  A request's result equals
+ * that of RMETIS_PartGraphKway with its seed, whatever it was batched with.
+ * Returns NULL on error (rmetis_hip_last_error()). */
+typedef struct rmetis_hip_server rmetis_hip_server;
+rmetis_hip_server* rmetis_hip_server_create(int32_t slots, int64_t memory_budget, int64_t max_batch_vertices);
+
+/* Queues one request; host arrays are validated (src/graph/validate.rs:16-142,
+ * nparts in [2, nvtxs]) and copied.  seed 0 means options' SEED.  Returns the
+ * request id (> 0) or a METIS error code (< 0). */
+int64_t rmetis_hip_server_submit(rmetis_hip_server* server, idx_t nvtxs, idx_t ncon, const idx_t* xadj,
+                                 const idx_t* adjncy, const idx_t* vwgt, const idx_t* adjwgt, idx_t nparts,
+                                 const real_t* ubvec, const idx_t* options, uint64_t seed);
+
+/* Result of request id: METIS_OK with part[nvtxs] and objval written (and
+ * the request forgotten), 0 while it is pending (wait = 0), or its METIS
+ * error code.  times_ms[3] (may be NULL) = queued, running, total
+ * milliseconds; *batch_graphs = graphs of the batched call that ran it. */
+int32_t rmetis_hip_server_result(rmetis_hip_server* server, int64_t id, int32_t wait, idx_t* part, idx_t* objval,
+                                 double* times_ms, int32_t* batch_graphs);
+
+/* out[5] = batched calls run, requests served, batches in flight, most
+ * batches in flight at once, most graphs in one batch. */
+int32_t rmetis_hip_server_stats(rmetis_hip_server* server, int64_t* out);
+
+/* Stops the slots after their current batch; pending requests fail. */
+void rmetis_hip_server_destroy(rmetis_hip_server* server);
 
 /* Graph::edge_cut, src/graph/mod.rs:123-137 (host pointers; adjwgt may be NULL). */
 int32_t rmetis_hip_edge_cut(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* adjwgt,

@@ -319,8 +319,10 @@ def part_graph_kway(graph, nparts, tpwgts=None, ubvec=None, options=None):
 
 def pack_batch(graphs):
     """Lays graphs out one after another in one CSR with global vertex ids.
-    Returns (gvtx, xadj, adjncy, vwgt, adjwgt, ncon); missing vertex weights
-    become ones, missing edge weights ones when any graph has edge weights."""
+    Returns (gvtx, xadj, adjncy, vwgt, adjwgt, ncon).  Vertex weights must be
+    given for all graphs or for none (absent weights are not the same as
+    ones: coarse vertices of an unweighted graph keep weight 1); missing edge
+    weights become ones when any graph has edge weights."""
     if not graphs:
         raise InputError("a batch needs at least one graph")
     ncon = graphs[0].ncon
@@ -336,8 +338,10 @@ def pack_batch(graphs):
     xadj = np.concatenate([g.xadj[:-1].astype(np.int64) + eoff[i] for i, g in enumerate(graphs)]
                           + [np.array([eoff[-1]])]).astype(np.int32)
     adjncy = np.concatenate([g.adjncy.astype(np.int64) + gvtx[i] for i, g in enumerate(graphs)]).astype(np.int32)
-    vwgt = np.concatenate([g.vwgt if g.vwgt is not None else np.ones(n * ncon, np.int32)
-                           for g, n in zip(graphs, nv)]).astype(np.int32)
+    has_vw = [g.vwgt is not None for g in graphs]
+    if any(has_vw) and not all(has_vw):
+        raise InputError("vertex weights must be given for every graph of a batch or for none")
+    vwgt = np.concatenate([g.vwgt for g in graphs]).astype(np.int32) if all(has_vw) else None
     adjwgt = None
     if any(g.adjwgt is not None for g in graphs):
         adjwgt = np.concatenate([g.adjwgt if g.adjwgt is not None else np.ones(len(g.adjncy), np.int32)
@@ -366,6 +370,80 @@ def part_graph_kway_batch(graphs, nparts, ubvec=None, options=None, seeds=None):
                                           _p(part))
     _raise(rc)
     return [PartitionResult(part=part[gvtx[i]:gvtx[i + 1]].copy(), objval=int(obj[i])) for i in range(G)]
+
+
+@dataclass
+class ServedResult:
+    """A served request's result: part and objval as PartitionResult, the
+    queued / running / total milliseconds, and the graphs of its batch."""
+    part: np.ndarray
+    objval: int
+    queued_ms: float
+    running_ms: float
+    total_ms: float
+    batch_graphs: int
+
+
+class Server:
+    """k-way requests with continuous batching (rmetis_hip_server_*): up to
+    `slots` batched calls run at once; every request's result equals that of
+    part_graph_kway with its seed, whatever it is batched with."""
+
+    def __init__(self, slots=2, memory_budget=0, max_batch_vertices=0):
+        self._h = _lib.lib().rmetis_hip_server_create(int(slots), int(memory_budget), int(max_batch_vertices))
+        if not self._h:
+            raise InternalError(_lib.last_error())
+        self._n = {}
+
+    def submit(self, graph, nparts, ubvec=None, options=None, seed=0):
+        options = options if options is not None else Options()
+        ub = None if ubvec is None else np.ascontiguousarray(ubvec, np.float32)
+        rid = _lib.lib().rmetis_hip_server_submit(
+            self._h, graph.nvtxs, graph.ncon, _p(graph.xadj), _p(graph.adjncy), _p(graph.vwgt), _p(graph.adjwgt),
+            int(nparts), _p(ub, _lib.f32p), _p(options.to_c_array()), C.c_uint64(int(seed)))
+        if rid < 0:
+            _raise(int(rid))
+        self._n[rid] = graph.nvtxs
+        return rid
+
+    def result(self, rid, wait=True):
+        """ServedResult, or None while pending (wait=False)."""
+        if rid not in self._n:
+            raise InputError(f"unknown or already collected request id {rid}")
+        part = np.zeros(self._n[rid], np.int32)
+        obj = C.c_int32()
+        t = np.zeros(3, np.float64)
+        bg = C.c_int32()
+        rc = _lib.lib().rmetis_hip_server_result(self._h, int(rid), int(bool(wait)), _p(part), C.byref(obj),
+                                                 _p(t, _lib.f64p), C.byref(bg))
+        if rc == 0:
+            return None
+        del self._n[rid]
+        _raise(rc)
+        return ServedResult(part, obj.value, float(t[0]), float(t[1]), float(t[2]), bg.value)
+
+    def stats(self):
+        out = np.zeros(5, np.int64)
+        _raise(_lib.lib().rmetis_hip_server_stats(self._h, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return dict(zip(["batches", "served", "inflight_batches", "max_inflight_batches", "max_batch_graphs"],
+                        map(int, out)))
+
+    def close(self):
+        if self._h:
+            _lib.lib().rmetis_hip_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def device_count():

@@ -39,6 +39,12 @@ SIGNATURES = {
                                                  C.c_void_p, C.POINTER(HipStats)]),
     "rmetis_hip_kway_batch": (C.c_int32, [C.c_int32, i32p, C.c_int32, i32p, i32p, i32p, i32p, i32p, f32p, i32p,
                                           C.POINTER(C.c_uint64), i32p, i32p]),
+    "rmetis_hip_server_create": (C.c_void_p, [C.c_int32, C.c_int64, C.c_int64]),
+    "rmetis_hip_server_submit": (C.c_int64, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, C.c_int32,
+                                             f32p, i32p, C.c_uint64]),
+    "rmetis_hip_server_result": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, i32p, i32p, f64p, i32p]),
+    "rmetis_hip_server_stats": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "rmetis_hip_server_destroy": (None, [C.c_void_p]),
     "rmetis_hip_edge_cut": (C.c_int32, [C.c_int32, i32p, i32p, i32p, i32p, i32p]),
     "rmetis_hip_partition_weights": (C.c_int32, [C.c_int32, C.c_int32, i32p, i32p, C.c_int32, i32p]),
     "rmetis_hip_gain_table": (C.c_int32, [C.c_int32, i32p, i32p, i32p, i32p, C.c_int32, i32p, i32p, i32p,

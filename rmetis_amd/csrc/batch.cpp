@@ -87,7 +87,6 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
     if (!o.rtype_greedy)
         throw Error(-4, "rtype Fm is not provided by the GPU k-way path; set options[METIS_OPTION_RTYPE]=1 "
                         "(Greedy)");
-    if (!g.vwgt) throw Error(-2, "a batch needs explicit vertex weights");
     if (gvtx[0] != 0 || gvtx[G] != g.n) throw Error(-2, "graph offsets must start at 0 and end at nvtxs");
     const int32_t ncon = g.ncon;
     std::vector<int32_t> gk(G);
@@ -128,10 +127,17 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
         for (int32_t i = 0; i < G; ++i) gn[i] = gvtx[i + 1] - gvtx[i];
         fill_gseg(G, d_gvtx, put(cx, gn), gseg0, s);
     }
-    // per-graph totals (graph/mod.rs:392-410) and the thresholds of every
-    // level: with explicit vertex weights they are the same at each level
+    // per-graph totals (graph/mod.rs:392-410).  With explicit vertex weights
+    // they, and so the thresholds, are the same at every level; without
+    // (every coarse vertex then counts 1, as the reference's None weights
+    // stay None when contracted) a level's total is its vertex count and
+    // the thresholds are rebuilt per level.
+    const bool unit = g.vwgt == nullptr;
     std::vector<std::vector<int32_t>> total(G, std::vector<int32_t>(ncon));
-    {
+    if (unit) {
+        for (int32_t i = 0; i < G; ++i)
+            for (int c = 0; c < ncon; ++c) total[i][c] = gvtx[i + 1] - gvtx[i];
+    } else {
         long long* d_sum = cx.arena.get<long long>((int64_t)G * ncon);
         graph_column_sums(g, d_gvtx, G, d_sum, s);
         long long* h = (long long*)cx.host_stage2(sizeof(long long) * (size_t)G * ncon);
@@ -148,14 +154,19 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
     }
     std::vector<double> ub(ncon);
     for (int32_t c = 0; c < ncon; ++c) ub[c] = ubvec ? (double)ubvec[c] : 1.0 + (double)o.ufactor / 1000.0;
-    {
+    // floor(target*ub) tables of the graphs (kway.rs:44-49, 67-69, 86-88);
+    // ltot(i) gives graph i's per-constraint totals at the level
+    auto upload_thresholds = [&](auto ltot) {
         std::vector<long long> fm(kc_total), om(kc_total);
-        for (int32_t i = 0; i < G; ++i)
-            level_thresholds(gk[i], ncon, nullptr, total[i].data(), ub.data(), fm.data() + gkc[i],
-                             om.data() + gkc[i]);
+        std::vector<int32_t> t(ncon);
+        for (int32_t i = 0; i < G; ++i) {
+            ltot(i, t.data());
+            level_thresholds(gk[i], ncon, nullptr, t.data(), ub.data(), fm.data() + gkc[i], om.data() + gkc[i]);
+        }
         put(cx, fm, B.rb.fmax);
         put(cx, om, B.rb.omax);
-    }
+    };
+    if (!unit) upload_thresholds([&](int32_t i, int32_t* t) { std::copy(total[i].begin(), total[i].end(), t); });
     int32_t* fin_heavy = cx.arena.get<int32_t>(g.n);
     const int32_t fin_nheavy = heavy_list_sync(g, fin_heavy, B.rb, s);
     const std::vector<Extent> ext0 = read_extents(cx, g, gseg0, G, d_ranges);
@@ -310,8 +321,10 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
                     tgt.assign((size_t)k * ncon, 0.0);
                     for (int32_t p = 0; p < k; ++p)
                         for (int32_t c = 0; c < ncon; ++c)
-                            tgt[(size_t)p * ncon + c] = (1.0 / (double)k) * (double)total[i][c];
-                    if (!grow_host(e.n, ncon, x, a, vw, k, tgt.data(), rng[i], hp)) failed.store(i);
+                            tgt[(size_t)p * ncon + c] =
+                                (1.0 / (double)k) * (double)(unit ? e.n : total[i][c]);
+                    if (!grow_host(e.n, ncon, x, a, unit ? nullptr : vw, k, tgt.data(), rng[i], hp))
+                        failed.store(i);
                 }
             };
             const unsigned nt = std::min<unsigned>(hw, (unsigned)G);
@@ -393,6 +406,10 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
                 RK_HIP(hipEventCreate(&e1));
                 RK_HIP(hipEventRecord(e0, s));
             }
+            if (unit)
+                upload_thresholds([&](int32_t i, int32_t* t) {
+                    for (int c = 0; c < ncon; ++c) t[c] = L.ext[i].n > 0 ? L.ext[i].n : 1;
+                });
             part_weights_batch(L.g, pf, L.gseg, d_gkc, d_gk, kc_total, B.rb.pw, s);   // greedy.rs:50
             RK_HIP(hipMemsetAsync(d_gdone, 0, sizeof(int32_t) * (size_t)G, s));
             refine_pass_batch(L.g, bl, 0, L.nheavy, L.heavy, B.rb, pf, s);                  // greedy.rs:60-123

@@ -8,6 +8,7 @@
 // pooled stream and arena, so concurrent calls are safe.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -15,6 +16,7 @@
 #include "../../include/rmetis.h"
 #include "driver.hpp"
 #include "profiler.hpp"
+#include "serve.hpp"
 
 namespace rk {
 
@@ -163,6 +165,11 @@ void fill_stats(const KwayStats& ks, rmetis_hip_stats* out) {
 
 using namespace rk;
 
+struct rmetis_hip_server {
+    std::unique_ptr<rk::Server> s;
+    int32_t device = 0;
+};
+
 extern "C" {
 
 int32_t RMETIS_SetDefaultOptions(idx_t* options) {
@@ -227,7 +234,7 @@ int32_t rmetis_hip_kway_batch_device(idx_t ngraphs, const idx_t* gvtx, idx_t nco
                                      const idx_t* nparts, const real_t* ubvec, const idx_t* options,
                                      const uint64_t* seeds, idx_t* objval, idx_t* d_part, rmetis_hip_stats* stats) {
     return guarded([&]() -> int32_t {
-        if (!gvtx || !d_xadj || !d_adjncy || !d_vwgt || !nparts || !objval || !d_part) return METIS_ERROR_INPUT;
+        if (!gvtx || !d_xadj || !d_adjncy || !nparts || !objval || !d_part) return METIS_ERROR_INPUT;
         if (ngraphs <= 0 || ncon <= 0 || gvtx[ngraphs] <= 0) return METIS_ERROR_INPUT;
         const Options o = parse_options(options);
         ContextGuard cx;
@@ -251,7 +258,7 @@ int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, cons
                               const idx_t* vwgt, const idx_t* adjwgt, const idx_t* nparts, const real_t* ubvec,
                               const idx_t* options, const uint64_t* seeds, idx_t* objval, idx_t* part) {
     return guarded([&]() -> int32_t {
-        if (!gvtx || !xadj || !adjncy || !vwgt || !nparts || !objval || !part) return METIS_ERROR_INPUT;
+        if (!gvtx || !xadj || !adjncy || !nparts || !objval || !part) return METIS_ERROR_INPUT;
         if (ngraphs <= 0 || ncon <= 0) return METIS_ERROR_INPUT;
         // graph/validate.rs:16-142 per graph: every row stays inside its graph
         if (gvtx[0] != 0) return METIS_ERROR_INPUT;
@@ -271,8 +278,9 @@ int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, cons
         for (int64_t j = 0; j < xadj[n]; ++j) cnt[adjncy[j]]++;
         for (int32_t v = 0; v < n; ++v)
             if (cnt[v] != xadj[v + 1] - xadj[v]) return METIS_ERROR_INPUT;
-        for (int64_t i = 0; i < (int64_t)n * ncon; ++i)
-            if (vwgt[i] <= 0) return METIS_ERROR_INPUT;
+        if (vwgt)
+            for (int64_t i = 0; i < (int64_t)n * ncon; ++i)
+                if (vwgt[i] <= 0) return METIS_ERROR_INPUT;
         const Options o = parse_options(options);
         ContextGuard cx;
         DevGraph g = upload_graph(*cx.c, n, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
@@ -284,6 +292,106 @@ int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, cons
         std::copy(obj.begin(), obj.end(), objval);
         return METIS_OK;
     });
+}
+
+rmetis_hip_server* rmetis_hip_server_create(int32_t slots, int64_t memory_budget, int64_t max_batch_vertices) {
+    rmetis_hip_server* out = nullptr;
+    const int32_t rc = guarded([&]() -> int32_t {
+        if (slots < 1 || slots > 16) return METIS_ERROR_INPUT;
+        int cnt = 0;
+        if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0)
+            throw Error(-4, "no HIP device is available (the k-way path runs only on the GPU)");
+        int dev = 0;
+        RK_HIP(hipGetDevice(&dev));
+        if (memory_budget <= 0) {
+            size_t free_b = 0, total_b = 0;
+            RK_HIP(hipMemGetInfo(&free_b, &total_b));
+            memory_budget = (int64_t)(free_b / 10 * 8);
+        }
+        if (max_batch_vertices <= 0) max_batch_vertices = 1ll << 25;
+        auto* h = new rmetis_hip_server();
+        h->device = dev;
+        h->s = std::make_unique<Server>(dev, slots, memory_budget, max_batch_vertices);
+        out = h;
+        return METIS_OK;
+    });
+    return rc == METIS_OK ? out : nullptr;
+}
+
+int64_t rmetis_hip_server_submit(rmetis_hip_server* server, idx_t nvtxs, idx_t ncon, const idx_t* xadj,
+                                 const idx_t* adjncy, const idx_t* vwgt, const idx_t* adjwgt, idx_t nparts,
+                                 const real_t* ubvec, const idx_t* options, uint64_t seed) {
+    int64_t id = 0;
+    const int32_t rc = guarded([&]() -> int32_t {
+        if (!server || !xadj || !adjncy) return METIS_ERROR_INPUT;
+        if (nvtxs <= 0 || ncon <= 0 || nparts < 2) return METIS_ERROR_INPUT;           // c_api.rs:157-167
+        if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt)) return METIS_ERROR_INPUT;   // validate.rs:16-142
+        if (nparts > nvtxs) return METIS_ERROR_INPUT;                                 // kway.rs:34-38
+        const Options o = parse_options(options);
+        if (!o.rtype_greedy)
+            throw Error(-4, "rtype Fm is not provided by the GPU k-way path; set options[METIS_OPTION_RTYPE]=1 "
+                            "(Greedy)");
+        std::vector<int64_t> tot(ncon, 0);
+        for (int64_t v = 0; v < nvtxs; ++v)
+            for (int c = 0; c < ncon; ++c) tot[c] += vwgt ? vwgt[v * ncon + c] : 1;
+        for (int c = 0; c < ncon; ++c)
+            if (tot[c] > INT32_MAX)
+                throw Error(-4, "total vertex weight of a constraint exceeds the int32 range");
+        auto r = std::make_shared<ServeRequest>();
+        r->n = nvtxs;
+        r->ncon = ncon;
+        r->k = nparts;
+        const int64_t m = xadj[nvtxs];
+        r->xadj.assign(xadj, xadj + nvtxs + 1);
+        r->adjncy.assign(adjncy, adjncy + m);
+        if (vwgt) r->vwgt.assign(vwgt, vwgt + (int64_t)nvtxs * ncon);
+        if (adjwgt) r->adjwgt.assign(adjwgt, adjwgt + m);
+        if (ubvec) r->ubvec.assign(ubvec, ubvec + ncon);
+        r->opt = o;
+        r->seed = seed != 0 ? seed : o.seed;
+        id = server->s->submit(r);
+        return METIS_OK;
+    });
+    return rc == METIS_OK ? id : (int64_t)rc;
+}
+
+int32_t rmetis_hip_server_result(rmetis_hip_server* server, int64_t id, int32_t wait, idx_t* part, idx_t* objval,
+                                 double* times_ms, int32_t* batch_graphs) {
+    return guarded([&]() -> int32_t {
+        if (!server) return METIS_ERROR_INPUT;
+        std::shared_ptr<ServeRequest> r = server->s->result(id, wait != 0);
+        if (!r) return 0;
+        if (r->status != METIS_OK) throw Error(r->status, r->error);
+        if (part) std::copy(r->part.begin(), r->part.end(), part);
+        if (objval) *objval = r->objval;
+        if (times_ms) {
+            using ms = std::chrono::duration<double, std::milli>;
+            times_ms[0] = ms(r->t_start - r->t_submit).count();
+            times_ms[1] = ms(r->t_done - r->t_start).count();
+            times_ms[2] = ms(r->t_done - r->t_submit).count();
+        }
+        if (batch_graphs) *batch_graphs = r->batch_graphs;
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_server_stats(rmetis_hip_server* server, int64_t* out) {
+    return guarded([&]() -> int32_t {
+        if (!server || !out) return METIS_ERROR_INPUT;
+        const ServerStats st = server->s->stats();
+        out[0] = st.batches;
+        out[1] = st.served;
+        out[2] = st.inflight_batches;
+        out[3] = st.max_inflight_batches;
+        out[4] = st.max_batch_graphs;
+        return METIS_OK;
+    });
+}
+
+void rmetis_hip_server_destroy(rmetis_hip_server* server) {
+    if (!server) return;
+    (void)hipSetDevice(server->device);
+    delete server;
 }
 
 int32_t rmetis_hip_edge_cut(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* adjwgt,

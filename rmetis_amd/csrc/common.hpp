@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -74,15 +75,15 @@ __device__ inline int group_lane() { return __lane_id() & (G - 1); }
 
 // Compute units of the current device, queried once per device.
 inline int device_cus() {
-    static int cache[64] = {0};
+    static std::atomic<int> cache[64] = {};   // calls on several threads (server slots) share it
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
-    if (cache[dev] == 0) {
+    if (cache[dev].load(std::memory_order_relaxed) == 0) {
         int c = 0;
         if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
-        cache[dev] = c;
+        cache[dev].store(c, std::memory_order_relaxed);
     }
-    return cache[dev];
+    return cache[dev].load(std::memory_order_relaxed);
 }
 
 __device__ inline void wave_sync() {

@@ -1,0 +1,244 @@
+// serve.cpp -- k-way requests served with continuous batching.
+//
+// Requests (one graph each, with its own nparts and seed) are queued on
+// submit.  Every slot is a worker thread with its own pooled context (HIP
+// stream and arena): when idle it takes the oldest pending request and every
+// later pending request that can share a batched call with it (same ncon,
+// vertex-weight presence, ubvec and options other than the seed), as far as
+// the memory budget and
+// the per-batch vertex cap allow, and runs them through kway_batch_device.
+// Requests that arrive while batches are on the GPU are picked up by the next
+// free slot, so the work of several batches overlaps on the device.  Every
+// result equals that of a separate partition_kway call with the request's
+// seed (kway.rs:20-141), whatever the batch it ran in.
+#include "serve.hpp"
+
+#include <algorithm>
+
+namespace rk {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// Device memory a request's hierarchy and work buffers may take: the input
+// CSR, the coarse levels (at most as large again), the refinement buffers
+// (tile records, movers, candidate pool) and the contraction tables.
+int64_t footprint_bytes(int64_t n, int64_t m, int64_t ncon) {
+    const int64_t input = 4 * (n + 1) + 8 * m + 4 * n * ncon;
+    return 3 * input + 36 * m + 160 * n + (1 << 20);
+}
+
+bool same_setup(const ServeRequest& a, const ServeRequest& b) {
+    const Options &x = a.opt, &y = b.opt;
+    // graphs without vertex weights keep unit coarse weights, so they batch
+    // only with each other
+    return a.ncon == b.ncon && a.vwgt.empty() == b.vwgt.empty() && a.ubvec == b.ubvec &&
+           x.ctype_shem == y.ctype_shem && x.iptype == y.iptype &&
+           x.rtype_greedy == y.rtype_greedy && x.ncuts == y.ncuts && x.niter == y.niter && x.ufactor == y.ufactor;
+}
+
+}  // namespace
+
+Server::Server(int device, int slots, int64_t budget, int64_t max_batch_vertices)
+    : device_(device), budget_(budget), max_batch_v_(max_batch_vertices) {
+    if (slots < 1) throw Error(-2, "a server needs at least one slot");
+    for (int i = 0; i < slots; ++i) threads_.emplace_back([this]() { worker(); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_work_.notify_all();
+    for (auto& t : threads_) t.join();
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto& r : pending_) {
+        r->status = -4;
+        r->error = "the server was destroyed before the request ran";
+        r->done = true;
+    }
+    pending_.clear();
+    cv_done_.notify_all();
+}
+
+int64_t Server::submit(std::shared_ptr<ServeRequest> r) {
+    r->footprint = footprint_bytes(r->n, (int64_t)r->adjncy.size(), r->ncon);
+    if (r->footprint > budget_)
+        throw Error(-3, "request needs about " + std::to_string(r->footprint >> 20) +
+                            " MiB of device memory, above the server's budget");
+    std::lock_guard<std::mutex> lk(mu_);
+    r->id = next_id_++;
+    r->t_submit = Clock::now();
+    reqs_[r->id] = r;
+    pending_.push_back(r);
+    cv_work_.notify_one();
+    return r->id;
+}
+
+std::shared_ptr<ServeRequest> Server::result(int64_t id, bool wait) {
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = reqs_.find(id);
+    if (it == reqs_.end()) throw Error(-2, "unknown request id " + std::to_string(id));
+    std::shared_ptr<ServeRequest> r = it->second;
+    if (wait) cv_done_.wait(lk, [&]() { return r->done; });
+    if (!r->done) return nullptr;
+    reqs_.erase(it);
+    return r;
+}
+
+ServerStats Server::stats() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return st_;
+}
+
+void Server::worker() {
+    (void)hipSetDevice(device_);
+    Context* cx = nullptr;
+    try {
+        cx = acquire_context();
+    } catch (const Error&) {
+        return;   // no usable device: the queue is failed by the destructor
+    }
+    for (;;) {
+        std::vector<std::shared_ptr<ServeRequest>> batch;
+        int64_t fp = 0;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            // the oldest request runs as soon as it fits next to the batches
+            // in flight (alone it always fits: submit checked the budget)
+            cv_work_.wait(lk, [&]() {
+                return stop_ || (!pending_.empty() && (st_.inflight_batches == 0 ||
+                                                       in_use_ + pending_.front()->footprint <= budget_));
+            });
+            if (stop_) break;
+            const std::shared_ptr<ServeRequest> head = pending_.front();
+            int64_t nv = 0;
+            for (auto it = pending_.begin(); it != pending_.end();) {
+                const auto& r = *it;
+                const bool first = batch.empty();
+                if (!first && (!same_setup(*head, *r) || in_use_ + fp + r->footprint > budget_ ||
+                               nv + r->n > max_batch_v_)) {
+                    ++it;
+                    continue;
+                }
+                fp += r->footprint;
+                nv += r->n;
+                batch.push_back(r);
+                it = pending_.erase(it);
+            }
+            in_use_ += fp;
+            st_.inflight_batches += 1;
+            st_.max_inflight_batches = std::max(st_.max_inflight_batches, st_.inflight_batches);
+        }
+        const auto t_start = Clock::now();
+        int32_t status = 1;
+        std::string error;
+        const int32_t G = (int32_t)batch.size();
+        std::vector<int32_t> objval(G, 0);
+        std::vector<int32_t> host_part;
+        std::vector<int32_t> gvtx(G + 1, 0);
+        try {
+            // the union CSR: graphs one after another, global vertex ids
+            const int32_t ncon = batch[0]->ncon;
+            int64_t n = 0, m = 0;
+            bool any_w = false;
+            for (int32_t i = 0; i < G; ++i) {
+                gvtx[i] = (int32_t)n;
+                n += batch[i]->n;
+                m += (int64_t)batch[i]->adjncy.size();
+                any_w = any_w || !batch[i]->adjwgt.empty();
+            }
+            if (n > INT32_MAX || m > INT32_MAX) throw Error(-4, "batch too large for int32 CSR");
+            gvtx[G] = (int32_t)n;
+            Arena& ar = cx->arena;
+            ar.reset();
+            cx->upload_reset();
+            int32_t* hx = (int32_t*)cx->upload_stage(sizeof(int32_t) * (size_t)(n + 1));
+            int32_t* ha = (int32_t*)cx->upload_stage(sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1));
+            const bool unit = batch[0]->vwgt.empty();
+            int32_t* hv = unit ? nullptr : (int32_t*)cx->upload_stage(sizeof(int32_t) * (size_t)(n * ncon));
+            int32_t* hw = any_w ? (int32_t*)cx->upload_stage(sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1))
+                                : nullptr;
+            std::vector<int32_t> nparts(G);
+            std::vector<uint64_t> seeds(G);
+            int64_t eo = 0;
+            for (int32_t i = 0; i < G; ++i) {
+                const ServeRequest& r = *batch[i];
+                const int32_t v0 = gvtx[i];
+                for (int32_t v = 0; v < r.n; ++v) hx[v0 + v] = (int32_t)(eo + r.xadj[v]);
+                for (size_t j = 0; j < r.adjncy.size(); ++j) ha[eo + j] = r.adjncy[j] + v0;
+                if (hv) std::copy(r.vwgt.begin(), r.vwgt.end(), hv + (int64_t)v0 * ncon);
+                if (hw) {
+                    if (r.adjwgt.empty()) std::fill(hw + eo, hw + eo + r.adjncy.size(), 1);
+                    else std::copy(r.adjwgt.begin(), r.adjwgt.end(), hw + eo);
+                }
+                eo += (int64_t)r.adjncy.size();
+                nparts[i] = r.k;
+                seeds[i] = r.seed;
+            }
+            hx[n] = (int32_t)m;
+            DevGraph g;
+            g.n = (int32_t)n;
+            g.m = (int32_t)m;
+            g.ncon = ncon;
+            g.xadj = ar.get<int32_t>(n + 1);
+            g.adjncy = ar.get<int32_t>(m);
+            g.vwgt = hv ? ar.get<int32_t>(n * ncon) : nullptr;
+            g.adjwgt = hw ? ar.get<int32_t>(m) : nullptr;
+            hipStream_t s = cx->stream;
+            RK_HIP(hipMemcpyAsync(g.xadj, hx, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyHostToDevice, s));
+            RK_HIP(hipMemcpyAsync(g.adjncy, ha, sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, s));
+            if (hv) RK_HIP(hipMemcpyAsync(g.vwgt, hv, sizeof(int32_t) * (size_t)(n * ncon), hipMemcpyHostToDevice, s));
+            if (hw) RK_HIP(hipMemcpyAsync(g.adjwgt, hw, sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, s));
+            int32_t* d_part = ar.get<int32_t>(n);
+            const float* ub = batch[0]->ubvec.empty() ? nullptr : batch[0]->ubvec.data();
+            kway_batch_device(*cx, g, G, gvtx.data(), nparts.data(), ub, batch[0]->opt, seeds.data(), d_part,
+                              objval.data(), nullptr);
+            host_part.resize(n);
+            int32_t* hp = (int32_t*)cx->host_stage2(sizeof(int32_t) * (size_t)n);
+            RK_HIP(hipMemcpyAsync(hp, d_part, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
+            RK_HIP(hipStreamSynchronize(s));
+            std::copy(hp, hp + n, host_part.begin());
+        } catch (const Error& e) {
+            status = e.code;
+            error = e.what();
+        } catch (const std::bad_alloc&) {
+            status = -3;
+            error = "host out of memory";
+        } catch (const std::exception& e) {
+            status = -4;
+            error = e.what();
+        }
+        cx->arena.reset();
+        cx->upload_reset();
+        const auto t_done = Clock::now();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (int32_t i = 0; i < G; ++i) {
+                ServeRequest& r = *batch[i];
+                r.t_start = t_start;
+                r.t_done = t_done;
+                r.batch_graphs = G;
+                r.status = status;
+                r.error = error;
+                if (status == 1) {
+                    r.part.assign(host_part.begin() + gvtx[i], host_part.begin() + gvtx[i + 1]);
+                    r.objval = objval[i];
+                }
+                r.done = true;
+            }
+            in_use_ -= fp;
+            st_.inflight_batches -= 1;
+            st_.batches += 1;
+            st_.served += G;
+            st_.max_batch_graphs = std::max<int64_t>(st_.max_batch_graphs, G);
+        }
+        cv_done_.notify_all();
+        cv_work_.notify_all();
+    }
+    release_context(cx);
+}
+
+}  // namespace rk

@@ -154,7 +154,7 @@ def test_generators_shape():
 
 def test_pack_batch_layout():
     from graphs import path
-    g1 = R.Graph(4, 1, *cycle4())
+    g1 = R.Graph(4, 1, *cycle4(), vwgt=np.ones(4, np.int32))
     x, a = path(5)
     g2 = R.Graph(5, 1, x, a, vwgt=np.arange(1, 6), adjwgt=np.full(len(a), 3))
     gvtx, xadj, adjncy, vwgt, adjwgt, ncon = R.pack_batch([g1, g2])
@@ -167,6 +167,10 @@ def test_pack_batch_layout():
     assert np.array_equal(adjwgt, [1] * 8 + [3] * len(a))
     # the packed union is itself a valid CSR with no edge between graphs
     R.Graph(9, 1, xadj, adjncy, vwgt, adjwgt)
+    # vertex weights: all or none (absent weights are not ones for a batch)
+    assert R.pack_batch([R.Graph(4, 1, *cycle4()), R.Graph(5, 1, x, a)])[3] is None
+    with pytest.raises(R.InputError):
+        R.pack_batch([R.Graph(4, 1, *cycle4()), g2])
 
 
 def _batch_raw(gvtx, xadj, adjncy, vwgt, nparts):
@@ -186,7 +190,7 @@ def test_batch_abi_input_errors():
     xx = np.concatenate([x, x[1:] + x[-1]])
     aa = np.concatenate([a, a + 4])
     ones = np.ones(8, np.int32)
-    assert _batch_raw([0, 4, 8], xx, aa, None, [2, 2])[0] == -2            # vwgt required
+    assert _batch_raw([0, 4, 8], xx, aa, None, [2, 2])[0] in (1, -4)       # valid without vwgt (-4: no GPU)
     assert _batch_raw([0, 4, 8], xx, aa, ones, [2, 5])[0] == -2            # nparts > nvtxs
     assert _batch_raw([0, 4, 8], xx, aa, ones, [1, 2])[0] == -2            # nparts < 2
     assert _batch_raw([0, 4, 4, 8], np.concatenate([x, x[1:] + x[-1]]), aa, ones, [2, 2, 2])[0] == -2  # empty graph
@@ -195,3 +199,11 @@ def test_batch_abi_input_errors():
     assert _batch_raw([0, 4, 8], xx, cross, ones, [2, 2])[0] == -2
     rc, obj = _batch_raw([0, 4, 8], xx, aa, np.zeros(8, np.int32), [2, 2])
     assert rc == -2 and (obj == -99).all()                                # outputs untouched on error
+
+
+def test_server_without_gpu_fails_loudly():
+    if R.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(R.InternalError, match="no HIP device"):
+        R.Server(slots=2)
+    assert _lib.lib().rmetis_hip_server_create(0, 0, 0) is None      # slots < 1

@@ -99,6 +99,21 @@ def test_batch_single_graph_and_order_independence():
     _check(rev, alone)
 
 
+def test_batch_unweighted_graphs():
+    # without vertex weights coarse vertices keep weight 1 (graph/mod.rs:300-314)
+    graphs = [_grid_plain(20, 30), _grid_plain(64, 64), _grid_plain(7, 90), _grid_plain(128, 40)]
+    ks = [4, 16, 2, 64]
+    opts = R.Options.for_kway(seed=4, ncuts=2, rtype=R.RType.Greedy)
+    _check(R.part_graph_kway_batch(graphs, ks, options=opts), _alone(graphs, ks, lambda i: opts))
+    with pytest.raises(R.InputError):                                   # weighted and unweighted mixed
+        R.part_graph_kway_batch([graphs[0], _weighted_grid(10, 10, 1)], [2, 2], options=opts)
+
+
+def _grid_plain(rows, cols):
+    g = synth.grid(rows, cols)
+    return R.Graph(g.n, 1, g.xadj, g.adjncy)
+
+
 def test_batch_errors():
     graphs, ks = _batch()
     opts = R.Options.for_kway(seed=1, rtype=R.RType.Greedy)
