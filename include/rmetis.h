@@ -136,7 +136,18 @@ int64_t rmetis_hip_server_submit(rmetis_hip_server* server, idx_t nvtxs, idx_t n
                                  const idx_t* adjncy, const idx_t* vwgt, const idx_t* adjwgt, idx_t nparts,
                                  const real_t* ubvec, const idx_t* options, uint64_t seed);
 
-/* Result of request id: METIS_OK with part[nvtxs] and objval written (and
+/* rmetis_hip_server_submit for a graph already in device memory: d_xadj,
+ * d_adjncy, d_vwgt, d_adjwgt (the weights may be NULL) are read in place and
+ * must stay valid until the result is collected, which also means the part
+ * has been written to d_part[nvtxs] (device).  Not validated, like
+ * rmetis_hip_kway_device. */
+int64_t rmetis_hip_server_submit_device(rmetis_hip_server* server, idx_t nvtxs, idx_t ncon, const idx_t* d_xadj,
+                                        const idx_t* d_adjncy, const idx_t* d_vwgt, const idx_t* d_adjwgt,
+                                        idx_t nparts, const real_t* ubvec, const idx_t* options, uint64_t seed,
+                                        idx_t* d_part);
+
+/* Result of request id: METIS_OK with part[nvtxs] (host requests; may be
+ * NULL, and is not written for device requests) and objval written (and
  * the request forgotten), 0 while it is pending (wait = 0), or its METIS
  * error code.  times_ms[3] (may be NULL) = queued, running, total
  * milliseconds; *batch_graphs = graphs of the batched call that ran it. */

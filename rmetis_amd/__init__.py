@@ -406,11 +406,28 @@ class Server:
         self._n[rid] = graph.nvtxs
         return rid
 
+    def submit_device(self, nvtxs, ncon, d_xadj, d_adjncy, d_vwgt, d_adjwgt, nparts, d_part, ubvec=None,
+                      options=None, seed=0):
+        """A request whose CSR already lives in device memory (_lib.DeviceArray
+        or raw pointers); the part is written to d_part, and the arrays must
+        stay alive until result() has returned."""
+        ptr = lambda a: None if a is None else (a.ptr if hasattr(a, "ptr") else a)
+        options = options if options is not None else Options()
+        ub = None if ubvec is None else np.ascontiguousarray(ubvec, np.float32)
+        rid = _lib.lib().rmetis_hip_server_submit_device(
+            self._h, int(nvtxs), int(ncon), ptr(d_xadj), ptr(d_adjncy), ptr(d_vwgt), ptr(d_adjwgt), int(nparts),
+            _p(ub, _lib.f32p), _p(options.to_c_array()), C.c_uint64(int(seed)), ptr(d_part))
+        if rid < 0:
+            _raise(int(rid))
+        self._n[rid] = None
+        return rid
+
     def result(self, rid, wait=True):
-        """ServedResult, or None while pending (wait=False)."""
+        """ServedResult (part None for device requests), or None while
+        pending (wait=False)."""
         if rid not in self._n:
             raise InputError(f"unknown or already collected request id {rid}")
-        part = np.zeros(self._n[rid], np.int32)
+        part = np.zeros(self._n[rid], np.int32) if self._n[rid] is not None else None
         obj = C.c_int32()
         t = np.zeros(3, np.float64)
         bg = C.c_int32()

@@ -42,6 +42,9 @@ SIGNATURES = {
     "rmetis_hip_server_create": (C.c_void_p, [C.c_int32, C.c_int64, C.c_int64]),
     "rmetis_hip_server_submit": (C.c_int64, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, C.c_int32,
                                              f32p, i32p, C.c_uint64]),
+    "rmetis_hip_server_submit_device": (C.c_int64, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                                    C.c_void_p, C.c_void_p, C.c_int32, f32p, i32p, C.c_uint64,
+                                                    C.c_void_p]),
     "rmetis_hip_server_result": (C.c_int32, [C.c_void_p, C.c_int64, C.c_int32, i32p, i32p, f64p, i32p]),
     "rmetis_hip_server_stats": (C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     "rmetis_hip_server_destroy": (None, [C.c_void_p]),

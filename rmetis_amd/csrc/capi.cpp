@@ -355,6 +355,40 @@ int64_t rmetis_hip_server_submit(rmetis_hip_server* server, idx_t nvtxs, idx_t n
     return rc == METIS_OK ? id : (int64_t)rc;
 }
 
+int64_t rmetis_hip_server_submit_device(rmetis_hip_server* server, idx_t nvtxs, idx_t ncon, const idx_t* d_xadj,
+                                        const idx_t* d_adjncy, const idx_t* d_vwgt, const idx_t* d_adjwgt,
+                                        idx_t nparts, const real_t* ubvec, const idx_t* options, uint64_t seed,
+                                        idx_t* d_part) {
+    int64_t id = 0;
+    const int32_t rc = guarded([&]() -> int32_t {
+        if (!server || !d_xadj || !d_adjncy || !d_part) return METIS_ERROR_INPUT;
+        if (nvtxs <= 0 || ncon <= 0 || nparts < 2 || nparts > nvtxs) return METIS_ERROR_INPUT;
+        const Options o = parse_options(options);
+        if (!o.rtype_greedy)
+            throw Error(-4, "rtype Fm is not provided by the GPU k-way path; set options[METIS_OPTION_RTYPE]=1 "
+                            "(Greedy)");
+        int32_t m = 0;
+        RK_HIP(hipMemcpy(&m, d_xadj + nvtxs, sizeof(int32_t), hipMemcpyDeviceToHost));
+        auto r = std::make_shared<ServeRequest>();
+        r->on_device = true;
+        r->n = nvtxs;
+        r->ncon = ncon;
+        r->k = nparts;
+        r->m = m;
+        r->d_xadj = d_xadj;
+        r->d_adjncy = d_adjncy;
+        r->d_vwgt = d_vwgt;
+        r->d_adjwgt = d_adjwgt;
+        r->d_part = d_part;
+        if (ubvec) r->ubvec.assign(ubvec, ubvec + ncon);
+        r->opt = o;
+        r->seed = seed != 0 ? seed : o.seed;
+        id = server->s->submit(r);
+        return METIS_OK;
+    });
+    return rc == METIS_OK ? id : (int64_t)rc;
+}
+
 int32_t rmetis_hip_server_result(rmetis_hip_server* server, int64_t id, int32_t wait, idx_t* part, idx_t* objval,
                                  double* times_ms, int32_t* batch_graphs) {
     return guarded([&]() -> int32_t {
@@ -362,7 +396,7 @@ int32_t rmetis_hip_server_result(rmetis_hip_server* server, int64_t id, int32_t 
         std::shared_ptr<ServeRequest> r = server->s->result(id, wait != 0);
         if (!r) return 0;
         if (r->status != METIS_OK) throw Error(r->status, r->error);
-        if (part) std::copy(r->part.begin(), r->part.end(), part);
+        if (part && !r->on_device) std::copy(r->part.begin(), r->part.end(), part);
         if (objval) *objval = r->objval;
         if (times_ms) {
             using ms = std::chrono::duration<double, std::milli>;

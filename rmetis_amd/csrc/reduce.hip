@@ -3,6 +3,8 @@
 // edge cut:      graph/mod.rs:123-137 (sum over directed entries, i32 wrap, /2)
 // part weights:  graph/mod.rs:140-149 (pw[part[v]][c] += vwgt[v][c], i32 wrap)
 // projection:    partition/kway.rs:81-84 (fine_part[v] = part[cmap[v]])
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -249,6 +251,16 @@ __global__ __launch_bounds__(kBlock) void k_graph_colsum(const int32_t* __restri
     }
 }
 
+// One graph's CSR into a union at vertex v0 / entry e0: row offsets shifted
+// by e0, neighbour ids by v0.
+__global__ __launch_bounds__(kBlock) void k_pack_csr(int32_t n, int64_t m, const int32_t* __restrict__ xadj,
+                                                     const int32_t* __restrict__ adjncy, int32_t v0, int32_t e0,
+                                                     int32_t* __restrict__ uxadj, int32_t* __restrict__ uadjncy) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) uxadj[v0 + i] = xadj[i] + e0;
+    for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < m; j += stride) uadjncy[e0 + j] = adjncy[j] + v0;
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_gseg(int32_t ng, const int32_t* __restrict__ gv0,
                                                       const int32_t* __restrict__ gn, int32_t* __restrict__ gseg) {
     // one block per graph
@@ -392,6 +404,16 @@ void fill_tiles(const int4* desc, int32_t ndesc, int32_t* tile_v0, int32_t* tile
 void graph_column_sums(const DevGraph& g, const int32_t* gvtx, int32_t ngraphs, long long* out, hipStream_t s) {
     if (ngraphs > 0)
         hipLaunchKernelGGL(k_graph_colsum, dim3(ngraphs), dim3(kBlock), 0, s, gvtx, g.ncon, g.vwgt, out);
+    RK_HIP(hipGetLastError());
+}
+
+void pack_csr(int32_t n, int64_t m, const int32_t* xadj, const int32_t* adjncy, int32_t v0, int32_t e0,
+              int32_t* uxadj, int32_t* uadjncy, hipStream_t s) {
+    const int64_t work = std::max<int64_t>(n, m);
+    if (work > 0)
+        hipLaunchKernelGGL(k_pack_csr, dim3((unsigned)std::min<int64_t>((work + kBlock - 1) / kBlock, 8192)),
+                           dim3(kBlock), 0, s, n, m, xadj, adjncy, v0,
+                           e0, uxadj, uadjncy);
     RK_HIP(hipGetLastError());
 }
 

@@ -33,7 +33,7 @@ bool same_setup(const ServeRequest& a, const ServeRequest& b) {
     const Options &x = a.opt, &y = b.opt;
     // graphs without vertex weights keep unit coarse weights, so they batch
     // only with each other
-    return a.ncon == b.ncon && a.vwgt.empty() == b.vwgt.empty() && a.ubvec == b.ubvec &&
+    return a.ncon == b.ncon && a.has_vwgt() == b.has_vwgt() && a.ubvec == b.ubvec &&
            x.ctype_shem == y.ctype_shem && x.iptype == y.iptype &&
            x.rtype_greedy == y.rtype_greedy && x.ncuts == y.ncuts && x.niter == y.niter && x.ufactor == y.ufactor;
 }
@@ -64,7 +64,8 @@ Server::~Server() {
 }
 
 int64_t Server::submit(std::shared_ptr<ServeRequest> r) {
-    r->footprint = footprint_bytes(r->n, (int64_t)r->adjncy.size(), r->ncon);
+    if (!r->on_device) r->m = (int64_t)r->adjncy.size();
+    r->footprint = footprint_bytes(r->n, r->m, r->ncon);
     if (r->footprint > budget_)
         throw Error(-3, "request needs about " + std::to_string(r->footprint >> 20) +
                             " MiB of device memory, above the server's budget");
@@ -147,58 +148,98 @@ void Server::worker() {
             for (int32_t i = 0; i < G; ++i) {
                 gvtx[i] = (int32_t)n;
                 n += batch[i]->n;
-                m += (int64_t)batch[i]->adjncy.size();
-                any_w = any_w || !batch[i]->adjwgt.empty();
+                m += batch[i]->m;
+                any_w = any_w || batch[i]->has_adjwgt();
             }
             if (n > INT32_MAX || m > INT32_MAX) throw Error(-4, "batch too large for int32 CSR");
             gvtx[G] = (int32_t)n;
             Arena& ar = cx->arena;
             ar.reset();
             cx->upload_reset();
-            int32_t* hx = (int32_t*)cx->upload_stage(sizeof(int32_t) * (size_t)(n + 1));
-            int32_t* ha = (int32_t*)cx->upload_stage(sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1));
-            const bool unit = batch[0]->vwgt.empty();
-            int32_t* hv = unit ? nullptr : (int32_t*)cx->upload_stage(sizeof(int32_t) * (size_t)(n * ncon));
-            int32_t* hw = any_w ? (int32_t*)cx->upload_stage(sizeof(int32_t) * (size_t)std::max<int64_t>(m, 1))
-                                : nullptr;
+            // pinned staging only for what comes from the host
+            bool any_host = false, fill_w = false;
+            for (int32_t i = 0; i < G; ++i) {
+                any_host = any_host || !batch[i]->on_device;
+                fill_w = fill_w || !batch[i]->on_device || !batch[i]->has_adjwgt();
+            }
+            const bool unit = !batch[0]->has_vwgt();
+            auto stage = [&](int64_t cnt) { return (int32_t*)cx->upload_stage(sizeof(int32_t) * (size_t)std::max<int64_t>(cnt, 1)); };
+            int32_t* hx = any_host ? stage(n) : nullptr;
+            int32_t* ha = any_host ? stage(m) : nullptr;
+            int32_t* hv = !unit && any_host ? stage(n * ncon) : nullptr;
+            int32_t* hw = any_w && fill_w ? stage(m) : nullptr;
+            int32_t* hend = stage(1);
             std::vector<int32_t> nparts(G);
             std::vector<uint64_t> seeds(G);
-            int64_t eo = 0;
-            for (int32_t i = 0; i < G; ++i) {
-                const ServeRequest& r = *batch[i];
-                const int32_t v0 = gvtx[i];
-                for (int32_t v = 0; v < r.n; ++v) hx[v0 + v] = (int32_t)(eo + r.xadj[v]);
-                for (size_t j = 0; j < r.adjncy.size(); ++j) ha[eo + j] = r.adjncy[j] + v0;
-                if (hv) std::copy(r.vwgt.begin(), r.vwgt.end(), hv + (int64_t)v0 * ncon);
-                if (hw) {
-                    if (r.adjwgt.empty()) std::fill(hw + eo, hw + eo + r.adjncy.size(), 1);
-                    else std::copy(r.adjwgt.begin(), r.adjwgt.end(), hw + eo);
-                }
-                eo += (int64_t)r.adjncy.size();
-                nparts[i] = r.k;
-                seeds[i] = r.seed;
-            }
-            hx[n] = (int32_t)m;
+            // the union in HBM: graphs from the host through the pinned
+            // arrays (one copy each per array), graphs already on the device
+            // by the packing kernel and device copies
             DevGraph g;
             g.n = (int32_t)n;
             g.m = (int32_t)m;
             g.ncon = ncon;
             g.xadj = ar.get<int32_t>(n + 1);
             g.adjncy = ar.get<int32_t>(m);
-            g.vwgt = hv ? ar.get<int32_t>(n * ncon) : nullptr;
-            g.adjwgt = hw ? ar.get<int32_t>(m) : nullptr;
+            g.vwgt = unit ? nullptr : ar.get<int32_t>(n * ncon);
+            g.adjwgt = any_w ? ar.get<int32_t>(m) : nullptr;
             hipStream_t s = cx->stream;
-            RK_HIP(hipMemcpyAsync(g.xadj, hx, sizeof(int32_t) * (size_t)(n + 1), hipMemcpyHostToDevice, s));
-            RK_HIP(hipMemcpyAsync(g.adjncy, ha, sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, s));
-            if (hv) RK_HIP(hipMemcpyAsync(g.vwgt, hv, sizeof(int32_t) * (size_t)(n * ncon), hipMemcpyHostToDevice, s));
-            if (hw) RK_HIP(hipMemcpyAsync(g.adjwgt, hw, sizeof(int32_t) * (size_t)m, hipMemcpyHostToDevice, s));
+            auto h2d = [&](int32_t* dst, const int32_t* src, int64_t cnt) {
+                if (cnt > 0) RK_HIP(hipMemcpyAsync(dst, src, sizeof(int32_t) * (size_t)cnt, hipMemcpyHostToDevice, s));
+            };
+            auto d2d = [&](int32_t* dst, const int32_t* src, int64_t cnt) {
+                if (cnt > 0)
+                    RK_HIP(hipMemcpyAsync(dst, src, sizeof(int32_t) * (size_t)cnt, hipMemcpyDeviceToDevice, s));
+            };
+            int64_t eo = 0;
+            for (int32_t i = 0; i < G; ++i) {
+                const ServeRequest& r = *batch[i];
+                const int32_t v0 = gvtx[i];
+                if (r.on_device) {
+                    pack_csr(r.n, r.m, r.d_xadj, r.d_adjncy, v0, (int32_t)eo, g.xadj, g.adjncy, s);
+                    if (!unit) d2d(g.vwgt + (int64_t)v0 * ncon, r.d_vwgt, (int64_t)r.n * ncon);
+                    if (any_w) {
+                        if (r.d_adjwgt) {
+                            d2d(g.adjwgt + eo, r.d_adjwgt, r.m);
+                        } else {
+                            std::fill(hw + eo, hw + eo + r.m, 1);
+                            h2d(g.adjwgt + eo, hw + eo, r.m);
+                        }
+                    }
+                } else {
+                    for (int32_t v = 0; v < r.n; ++v) hx[v0 + v] = (int32_t)(eo + r.xadj[v]);
+                    for (int64_t j = 0; j < r.m; ++j) ha[eo + j] = r.adjncy[j] + v0;
+                    h2d(g.xadj + v0, hx + v0, r.n);
+                    h2d(g.adjncy + eo, ha + eo, r.m);
+                    if (!unit) {
+                        std::copy(r.vwgt.begin(), r.vwgt.end(), hv + (int64_t)v0 * ncon);
+                        h2d(g.vwgt + (int64_t)v0 * ncon, hv + (int64_t)v0 * ncon, (int64_t)r.n * ncon);
+                    }
+                    if (any_w) {
+                        if (r.adjwgt.empty()) std::fill(hw + eo, hw + eo + r.m, 1);
+                        else std::copy(r.adjwgt.begin(), r.adjwgt.end(), hw + eo);
+                        h2d(g.adjwgt + eo, hw + eo, r.m);
+                    }
+                }
+                eo += r.m;
+                nparts[i] = r.k;
+                seeds[i] = r.seed;
+            }
+            *hend = (int32_t)m;
+            h2d(g.xadj + n, hend, 1);
             int32_t* d_part = ar.get<int32_t>(n);
             const float* ub = batch[0]->ubvec.empty() ? nullptr : batch[0]->ubvec.data();
             kway_batch_device(*cx, g, G, gvtx.data(), nparts.data(), ub, batch[0]->opt, seeds.data(), d_part,
                               objval.data(), nullptr);
             host_part.resize(n);
             int32_t* hp = (int32_t*)cx->host_stage2(sizeof(int32_t) * (size_t)n);
-            RK_HIP(hipMemcpyAsync(hp, d_part, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
+            for (int32_t i = 0; i < G; ++i) {
+                const ServeRequest& r = *batch[i];
+                if (r.on_device)
+                    d2d(r.d_part, d_part + gvtx[i], r.n);
+                else
+                    RK_HIP(hipMemcpyAsync(hp + gvtx[i], d_part + gvtx[i], sizeof(int32_t) * (size_t)r.n,
+                                          hipMemcpyDeviceToHost, s));
+            }
             RK_HIP(hipStreamSynchronize(s));
             std::copy(hp, hp + n, host_part.begin());
         } catch (const Error& e) {
@@ -224,7 +265,7 @@ void Server::worker() {
                 r.status = status;
                 r.error = error;
                 if (status == 1) {
-                    r.part.assign(host_part.begin() + gvtx[i], host_part.begin() + gvtx[i + 1]);
+                    if (!r.on_device) r.part.assign(host_part.begin() + gvtx[i], host_part.begin() + gvtx[i + 1]);
                     r.objval = objval[i];
                 }
                 r.done = true;

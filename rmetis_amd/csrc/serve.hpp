@@ -20,6 +20,12 @@ struct ServeRequest {
     int64_t id = 0;
     int32_t n = 0, ncon = 1, k = 2;
     std::vector<int32_t> xadj, adjncy, vwgt, adjwgt;   // graph-local ids; empty weights mean ones
+    // a graph already in device memory (the caller keeps it valid until the
+    // result is collected): read in place, and the part written to d_part
+    bool on_device = false;
+    int64_t m = 0;
+    const int32_t *d_xadj = nullptr, *d_adjncy = nullptr, *d_vwgt = nullptr, *d_adjwgt = nullptr;
+    int32_t* d_part = nullptr;
     std::vector<float> ubvec;                           // ncon entries, or empty for 1 + ufactor/1000
     Options opt;
     uint64_t seed = 0;
@@ -31,6 +37,9 @@ struct ServeRequest {
     std::vector<int32_t> part;
     int32_t objval = 0;
     int32_t batch_graphs = 0;                           // graphs of the batched call that served it
+
+    bool has_vwgt() const { return on_device ? d_vwgt != nullptr : !vwgt.empty(); }
+    bool has_adjwgt() const { return on_device ? d_adjwgt != nullptr : !adjwgt.empty(); }
 };
 
 struct ServerStats {

@@ -91,3 +91,25 @@ def test_server_errors():
         assert r.part.shape == (16,)
         with pytest.raises(R.InputError):
             srv.result(rid)                                        # already collected
+
+
+def test_served_device_requests_equal_alone():
+    # graphs already in HBM: submitted by device pointer, part written in place
+    from rmetis_amd import _lib
+    DA = _lib.DeviceArray
+    graphs = [(_rgg(30000, 41, ncon=2, vmax=8), 128, 7), (_rgg(2000, 42, ncon=2, vmax=8), 8, 8),
+              (_rgg(70000, 43, ncon=2, vmax=8), 512, 9)]
+    ub = np.array([1.05, 1.05], np.float32)
+    dev = [(DA(g.xadj), DA(g.adjncy), DA(g.vwgt), DA(g.adjwgt), DA(count=g.nvtxs, dtype=np.int32))
+           for g, _, _ in graphs]
+    with R.Server(slots=2) as srv:
+        ids = [srv.submit_device(g.nvtxs, g.ncon, *d[:4], k, d[4], ubvec=ub, options=_opts(1), seed=s)
+               for (g, k, s), d in zip(graphs, dev)]
+        host = srv.submit(graphs[1][0], 16, ubvec=ub, options=_opts(2))       # a host request in the mix
+        res = [srv.result(rid) for rid in ids]
+        hres = srv.result(host)
+    for (g, k, s), d, r in zip(graphs, dev, res):
+        alone = R.part_graph_kway(g, k, ubvec=ub, options=_opts(s))
+        assert r.part is None and r.objval == alone.objval
+        assert np.array_equal(d[4].numpy(), alone.part)
+    _check(hres, R.part_graph_kway(graphs[1][0], 16, ubvec=ub, options=_opts(2)))
