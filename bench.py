@@ -38,6 +38,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="rgg20",
                     help="rgg20 | grid256 | rmat22 | rggNN | rmatNN | batch1024 (uneven batch of that many graphs)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="serve: length of the arrival stream")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-pool", type=int, default=48, help="serve: distinct graphs the requests draw from")
+    ap.add_argument("--slots", type=int, default=3, help="serve: batched calls in flight at once")
+    ap.add_argument("--plain-sample", type=int, default=40, help="serve: requests timed on the plain path")
     ap.add_argument("--plain-stride", type=int, default=3,
                     help="batch workloads: the plain path is timed on every n-th graph (odd: grids and RGGs alike)")
     ap.add_argument("--seed", type=int, default=1)
@@ -63,6 +68,8 @@ def main():
         raise SystemExit("no HIP device")
     if args.workload.startswith("batch"):
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     t0 = time.time()
     g, k = synth.config_graph(args.workload, seed=args.seed)
     log(f"graph {g} k={k} generated in {time.time() - t0:.1f}s")
@@ -381,6 +388,155 @@ def bench_batch(args):
                      "traffic": None, "bytes_per_launch": int(bytes_per_launch),
                      "avg_launch_us": round(avg_ms * 1e3, 2), "launches": g_launch},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Open-loop serving: Poisson arrivals at --serve-rate for --serve-seconds;
+    each request partitions a graph drawn from a pool of --serve-pool graphs
+    built as the uneven batch's (n log-uniform in 2^12..2^22, ncon 2, vwgt
+    U{1..8} per constraint, adjwgt U{1..10}; uploaded to HBM once) into k parts,
+    k uniform over {8, 16, ..., 512}, ubvec (1.05, 1.05), with its own seed.
+    Requests go to a Server by device pointer; latency is submit to result.
+    The plain path (rmetis_hip_kway_device, one request at a time) is timed on
+    a sample of the same requests."""
+    import threading
+    import queue
+    import numpy as np
+    import rmetis_amd as R
+    from rmetis_amd import _lib, synth
+
+    L = _lib.lib()
+    DA = _lib.DeviceArray
+    i32 = _lib.i32p
+    t0 = time.time()
+    workers = max(1, min(16, os.cpu_count() or 1))
+    pool = synth.serve_pool(args.serve_pool, args.seed, workers=workers)
+    dev = [(DA(g.xadj), DA(g.adjncy), DA(g.vwgt), DA(g.adjwgt)) for g in pool]
+    L.rmetis_hip_device_synchronize()
+    log(f"pool of {len(pool)} graphs (n {min(g.n for g in pool)}..{max(g.n for g in pool)}, "
+        f"{sum(g.n for g in pool)} vertices) built and uploaded in {time.time() - t0:.1f}s")
+    ub = np.array([1.05, 1.05], np.float32)
+    opts = R.Options(rtype=R.RType.Greedy, ncuts=1, niter=10, ufactor=30)
+    maxn = max(g.n for g in pool)
+    ring = 256
+    outs = [DA(count=maxn, dtype=np.int32) for _ in range(ring)]
+    srv = R.Server(slots=args.slots)
+    # warm-up (untimed): every pool graph once, so arenas and staging are grown
+    wids = [srv.submit_device(g.n, 2, *d, 8, outs[i % ring], ubvec=ub, options=opts, seed=1)
+            for i, (g, d) in enumerate(zip(pool, dev)) if i < ring]
+    for w in wids:
+        srv.result(w)
+
+    def stream(rate, seconds, seed):
+        """Runs one Poisson stream through the server; returns its requests,
+        per-request (total, queued, running) ms, batch sizes and objvals."""
+        rng = np.random.default_rng(seed)
+        reqs, t = [], 0.0
+        while True:
+            t += rng.exponential(1.0 / rate)
+            if t > seconds:
+                break
+            gi = int(rng.integers(len(pool)))
+            k = min(int(2 ** rng.integers(3, 10)), pool[gi].n)
+            reqs.append((t, gi, k, len(reqs) + 1))
+        log(f"{len(reqs)} requests over {seconds:.0f}s at {rate:.0f}/s, {args.slots} slots")
+        ids = queue.Queue()
+        freed = [threading.Semaphore(0) for _ in range(ring)]
+        start = time.perf_counter() + 0.05
+
+        def submitter():
+            for i, (ta, gi, k, sd) in enumerate(reqs):
+                if i >= ring:
+                    freed[i % ring].acquire()              # request i - ring has been collected
+                dt = start + ta - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                g, d = pool[gi], dev[gi]
+                ids.put(srv.submit_device(g.n, 2, *d, k, outs[i % ring], ubvec=ub, options=opts, seed=sd))
+
+        th = threading.Thread(target=submitter, daemon=True)
+        th.start()
+        rows, objs = [], []
+        done_t = start
+        for i in range(len(reqs)):
+            r = srv.result(ids.get())
+            rows.append((r.total_ms, r.queued_ms, r.running_ms, r.batch_graphs))
+            objs.append(r.objval)
+            done_t = time.perf_counter()
+            freed[i % ring].release()
+            if i % 1000 == 999:
+                log(f"{i + 1} served")
+        th.join()
+        return reqs, np.array(rows), objs, done_t - start
+
+    b0 = srv.stats()["batches"]
+    reqs, rows, objs, elapsed = stream(args.serve_rate, args.serve_seconds, args.seed + 1)
+    nb = srv.stats()["batches"] - b0
+    value = len(reqs) / elapsed
+    lat = rows[:, 0]
+    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+    log(f"served {len(reqs)} in {elapsed:.2f}s = {value:.1f} graphs/s, p50 {p50:.1f} ms, p99 {p99:.1f} ms, "
+        f"{nb} batches")
+    # capacity: a stream at five times the rate saturates the server
+    creqs, crows, _, celapsed = stream(5 * args.serve_rate, max(3.0, args.serve_seconds / 4), args.seed + 3)
+    capacity = len(creqs) / celapsed
+    log(f"capacity: {len(creqs)} in {celapsed:.2f}s = {capacity:.1f} graphs/s, "
+        f"mean batch {crows[:, 3].mean():.1f} graphs")
+    st = srv.stats()
+    srv.close()
+    # the plain path, one request at a time, on a sample of the same requests
+    samp = np.random.default_rng(args.seed + 2).choice(len(reqs), size=min(args.plain_sample, len(reqs)),
+                                                        replace=False)
+    obj = C.c_int32()
+    plain_ms, mism = [], 0
+    for j in samp:
+        _, gi, k, seed = reqs[j]
+        g, d = pool[gi], dev[gi]
+        o = R.Options(rtype=R.RType.Greedy, ncuts=1, niter=10, ufactor=30, seed=seed).to_c_array()
+        t1 = time.perf_counter()
+        rc = L.rmetis_hip_kway_device(g.n, 2, d[0].ptr, d[1].ptr, d[2].ptr, d[3].ptr, k, None,
+                                      ub.ctypes.data_as(_lib.f32p), o.ctypes.data_as(i32), C.byref(obj),
+                                      outs[0].ptr, None)
+        if rc != 1:
+            raise RuntimeError(f"plain kway failed: {_lib.last_error()}")
+        plain_ms.append((time.perf_counter() - t1) * 1e3)
+        mism += int(obj.value != objs[j])
+    plain_ms = np.array(plain_ms)
+    plain_value = 1e3 / plain_ms.mean()
+    log(f"plain: {len(samp)} requests, {plain_value:.1f} graphs/s sequential, objval mismatches {mism}")
+    out = {
+        "metric": "graphs/s (served open-loop stream, continuous batching)",
+        "value": round(value, 2),
+        "unit": "graphs/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": len(wids),
+        "ms_per_step": round(1e3 * elapsed / len(reqs), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"serve: Poisson {args.serve_rate:.0f}/s for {args.serve_seconds:.0f}s, "
+                               f"{len(pool)}-graph pool n 2^12..2^22 ncon 2, k 8..512",
+                   "rtype": "greedy", "ubvec": [1.05, 1.05], "ncuts": 1, "slots": args.slots,
+                   "parallelism": "single GPU, continuous batching"},
+        "latency_ms": {"p50": round(p50, 2), "p99": round(p99, 2), "mean": round(float(lat.mean()), 2),
+                       "queued_p50": round(float(np.percentile(rows[:, 1], 50)), 2),
+                       "running_p50": round(float(np.percentile(rows[:, 2], 50)), 2)},
+        "batches": nb, "mean_batch_graphs": round(float(rows[:, 3].mean()), 2),
+        "max_inflight_batches": st["max_inflight_batches"],
+        "capacity": {"value": round(capacity, 2), "unit": "graphs/s", "requests": len(creqs),
+                     "rate": 5 * args.serve_rate, "mean_batch_graphs": round(float(crows[:, 3].mean()), 2),
+                     "p50_ms": round(float(np.percentile(crows[:, 0], 50)), 2)},
+        "offered_vertices_per_s": round(sum(pool[gi].n for _, gi, _, _ in reqs) / args.serve_seconds, 1),
+        "plain": {"value": round(plain_value, 2), "unit": "graphs/s", "requests": len(samp),
+                  "service_ms_p50": round(float(np.percentile(plain_ms, 50)), 2),
+                  "service_ms_p99": round(float(np.percentile(plain_ms, 99)), 2), "objval_mismatches": mism,
+                  "note": "rmetis_hip_kway_device one request at a time, same requests"},
+        "capacity_vs_plain": round(capacity / plain_value, 2),
     }
     print(json.dumps(out), flush=True)
 

@@ -123,7 +123,7 @@ int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, cons
  * every idle slot taking the oldest pending request plus the pending
  * requests that can share its batch (same ncon, ubvec and options but the
  * seed) within the device-memory budget (<= 0: 80 % of the free memory at
- * creation) and max_batch_vertices (<= 0: 2^25).  A request's result equals
+ * creation) and max_batch_vertices (<= 0: 2^23).  A request's result equals
  * that of RMETIS_PartGraphKway with its seed, whatever it was batched with.
  * Returns NULL on error (rmetis_hip_last_error()). */
 typedef struct rmetis_hip_server rmetis_hip_server;

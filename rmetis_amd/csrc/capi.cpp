@@ -308,7 +308,7 @@ rmetis_hip_server* rmetis_hip_server_create(int32_t slots, int64_t memory_budget
             RK_HIP(hipMemGetInfo(&free_b, &total_b));
             memory_budget = (int64_t)(free_b / 10 * 8);
         }
-        if (max_batch_vertices <= 0) max_batch_vertices = 1ll << 25;
+        if (max_batch_vertices <= 0) max_batch_vertices = 1ll << 23;
         auto* h = new rmetis_hip_server();
         h->device = dev;
         h->s = std::make_unique<Server>(dev, slots, memory_budget, max_batch_vertices);

@@ -41,6 +41,20 @@ public:
     T* get(int64_t count) {
         return (T*)alloc((size_t)(count > 0 ? count : 1) * sizeof(T));
     }
+    // Makes the first chunk hold at least `bytes`, so that a call whose
+    // needs are known ahead takes one allocation instead of a growing list
+    // of chunks.  Frees the current chunks (the caller has synchronised its
+    // stream) only when they are too small.
+    void reserve(size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        if (!chunks_.empty() && chunks_[0].size >= bytes) return;
+        release();
+        void* p = nullptr;
+        RK_HIP(hipMalloc(&p, bytes));
+        chunks_.push_back({(char*)p, bytes});
+        cur_ = 0;
+        off_ = 0;
+    }
     Mark mark() const { return {cur_, off_}; }
     void rewind(Mark m) {
         cur_ = m.chunk;

@@ -155,6 +155,7 @@ void Server::worker() {
             gvtx[G] = (int32_t)n;
             Arena& ar = cx->arena;
             ar.reset();
+            ar.reserve((size_t)fp);
             cx->upload_reset();
             // pinned staging only for what comes from the host
             bool any_host = false, fill_w = false;

@@ -178,8 +178,10 @@ def rmat(scale, ef, seed, a=0.57, b=0.19, c=0.19, name=None, chunk=1 << 23):
 
 
 def _batch_member(args):
-    """Graph i of the uneven batch from its drawn (i, n, k, gseed)."""
-    i, n, k, gseed = args
+    """Graph i of the uneven batch from its drawn (i, n, k, gseed[, ncon,
+    vwgt_max])."""
+    i, n, k, gseed = args[:4]
+    ncon, vmax = (args[4], args[5]) if len(args) > 4 else (1, 4)
     grng = np.random.default_rng(gseed)
     if i % 2 == 0:
         aspect = 2.0 ** grng.uniform(-2, 2)
@@ -194,12 +196,13 @@ def _batch_member(args):
         ukey, inv = np.unique(lo * g.n + hi, return_inverse=True)
         w = grng.integers(1, 11, size=len(ukey), dtype=np.int64).astype(np.int32)
         g.adjwgt = np.ascontiguousarray(w[inv])
-        g.vwgt = grng.integers(1, 5, size=g.n, dtype=np.int64).astype(np.int32)
+        g.vwgt = grng.integers(1, vmax + 1, size=g.n * ncon, dtype=np.int64).astype(np.int32)
+        g.ncon = ncon
         assert len(ukey) == m2
     else:
         deg = grng.uniform(8, 16)
         r = math.sqrt(deg / (math.pi * n))
-        g = rgg(n, gseed, r=r, wmax=10, vwgt_max=4, name=f"b{i}_rgg{n}", rng=grng)
+        g = rgg(n, gseed, r=r, wmax=10, vwgt_max=vmax, ncon=ncon, name=f"b{i}_rgg{n}", rng=grng)
     return g, min(k, g.n)
 
 
@@ -227,6 +230,28 @@ def uneven_batch(count, seed, nmin_log2=10, nmax_log2=18, workers=1):
     out = [None] * count
     for j, i in enumerate(order):
         out[i] = built[j]
+    return out
+
+
+def serve_pool(count, seed, nmin_log2=12, nmax_log2=22, workers=1):
+    """Graphs of the open-loop serving stream: built as in uneven_batch, with
+    n log-uniform in [2^nmin_log2, 2^nmax_log2] and two constraints with
+    vertex weights U{1..8} each.  Returns a list of CSR (ncon 2)."""
+    rng = np.random.default_rng(seed)
+    params = []
+    for i in range(count):
+        n = int(round(2.0 ** rng.uniform(nmin_log2, nmax_log2)))
+        gseed = int(rng.integers(0, 2**63 - 1))
+        params.append((i, n, 2, gseed, 2, 8))
+    if workers <= 1:
+        return [_batch_member(p)[0] for p in params]
+    import multiprocessing as mp
+    order = sorted(range(count), key=lambda i: -params[i][1])
+    with mp.get_context("fork").Pool(workers) as pool:
+        built = pool.map(_batch_member, [params[i] for i in order], chunksize=1)
+    out = [None] * count
+    for j, i in enumerate(order):
+        out[i] = built[j][0]
     return out
 
 
