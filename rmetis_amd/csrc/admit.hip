@@ -8,7 +8,7 @@
 // carry tens to a few thousand candidates, and every decision may depend on
 // the one before it, so this step is a latency-bound sequential walk:
 //
-//  * all 256 threads list the tiles with movers (a bitmap set by the gain
+//  * all 256 threads list the tiles with movers (a bitmap kept by the gain
 //    kernel), gather their records into one dense ordered list, and stage
 //    windows of 256 movers (record, vertex weights, first candidates) in LDS;
 //  * the running state is one LDS table slack[p][c] = fmax[p][c] - pw[p][c]:
@@ -612,6 +612,10 @@ RK_INL void admit_body(const AdmitArgs& a, const int mode) {
     if (*a.done) return;
     const int tid = threadIdx.x;
     const int kc = a.k * a.ncon;
+    // tiles the next cut pass recomputes: marked here for the graphs of a
+    // batched level (a.tile_dirty given), by k_mark_moves for one graph
+    const bool mark = mode == 1 && a.tile_dirty != nullptr;
+    if (mode == 1 && a.full && tid == 0) *a.full = 0;
     int32_t *slack, *fm, *om, *uni;
     uint64_t *m_in = nullptr, *m_out = nullptr;
     if constexpr (L) {
@@ -646,14 +650,19 @@ RK_INL void admit_body(const AdmitArgs& a, const int mode) {
     const int nwords = (a.ntiles + 31) / 32;
     const int wper = (nwords + kAT - 1) / kAT;
     const int wd0 = min(tid * wper, nwords), wd1 = min(wd0 + wper, nwords);
+    // the bits of tiles past ntiles (a finer level's) are not this graph's
+    auto word = [&](int wd) __attribute__((always_inline)) -> uint32_t {
+        const uint32_t b = a.tile_bits[wd];
+        const int past = (wd + 1) * 32 - a.ntiles;
+        return past > 0 ? b & (0xffffffffu >> past) : b;
+    };
     int nz = 0;
-    for (int wd = wd0; wd < wd1; ++wd) nz += __popc(a.tile_bits[wd]);
+    for (int wd = wd0; wd < wd1; ++wd) nz += __popc(word(wd));
     int nlisted;
     {
         int off = block_excl_scan_int(nz, s_w, &nlisted);
         for (int wd = wd0; wd < wd1; ++wd) {
-            uint32_t bits = a.tile_bits[wd];
-            if (bits) a.tile_bits[wd] = 0;
+            uint32_t bits = word(wd);
             while (bits) {
                 a.tile_list[off++] = wd * 32 + __ffs(bits) - 1;
                 bits &= bits - 1;
@@ -766,6 +775,19 @@ RK_INL void admit_body(const AdmitArgs& a, const int mode) {
         if (d >= 0) {
             a.moves[nout + pos] = make_int2(r.x, d);
             a.part[r.x] = d;
+            if (mark) {
+                // the mover's tile and its neighbours' tiles: their gain
+                // tables change; a hub sends the next pass over every tile
+                auto dirty = [&](int32_t u) __attribute__((always_inline)) {
+                    const int32_t t = (u - a.v0) >> 6;
+                    atomicOr(&a.tile_dirty[t >> 5], 1u << (t & 31));
+                };
+                dirty(r.x);
+                const int32_t b0 = a.xadj[r.x], b1 = a.xadj[r.x + 1];
+                if (b1 - b0 > kMarkMaxDeg) *a.full = 1;
+                else
+                    for (int32_t j = b0; j < b1; ++j) dirty(a.adjncy[j]);
+            }
         }
         nout += tot;
         __syncthreads();
@@ -828,7 +850,7 @@ void launch_admit(const AdmitArgs& aa, const AdmitArgs* table, int grid, int32_t
 void admit_pass(const DevGraph& g, int32_t k, int mode, int ntiles, RefineBufs& b, int32_t* part, hipStream_t s) {
     const AdmitArgs aa{mode, k, g.ncon, ntiles, b.tile_cnt, b.tile_bits, b.tile_off, b.tile_rec, b.mov, b.pool,
                        g.xadj, g.vwgt, b.fmax, b.omax, b.pw, part, b.moves, b.nmoves, b.done, b.passes,
-                       b.adm_scratch};
+                       b.adm_scratch, g.adjncy, nullptr, b.full, 0};
     launch_admit(aa, nullptr, 1, k, g.ncon, s);
 }
 

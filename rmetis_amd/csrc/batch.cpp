@@ -114,10 +114,11 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
     int32_t* d_gk = put(cx, gk);
     int32_t* d_gkc = put(cx, gkc);
     int32_t* d_gvtx = put(cx, std::vector<int32_t>(gvtx, gvtx + G + 1));
-    int32_t* d_gdone = cx.arena.get<int32_t>(3 * (int64_t)G);
+    int32_t* d_gdone = cx.arena.get<int32_t>(4 * (int64_t)G);
     int32_t* d_gnmoves = d_gdone + G;
     int32_t* d_gpasses = d_gdone + 2 * (int64_t)G;
-    RK_HIP(hipMemsetAsync(d_gdone, 0, sizeof(int32_t) * 3 * (size_t)G, s));
+    int32_t* d_gfull = d_gdone + 3 * (int64_t)G;
+    RK_HIP(hipMemsetAsync(d_gdone, 0, sizeof(int32_t) * 4 * (size_t)G, s));
     int32_t* d_ranges = cx.arena.get<int32_t>(4 * (int64_t)G);
     unsigned long long* d_cutacc = cx.arena.get<unsigned long long>(G);
     int32_t* d_cut = cx.arena.get<int32_t>(G);
@@ -383,6 +384,10 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
                 aa.done = d_gdone + i;
                 aa.passes = d_gpasses + i;
                 aa.gscratch = B.rb.adm_scratch + 3 * (int64_t)gkc[i];
+                aa.adjncy = L.g.adjncy;
+                aa.tile_dirty = B.rb.tile_dirty + tb / 32;
+                aa.full = d_gfull + i;
+                aa.v0 = e.v0;
                 adm.push_back(aa);
                 tb += padded_tiles(e.n);
             }
@@ -397,6 +402,7 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
             bl.gkc = d_gkc;
             bl.gk = d_gk;
             bl.gdone = d_gdone;
+            bl.gfull = d_gfull;
             bl.admit = put(cx, adm);
             bl.ngraphs = (int32_t)adm.size();
             bl.kmax = kmax;
@@ -413,8 +419,8 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
             part_weights_batch(L.g, pf, L.gseg, d_gkc, d_gk, kc_total, B.rb.pw, s);   // greedy.rs:50
             RK_HIP(hipMemsetAsync(d_gdone, 0, sizeof(int32_t) * (size_t)G, s));
             refine_pass_batch(L.g, bl, 0, L.nheavy, L.heavy, B.rb, pf, s);                  // greedy.rs:60-123
-            for (int it = 0; it < o.niter; ++it)
-                refine_pass_batch(L.g, bl, 1, L.nheavy, L.heavy, B.rb, pf, s);              // greedy.rs:126-190
+            for (int it = 0; it < o.niter; ++it)                                           // greedy.rs:126-190
+                refine_pass_batch(L.g, bl, 1, L.nheavy, L.heavy, B.rb, pf, s, it > 0);
             if (st) {
                 RK_HIP(hipEventRecord(e1, s));
                 RK_HIP(hipEventSynchronize(e1));

@@ -102,8 +102,9 @@ Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon
     r.tile_rec = a.get<int4>(ntiles * kTile);
     r.tile_cnt = a.get<int32_t>(ntiles);
     r.tile_off = a.get<int32_t>(ntiles);
-    r.tile_bits = (uint32_t*)a.get<int32_t>((ntiles + 31) / 32);
-    RK_HIP(hipMemsetAsync(r.tile_bits, 0, (size_t)((ntiles + 31) / 32) * sizeof(uint32_t), s));
+    r.tile_bits = (uint32_t*)a.get<int32_t>(2 * ((ntiles + 31) / 32));
+    r.tile_dirty = r.tile_bits + (ntiles + 31) / 32;
+    RK_HIP(hipMemsetAsync(r.tile_bits, 0, (size_t)(2 * ((ntiles + 31) / 32)) * sizeof(uint32_t), s));
     r.mov = a.get<int4>(n);
     r.pool = a.get<int32_t>(m);
     r.heavy_rec = a.get<int4>(n);
@@ -119,6 +120,7 @@ Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon
     r.nmoves = cnt + 2;
     r.done = cnt + 3;
     r.passes = cnt + 4;
+    r.full = cnt + 5;
     r.host_small = cx.host_small();
     r.scan_tmp = a.alloc(scan_temp_bytes(std::max<int64_t>(ntiles, n)));
     CoarsenBufs& c = B.cb;
@@ -182,7 +184,9 @@ void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* d_
     part_weights(g, d_part, k, B.rb.pw, s);                                          // greedy.rs:50
     RK_HIP(hipMemsetAsync(B.rb.done, 0, sizeof(int32_t), s));
     refine_pass(g, k, 0, nheavy, heavy_list, B.rb, d_part, s);                       // greedy.rs:60-123
-    for (int it = 0; it < niter; ++it) refine_pass(g, k, 1, nheavy, heavy_list, B.rb, d_part, s);  // 126-190
+    // cut rounds (126-190); after the first, only the tiles around the last
+    // round's moves are recomputed
+    for (int it = 0; it < niter; ++it) refine_pass(g, k, 1, nheavy, heavy_list, B.rb, d_part, s, it > 0);
     if (st) {
         RK_HIP(hipEventRecord(e1, s));
         int32_t passes = 0;

@@ -51,6 +51,7 @@ void column_sums(const DevGraph& g, long long* out, hipStream_t s);
 
 // ---- refine.hip ----
 constexpr int kHeavyGlobalBlocks = 256;
+constexpr int kMarkMaxDeg = 1024;   // a cut-pass mover of higher degree makes the next cut pass a full one
 constexpr int kGainHeavy = 64;   // vertices of higher degree get a block each in the gain kernels
 
 // Work buffers of the greedy rounds, sized for the finest level of a call.
@@ -58,8 +59,9 @@ struct RefineBufs {
     int4* tile_rec = nullptr;      // [ntiles * kTile]
     int32_t* tile_cnt = nullptr;   // [ntiles]
     int32_t* tile_off = nullptr;   // [ntiles] ordered ids of the tiles with movers (admission scratch)
-    uint32_t* tile_bits = nullptr; // [ceil(ntiles/32)] bit per tile with movers; set by the gain kernel,
-                                   // cleared by the admission kernel that reads it
+    uint32_t* tile_bits = nullptr; // [ceil(ntiles/32)] bit per tile with movers, kept exact by the gain kernel
+    uint32_t* tile_dirty = nullptr; // [ceil(ntiles/32)] tiles with a vertex or neighbour moved by the last
+                                    // cut pass (set by admission, cleared by the gain kernel)
     int4* mov = nullptr;           // [n] dense candidate records, vertex order
     int32_t* pool = nullptr;       // [m] candidate parts
     int4* heavy_rec = nullptr;     // [n]
@@ -74,6 +76,7 @@ struct RefineBufs {
     int32_t* nmoves = nullptr;
     int32_t* done = nullptr;
     int32_t* passes = nullptr;     // count of executed passes (statistics)
+    int32_t* full = nullptr;       // "the next cut pass recomputes every tile" (a hub moved)
     int32_t* host_small = nullptr; // pinned host scratch for counter reads
     void* scan_tmp = nullptr;
 };
@@ -98,6 +101,11 @@ struct AdmitArgs {
     int32_t* done;
     int32_t* passes;
     int32_t* gscratch;   // 3*k*ncon ints when k*ncon exceeds the LDS tables
+    // dirty-tile marking for the next, incremental cut pass (null: none)
+    const int32_t* adjncy;
+    uint32_t* tile_dirty;
+    int32_t* full;
+    int32_t v0;          // first vertex of the graph (tiles count from it)
 };
 
 // Heavy-vertex list (degree > kGainHeavy) of a level with n = *n_dev (or n_max
@@ -112,7 +120,7 @@ int heavy_list_sync(const DevGraph& g, int32_t* list, RefineBufs& b, hipStream_t
 // cut-optimisation pass with no move sets *b.done, which turns every later
 // pass of the level into a no-op.
 void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32_t* heavy_list, RefineBufs& b,
-                 int32_t* part, hipStream_t s);
+                 int32_t* part, hipStream_t s, bool incremental = false);
 // ---- admit.hip ----
 // Ordered admission of the candidates that the gain kernels left in
 // b.tile_rec / b.tile_cnt (one workgroup; updates b.pw, part, b.moves).
@@ -134,6 +142,7 @@ struct BatchLevel {
     const int32_t* gkc = nullptr;     // [ngraphs_total] offset of the graph's k*ncon tables
     const int32_t* gk = nullptr;      // [ngraphs_total] part count of each graph
     int32_t* gdone = nullptr;         // [ngraphs_total] "a cut pass of this level moved nothing"
+    int32_t* gfull = nullptr;         // [ngraphs_total] "the next cut pass recomputes every tile"
     const AdmitArgs* admit = nullptr; // [ngraphs] admission arguments of the graphs present
     int32_t ngraphs = 0;              // graphs present at this level
     int32_t kmax = 0;                 // largest part count among them
@@ -141,7 +150,7 @@ struct BatchLevel {
 // refine_pass over every graph of a batched level (b.pw / b.omax hold the
 // per-graph tables at the offsets gkc).
 void refine_pass_batch(const DevGraph& g, const BatchLevel& bl, int mode, int nheavy, const int32_t* heavy_list,
-                       RefineBufs& b, int32_t* part, hipStream_t s);
+                       RefineBufs& b, int32_t* part, hipStream_t s, bool incremental = false);
 
 // Gain table of every vertex (repair-mode candidate lists with every part
 // treated as overweight): per-vertex record (v or -1, pv, ncand, nbefore |

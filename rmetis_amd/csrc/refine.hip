@@ -59,6 +59,12 @@ struct GainArgs {
     const int32_t* gkc;
     const int32_t* gk;
     const int32_t* gdone;
+    // incremental cut pass: only tiles marked in tile_dirty (or every tile
+    // of a graph whose full flag is set: full[g], or full[0] for one graph)
+    // are recomputed; the others keep last pass's records
+    uint32_t* tile_dirty;
+    const int32_t* full;
+    int incremental;
 };
 
 __device__ inline bool gain_qualifies(int mode, int32_t g) {
@@ -105,8 +111,13 @@ __device__ inline int wave_excl_scan_i(int x, int* total) {
 //    weight sum (aggregated in place over its edge range), ranked by
 //    (gain desc, part asc) into pool[xadj[v]...];
 //  * the tile's mover records are compacted with a ballot into
-//    tile_rec[tile*64...], their count into tile_cnt[tile], and a tile with
-//    movers sets its bit in tile_bits.
+//    tile_rec[tile*64...], their count into tile_cnt[tile], and its bit in
+//    tile_bits is set or cleared (the bitmap always describes the tiles'
+//    current records), its bit in tile_dirty cleared.
+// In an incremental cut pass the waves skip the tiles none of whose vertices
+// or neighbours moved in the last pass: their records, counts and bits are
+// still exact (a cut-pass record depends on the parts of the vertex and its
+// neighbours only).
 __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntiles) {
     __shared__ int32_t s_pu_all[kGWaves][kGCap];
     __shared__ int32_t s_w_all[kGWaves][kGCap];
@@ -139,9 +150,26 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
             hp = a.part[v];
         }
     };
-    if (gw < ntiles) load_head(gw);
+    // tiles this pass recomputes (all of them unless incremental)
+    auto live = [&](int tl) __attribute__((always_inline)) -> bool {
+        if (!a.incremental) return true;
+        int32_t f = a.full[0];
+        if (a.tile_g) {
+            const int32_t gi = a.tile_g[tl];
+            if (gi < 0 || a.gdone[gi]) return false;   // padding, or converged: records already cleared
+            f = a.full[gi];
+        }
+        return f != 0 || ((a.tile_dirty[tl >> 5] >> (tl & 31)) & 1u);
+    };
+    auto next_live = [&](int tl) __attribute__((always_inline)) -> int {
+        while (tl < ntiles && !live(tl)) tl += nwaves;
+        return tl;
+    };
+    int tile = next_live(gw);
+    if (tile < ntiles) load_head(tile);
     if (!a.tile_g && *a.done) return;   // read together with the first head
-    for (int tile = gw; tile < ntiles; tile += nwaves) {
+    for (; tile < ntiles;) {
+        const int nxt = next_live(tile + nwaves);
         const int64_t v = hv0 + lane;
         const bool own = lane < hnv;
         const int32_t beg = hb, end = he, pv = hp;
@@ -151,14 +179,18 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
         if (a.tile_g) {
             const int32_t gi = a.tile_g[tile];
             if (gi < 0 || a.gdone[gi]) {       // padding tile, or its graph has converged
-                if (lane == 0) a.tile_cnt[tile] = 0;
-                if (tile + nwaves < ntiles) load_head(tile + nwaves);
+                if (lane == 0) {
+                    a.tile_cnt[tile] = 0;
+                    atomicAnd(&a.tile_bits[tile >> 5], ~(1u << (tile & 31)));
+                }
+                if (nxt < ntiles) load_head(nxt);
+                tile = nxt;
                 continue;
             }
             pwg += a.gkc[gi];
             omg += a.gkc[gi];
         }
-        if (tile + nwaves < ntiles) load_head(tile + nwaves);
+        if (nxt < ntiles) load_head(nxt);
         const int32_t deg = end - beg;
         const bool heavy = own && deg > kGainHeavy;
         const int sdeg = (own && !heavy) ? deg : 0;
@@ -266,9 +298,13 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
         const uint64_t fm = __ballot(f);
         if (f) a.tile_rec[(int64_t)tile * kTile + __popcll(fm & ((1ull << lane) - 1ull))] = rec;
         if (lane == 0) {
+            const uint32_t bit = 1u << (tile & 31);
             a.tile_cnt[tile] = __popcll(fm);
-            if (fm) atomicOr(&a.tile_bits[tile >> 5], 1u << (tile & 31));
+            if (fm) atomicOr(&a.tile_bits[tile >> 5], bit);
+            else atomicAnd(&a.tile_bits[tile >> 5], ~bit);
+            if (a.tile_dirty) atomicAnd(&a.tile_dirty[tile >> 5], ~bit);
         }
+        tile = nxt;
     }
 }
 
@@ -371,6 +407,31 @@ __global__ __launch_bounds__(kBlock) void k_gain_heavy(GainArgs a, const int32_t
     }
 }
 
+// Tiles the next, incremental cut pass recomputes: those of the moved
+// vertices and of their neighbours (one wave per move); a hub sets *full.
+__global__ __launch_bounds__(kBlock) void k_mark_moves(const int2* __restrict__ moves, const int32_t* __restrict__ nmoves,
+                                                       const int32_t* __restrict__ xadj,
+                                                       const int32_t* __restrict__ adjncy, uint32_t* __restrict__ dirty,
+                                                       int32_t* __restrict__ full, int32_t max_deg) {
+    const int32_t nm = *nmoves;
+    const int lane = __lane_id();
+    const int64_t nw = (int64_t)gridDim.x * (kBlock / 64);
+    for (int64_t i = (int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); i < nm; i += nw) {
+        const int32_t v = moves[i].x;
+        const int32_t b0 = xadj[v], b1 = xadj[v + 1];
+        if (lane == 0) {
+            const int32_t t = v >> 6;
+            atomicOr(&dirty[t >> 5], 1u << (t & 31));
+            if (b1 - b0 > max_deg) *full = 1;
+        }
+        if (b1 - b0 > max_deg) continue;
+        for (int32_t j = b0 + lane; j < b1; j += 64) {
+            const int32_t t = adjncy[j] >> 6;
+            atomicOr(&dirty[t >> 5], 1u << (t & 31));
+        }
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_heavy_list(int32_t n_max, const int32_t* __restrict__ n_dev,
                                                        const int32_t* __restrict__ xadj, int32_t thresh,
                                                        int32_t* __restrict__ list, int32_t* __restrict__ count) {
@@ -436,11 +497,14 @@ int heavy_list_sync(const DevGraph& g, int32_t* list, RefineBufs& b, hipStream_t
 }
 
 void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32_t* heavy_list, RefineBufs& b,
-                 int32_t* part, hipStream_t s) {
+                 int32_t* part, hipStream_t s, bool incremental) {
     if (g.n <= 0) return;
     const int ntiles = (g.n + kTile - 1) / kTile;
     GainArgs ga{g.n, g.ncon, k, mode, g.xadj, g.adjncy, g.adjwgt, g.vwgt, part, b.pw, b.omax,
                 b.heavy_rec, b.pool, b.tile_rec, b.tile_cnt, b.tile_bits, b.done, nullptr, nullptr, nullptr};
+    ga.tile_dirty = b.tile_dirty;
+    ga.full = b.full;
+    ga.incremental = incremental && b.tile_dirty && b.full && mode == 1 ? 1 : 0;
     {
         // algorithmic bytes: xadj, adjncy, adjwgt, part[adjncy] gather, part[v]
         const double bytes = 4.0 * (g.n + 1) + 4.0 * g.n + (g.adjwgt ? 12.0 : 8.0) * g.m;
@@ -451,15 +515,21 @@ void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32
         ProfScope ps(kProfAdmit, 0.0, g.n, s);
         admit_pass(g, k, mode, ntiles, b, part, s);
     }
+    if (mode == 1 && b.tile_dirty && b.full)   // one wave per move, not on the admission's chain
+        hipLaunchKernelGGL(k_mark_moves, dim3(256), dim3(kBlock), 0, s, b.moves, b.nmoves, g.xadj, g.adjncy,
+                           b.tile_dirty, b.full, kMarkMaxDeg);
     RK_HIP(hipGetLastError());
 }
 
 void refine_pass_batch(const DevGraph& g, const BatchLevel& bl, int mode, int nheavy, const int32_t* heavy_list,
-                       RefineBufs& b, int32_t* part, hipStream_t s) {
+                       RefineBufs& b, int32_t* part, hipStream_t s, bool incremental) {
     if (g.n <= 0 || bl.ngraphs <= 0) return;
     GainArgs ga{g.n, g.ncon, bl.kmax, mode, g.xadj, g.adjncy, g.adjwgt, g.vwgt, part, b.pw, b.omax,
                 b.heavy_rec, b.pool, b.tile_rec, b.tile_cnt, b.tile_bits, b.done, nullptr, nullptr, nullptr,
                 bl.tile_v0, bl.tile_nv, bl.tile_g, bl.gseg, bl.gkc, bl.gk, bl.gdone};
+    ga.tile_dirty = b.tile_dirty;
+    ga.full = bl.gfull;
+    ga.incremental = incremental && b.tile_dirty && bl.gfull && mode == 1 ? 1 : 0;
     {
         const double bytes = 4.0 * (g.n + 1) + 4.0 * g.n + (g.adjwgt ? 12.0 : 8.0) * g.m;
         ProfScope ps(kProfGain, bytes, g.n, s);
