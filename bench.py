@@ -119,7 +119,7 @@ def main():
         L.rmetis_hip_profile_read(tag, size, C.byref(ln), C.byref(ms), C.byref(by))
         return ln.value, ms.value, by.value
 
-    prof_tags = {"gain": 0, "compact": 1, "admit": 2, "match": 3, "contract": 4}
+    prof_tags = {"gain": 0, "gain_inc": 6, "compact": 1, "admit": 2, "match": 3, "contract": 4}
     kernel_ms = {name: round(prof(t)[1] / args.steps, 3) for name, t in prof_tags.items()}
     g_launch, g_ms, g_bytes = prof(0, g.n)
     L.rmetis_hip_profile_enable(0)
@@ -286,7 +286,7 @@ def bench_batch(args):
     elapsed = time.perf_counter() - t_start
     value = count * args.steps / elapsed
     log(f"batched: {args.steps} steps in {elapsed:.2f}s = {value:.2f} graphs/s")
-    prof_tags = {"gain": 0, "compact": 1, "admit": 2, "match": 3, "contract": 4}
+    prof_tags = {"gain": 0, "gain_inc": 6, "compact": 1, "admit": 2, "match": 3, "contract": 4}
     kernel_ms = {name: round(_prof(L, t)[1] / args.steps, 2) for name, t in prof_tags.items()}
     g_launch, g_ms, g_bytes = _prof(L, 0, n)
     L.rmetis_hip_profile_enable(0)

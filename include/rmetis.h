@@ -237,8 +237,10 @@ int32_t rmetis_hip_device_synchronize(void);
 
 /* Kernel timing (no counterpart in the reference; used by the benchmark).
  * While enabled, launches of each kernel family are bracketed by HIP events
- * on their own stream.  Tags: 0 gain tables/candidates, 1 candidate
- * compaction, 2 ordered admission, 3 matching, 4 contraction, 5 other.
+ * on their own stream.  Tags: 0 gain tables/candidates (launches over every
+ * tile), 1 candidate compaction, 2 ordered admission, 3 matching,
+ * 4 contraction, 5 other, 6 incremental gain launches (only the tiles near
+ * the previous pass's moves; bytes not modelled).
  * enable(on) clears the counters; read returns launches, total ms and the
  * algorithmic HBM bytes of the timed launches (0 where not modelled), over
  * all levels (size < 0) or the levels with exactly `size` vertices. */

@@ -12,13 +12,14 @@
 namespace rk {
 
 enum ProfTag {
-    kProfGain = 0,      // k_gain + k_gain_heavy (boundary, gain tables, candidates)
+    kProfGain = 0,      // gain launches over every tile (boundary, gain tables, candidates)
     kProfCompact = 1,   // tile-count scan + ordered compaction
     kProfAdmit = 2,     // ordered admission (one workgroup)
     kProfMatch = 3,     // matching rounds
     kProfContract = 4,  // coarse numbering + contraction
     kProfOther = 5,     // part weights, projection, edge cut
-    kProfTags = 6
+    kProfGainInc = 6,   // incremental gain launches (only tiles near the last pass's moves)
+    kProfTags = 7
 };
 
 class Profiler {

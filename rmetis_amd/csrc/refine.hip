@@ -161,9 +161,25 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
         }
         return f != 0 || ((a.tile_dirty[tl >> 5] >> (tl & 31)) & 1u);
     };
+    // the wave's tiles are gw + i*nwaves; their live flags are fetched 64 at
+    // a time (one load per lane) into a ballot mask, and the next live tile
+    // is the next set bit
+    const int per = (ntiles - gw + nwaves - 1) / nwaves;   // tiles of this wave
+    int cbase = -64;                                         // first index of the fetched chunk
+    uint64_t cmask = 0;                                      // live tiles of the chunk
     auto next_live = [&](int tl) __attribute__((always_inline)) -> int {
-        while (tl < ntiles && !live(tl)) tl += nwaves;
-        return tl;
+        int i = (tl - gw) / nwaves;                          // index among the wave's tiles
+        for (;;) {
+            if (i >= per) return ntiles;
+            if (i >= cbase + 64) {
+                cbase = i - (i % 64);
+                const int li = cbase + lane;
+                cmask = __ballot(li < per && live(gw + li * nwaves));
+            }
+            const uint64_t m = cmask & (~0ull << (i - cbase));
+            if (m) return gw + (cbase + __builtin_ctzll(m)) * nwaves;
+            i = cbase + 64;
+        }
     };
     int tile = next_live(gw);
     if (tile < ntiles) load_head(tile);
@@ -506,9 +522,11 @@ void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32
     ga.full = b.full;
     ga.incremental = incremental && b.tile_dirty && b.full && mode == 1 ? 1 : 0;
     {
-        // algorithmic bytes: xadj, adjncy, adjwgt, part[adjncy] gather, part[v]
+        // algorithmic bytes of a full launch: xadj, adjncy, adjwgt, part[adjncy] gather, part[v]
         const double bytes = 4.0 * (g.n + 1) + 4.0 * g.n + (g.adjwgt ? 12.0 : 8.0) * g.m;
-        ProfScope ps(kProfGain, bytes, g.n, s);
+        // an incremental launch reads a data-dependent part of the level: its
+        // bytes are not modelled
+        ProfScope ps(ga.incremental ? kProfGainInc : kProfGain, ga.incremental ? 0.0 : bytes, g.n, s);
         launch_gain(ga, nheavy, heavy_list, b, s);
     }
     {
@@ -532,7 +550,9 @@ void refine_pass_batch(const DevGraph& g, const BatchLevel& bl, int mode, int nh
     ga.incremental = incremental && b.tile_dirty && bl.gfull && mode == 1 ? 1 : 0;
     {
         const double bytes = 4.0 * (g.n + 1) + 4.0 * g.n + (g.adjwgt ? 12.0 : 8.0) * g.m;
-        ProfScope ps(kProfGain, bytes, g.n, s);
+        // an incremental launch reads a data-dependent part of the level: its
+        // bytes are not modelled
+        ProfScope ps(ga.incremental ? kProfGainInc : kProfGain, ga.incremental ? 0.0 : bytes, g.n, s);
         launch_gain(ga, nheavy, heavy_list, b, s, bl.ntiles);
     }
     {
