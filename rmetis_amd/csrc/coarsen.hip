@@ -278,7 +278,7 @@ __device__ inline int wave_excl_scan_c(int x, int* total) {
 
 // Every wave merges tiles of 64 consecutive coarse vertices (lane = coarse
 // vertex) on its own, with its own LDS area and no block barriers; waves are
-// persistent over tiles gw, gw + nwaves, ... while tile*64 < cn.  The merged
+// persistent over XCD-contiguous tile sequences (wave_tiles).  The merged
 // sequences adj(a) ++ adj(b) of the tile's ordinary coarse vertices are
 // staged in batches of whole vertices of at most kCCap positions: each lane
 // marks its own positions in a byte array of owners, then kCItems
@@ -309,9 +309,10 @@ __global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* _
     int32_t *s_loff = s_loff_all[wv], *s_xa = s_xa_all[wv], *s_xb = s_xb_all[wv], *s_da = s_da_all[wv];
     const int32_t cn = *cn_dev;
     const int ntiles = (cn + 63) / 64;
-    const int gw = blockIdx.x * kCWaves + wv, nwaves = gridDim.x * kCWaves;
+    const WaveTiles wt = wave_tiles(ntiles, kCWaves);
     bool not_one = false;
-    for (int tile = gw; tile < ntiles; tile += nwaves) {
+    for (int it = 0; it < wt.count; ++it) {
+        const int tile = wt.base + it * wt.stride;
         const int64_t cv = (int64_t)tile * 64 + lane;
         const bool own = cv < cn;
         int32_t ds = 0, xa = 0, xb = 0, da = 0;

@@ -91,6 +91,38 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+constexpr int kXcds = 8;   // accelerator dies of the device, each with its own L2
+
+// Tiles of a persistent wave: base + i*stride for i < count.  Blocks are
+// dealt to the XCDs round-robin (block b runs on XCD b % kXcds), so when the
+// grid is a multiple of kXcds the tiles are split into kXcds contiguous
+// regions, one per XCD, shared by that XCD's waves: neighbouring tiles, whose
+// gathered lines overlap, then meet in the same L2.  Otherwise the waves
+// stride over all tiles.
+struct WaveTiles {
+    int base, stride, count;
+};
+__device__ inline WaveTiles wave_tiles(int ntiles, int waves_per_block) {
+    const int wv = threadIdx.x >> 6;
+    const int nwaves = gridDim.x * waves_per_block;
+    WaveTiles t;
+    if (gridDim.x % kXcds == 0) {
+        const int wx = nwaves / kXcds;                                      // waves per XCD
+        const int lw = (int)(blockIdx.x / kXcds) * waves_per_block + wv;    // wave within its XCD
+        const int reg = (ntiles + kXcds - 1) / kXcds;
+        const int r0 = min((int)(blockIdx.x % kXcds) * reg, ntiles), r1 = min(r0 + reg, ntiles);
+        t.base = r0 + lw;
+        t.stride = wx;
+        t.count = t.base < r1 ? (r1 - t.base + wx - 1) / wx : 0;
+    } else {
+        const int gw = (int)blockIdx.x * waves_per_block + wv;
+        t.base = gw;
+        t.stride = nwaves;
+        t.count = gw < ntiles ? (ntiles - gw + nwaves - 1) / nwaves : 0;
+    }
+    return t;
+}
+
 // Exclusive prefix of a 0/1 flag over a whole block (blockDim.x multiple of
 // 64, at most 1024).  s_tmp needs blockDim.x/64 ints.  Contains barriers.
 __device__ inline int block_flag_scan(bool flag, int* s_tmp, int* total) {

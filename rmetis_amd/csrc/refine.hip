@@ -100,7 +100,7 @@ __device__ inline int wave_excl_scan_i(int x, int* total) {
 // Gain tables of one pass.  Every wave handles tiles of 64 consecutive
 // vertices (lane = vertex) on its own, with its own LDS staging area and no
 // block barriers, so the waves of a CU overlap their load and compute phases
-// freely.  Waves are persistent over tiles gw, gw + nwaves, ...; the next
+// freely.  Waves are persistent over a strided sequence of tiles; the next
 // tile's CSR offsets and own parts are loaded while the current one is
 // processed.  Per tile:
 //  * the edges of vertices with degree <= kGainHeavy are staged in batches
@@ -119,16 +119,13 @@ __device__ inline int wave_excl_scan_i(int x, int* total) {
 // still exact (a cut-pass record depends on the parts of the vertex and its
 // neighbours only).
 __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntiles) {
-    __shared__ int32_t s_pu_all[kGWaves][kGCap];
-    __shared__ int32_t s_w_all[kGWaves][kGCap];
+    __shared__ int2 s_e_all[kGWaves][kGCap];        // staged edges: (part of the neighbour, weight)
     __shared__ int32_t s_loff_all[kGWaves][kTile + 1];
     __shared__ int32_t s_beg_all[kGWaves][kTile];
     const int wv = threadIdx.x >> 6, lane = __lane_id();
-    int32_t* s_pu = s_pu_all[wv];
-    int32_t* s_w = s_w_all[wv];
+    int2* s_e = s_e_all[wv];
     int32_t* s_loff = s_loff_all[wv];
     int32_t* s_beg = s_beg_all[wv];
-    const int gw = blockIdx.x * kGWaves + wv, nwaves = gridDim.x * kGWaves;
     // head of the next tile: its span and, per lane, xadj[v], xadj[v+1], part[v]
     int64_t hv0 = 0;
     int hnv = 0;
@@ -161,31 +158,33 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
         }
         return f != 0 || ((a.tile_dirty[tl >> 5] >> (tl & 31)) & 1u);
     };
-    // the wave's tiles are gw + i*nwaves; their live flags are fetched 64 at
-    // a time (one load per lane) into a ballot mask, and the next live tile
-    // is the next set bit
-    const int per = (ntiles - gw + nwaves - 1) / nwaves;   // tiles of this wave
+    // the wave's tiles are tb + i*ts, i < per (XCD-contiguous regions)
+    const WaveTiles wt = wave_tiles(ntiles, kGWaves);
+    const int tb = wt.base, ts = wt.stride, per = wt.count;
+    // live flags are fetched 64 tiles at a time (one load per lane) into a
+    // ballot mask; the next live tile is the next set bit
     int cbase = -64;                                         // first index of the fetched chunk
     uint64_t cmask = 0;                                      // live tiles of the chunk
-    auto next_live = [&](int tl) __attribute__((always_inline)) -> int {
-        int i = (tl - gw) / nwaves;                          // index among the wave's tiles
+    auto next_live = [&](int i) __attribute__((always_inline)) -> int {
         for (;;) {
-            if (i >= per) return ntiles;
+            if (i >= per) return per;
             if (i >= cbase + 64) {
                 cbase = i - (i % 64);
                 const int li = cbase + lane;
-                cmask = __ballot(li < per && live(gw + li * nwaves));
+                cmask = __ballot(li < per && live(tb + li * ts));
             }
             const uint64_t m = cmask & (~0ull << (i - cbase));
-            if (m) return gw + (cbase + __builtin_ctzll(m)) * nwaves;
+            if (m) return cbase + __builtin_ctzll(m);
             i = cbase + 64;
         }
     };
-    int tile = next_live(gw);
-    if (tile < ntiles) load_head(tile);
+    int it = next_live(0);
+    if (it < per) load_head(tb + it * ts);
     if (!a.tile_g && *a.done) return;   // read together with the first head
-    for (; tile < ntiles;) {
-        const int nxt = next_live(tile + nwaves);
+    for (; it < per;) {
+        const int tile = tb + it * ts;
+        const int nit = next_live(it + 1);
+        const int nxt = nit < per ? tb + nit * ts : ntiles;
         const int64_t v = hv0 + lane;
         const bool own = lane < hnv;
         const int32_t beg = hb, end = he, pv = hp;
@@ -200,7 +199,7 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
                     atomicAnd(&a.tile_bits[tile >> 5], ~(1u << (tile & 31)));
                 }
                 if (nxt < ntiles) load_head(nxt);
-                tile = nxt;
+                it = nit;
                 continue;
             }
             pwg += a.gkc[gi];
@@ -251,18 +250,26 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
 #pragma unroll
             for (int i = 0; i < kGItems; ++i) {
                 const int p = P0 + lane + i * 64;
-                if (p < P1) {
-                    s_pu[p - P0] = a.part[uu[i]];
-                    s_w[p - P0] = ww[i];
-                }
+                if (p < P1) s_e[p - P0] = make_int2(a.part[uu[i]], ww[i]);
             }
             wave_sync();
             if (lane >= vs && lane < ve && sdeg > 0) {
                 const int lb = loff - P0, le = lb + deg;
+                // weight to the own part and "any external edge"; four
+                // independent LDS reads per step (the ranges of the lanes
+                // differ, so each read is a gather)
                 uint32_t internal = 0;
                 bool ext = false;
-                for (int e = lb; e < le; ++e) {
-                    if (s_pu[e] == pv) internal += (uint32_t)s_w[e];
+                int e = lb;
+                for (; e + 4 <= le; e += 4) {
+                    const int2 x0 = s_e[e], x1 = s_e[e + 1], x2 = s_e[e + 2], x3 = s_e[e + 3];
+                    internal += (x0.x == pv ? (uint32_t)x0.y : 0u) + (x1.x == pv ? (uint32_t)x1.y : 0u) +
+                                (x2.x == pv ? (uint32_t)x2.y : 0u) + (x3.x == pv ? (uint32_t)x3.y : 0u);
+                    ext |= (x0.x != pv) | (x1.x != pv) | (x2.x != pv) | (x3.x != pv);
+                }
+                for (; e < le; ++e) {
+                    const int2 x = s_e[e];
+                    if (x.x == pv) internal += (uint32_t)x.y;
                     else ext = true;
                 }
                 if (ext && (a.mode == 1 || part_overweight(pwg, omg, a.ncon, pv))) {
@@ -270,29 +277,31 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
                     const int32_t g0 = wsub(0, ed_own);
                     // aggregate equal parts onto their first occurrence
                     for (int e = lb; e < le; ++e) {
-                        const int32_t q = s_pu[e];
+                        const int32_t q = s_e[e].x;
                         if (q < 0 || q == pv) continue;
-                        uint32_t sum = (uint32_t)s_w[e];
+                        uint32_t sum = (uint32_t)s_e[e].y;
                         for (int e2 = e + 1; e2 < le; ++e2)
-                            if (s_pu[e2] == q) {
-                                sum += (uint32_t)s_w[e2];
-                                s_pu[e2] = -1;
+                            if (s_e[e2].x == q) {
+                                sum += (uint32_t)s_e[e2].y;
+                                s_e[e2].x = -1;
                             }
-                        s_w[e] = (int32_t)sum;
+                        s_e[e].y = (int32_t)sum;
                     }
                     int ncand = 0, nbefore = 0;
                     for (int e = lb; e < le; ++e) {
-                        const int32_t q = s_pu[e];
-                        if (q < 0 || q == pv || s_w[e] == 0) continue;
-                        const int32_t g = wsub(s_w[e], ed_own);
+                        const int2 x = s_e[e];
+                        const int32_t q = x.x;
+                        if (q < 0 || q == pv || x.y == 0) continue;
+                        const int32_t g = wsub(x.y, ed_own);
                         if (!gain_qualifies(a.mode, g)) continue;
                         ++ncand;
                         if (g > g0) ++nbefore;
                         int rank = 0;
                         for (int e2 = lb; e2 < le; ++e2) {
-                            const int32_t q2 = s_pu[e2];
-                            if (q2 < 0 || q2 == pv || s_w[e2] == 0) continue;
-                            const int32_t g2 = wsub(s_w[e2], ed_own);
+                            const int2 x2 = s_e[e2];
+                            const int32_t q2 = x2.x;
+                            if (q2 < 0 || q2 == pv || x2.y == 0) continue;
+                            const int32_t g2 = wsub(x2.y, ed_own);
                             if (!gain_qualifies(a.mode, g2)) continue;
                             if (g2 > g || (g2 == g && q2 < q)) ++rank;
                         }
@@ -320,7 +329,7 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
             else atomicAnd(&a.tile_bits[tile >> 5], ~bit);
             if (a.tile_dirty) atomicAnd(&a.tile_dirty[tile >> 5], ~bit);
         }
-        tile = nxt;
+        it = nit;
     }
 }
 
