@@ -361,11 +361,16 @@ __global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* _
                     if (adjwgt) ww[i] = adjwgt[j];
                 }
             }
+            // all gathers in flight before the first LDS write (items past
+            // the batch read cmap[0])
+            int32_t cc[kCItems];
+#pragma unroll
+            for (int i = 0; i < kCItems; ++i) cc[i] = cmap[uu[i]];
 #pragma unroll
             for (int i = 0; i < kCItems; ++i) {
                 const int p = P0 + lane + i * 64;
                 if (p < P1) {
-                    s_cu[p - P0] = cmap[uu[i]];
+                    s_cu[p - P0] = cc[i];
                     s_w[p - P0] = ww[i];
                 }
             }

@@ -118,6 +118,10 @@ __device__ inline int wave_excl_scan_i(int x, int* total) {
 // or neighbours moved in the last pass: their records, counts and bits are
 // still exact (a cut-pass record depends on the parts of the vertex and its
 // neighbours only).
+// BATCH: the level is a union of graphs (tile_v0/tile_nv/tile_g describe the
+// tiles); instantiated separately so the single-graph kernel carries none of
+// the batch branches.
+template <bool BATCH>
 __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntiles) {
     __shared__ int2 s_e_all[kGWaves][kGCap];        // staged edges: (part of the neighbour, weight)
     __shared__ int32_t s_loff_all[kGWaves][kTile + 1];
@@ -131,7 +135,7 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
     int hnv = 0;
     int32_t hb = 0, he = 0, hp = -1;
     auto load_head = [&](int tl) {
-        if (a.tile_v0) {
+        if constexpr (BATCH) {
             hv0 = a.tile_v0[tl];
             hnv = a.tile_nv[tl];
         } else {
@@ -151,7 +155,7 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
     auto live = [&](int tl) __attribute__((always_inline)) -> bool {
         if (!a.incremental) return true;
         int32_t f = a.full[0];
-        if (a.tile_g) {
+        if constexpr (BATCH) {
             const int32_t gi = a.tile_g[tl];
             if (gi < 0 || a.gdone[gi]) return false;   // padding, or converged: records already cleared
             f = a.full[gi];
@@ -180,7 +184,7 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
     };
     int it = next_live(0);
     if (it < per) load_head(tb + it * ts);
-    if (!a.tile_g && *a.done) return;   // read together with the first head
+    if (!BATCH && *a.done) return;   // read together with the first head
     for (; it < per;) {
         const int tile = tb + it * ts;
         const int nit = next_live(it + 1);
@@ -191,21 +195,23 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
         // this tile's graph tables (a single graph: the arguments themselves)
         const int32_t* pwg = a.pw;
         const long long* omg = a.omax;
-        if (a.tile_g) {
+        if constexpr (BATCH) {
             const int32_t gi = a.tile_g[tile];
             if (gi < 0 || a.gdone[gi]) {       // padding tile, or its graph has converged
                 if (lane == 0) {
                     a.tile_cnt[tile] = 0;
                     atomicAnd(&a.tile_bits[tile >> 5], ~(1u << (tile & 31)));
                 }
-                if (nxt < ntiles) load_head(nxt);
+                load_head(nxt < ntiles ? nxt : tile);
                 it = nit;
                 continue;
             }
             pwg += a.gkc[gi];
             omg += a.gkc[gi];
         }
-        if (nxt < ntiles) load_head(nxt);
+        // the next tile's head (the current one again after the last tile:
+        // an unconditional load keeps the prefetch out of any branch)
+        load_head(nxt < ntiles ? nxt : tile);
         const int32_t deg = end - beg;
         const bool heavy = own && deg > kGainHeavy;
         const int sdeg = (own && !heavy) ? deg : 0;
@@ -247,10 +253,16 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
                     if (a.adjwgt) ww[i] = a.adjwgt[j];
                 }
             }
+            // all gathers in flight before the first LDS write (a gather
+            // inside the per-item branch would be waited for one at a time);
+            // items past the batch read part[0]
+            int32_t pu[kGItems];
+#pragma unroll
+            for (int i = 0; i < kGItems; ++i) pu[i] = a.part[P0 + lane + i * 64 < P1 ? uu[i] : 0];
 #pragma unroll
             for (int i = 0; i < kGItems; ++i) {
                 const int p = P0 + lane + i * 64;
-                if (p < P1) s_e[p - P0] = make_int2(a.part[uu[i]], ww[i]);
+                if (p < P1) s_e[p - P0] = make_int2(pu[i], ww[i]);
             }
             wave_sync();
             if (lane >= vs && lane < ve && sdeg > 0) {
@@ -482,8 +494,9 @@ void launch_gain(const GainArgs& ga, int nheavy, const int32_t* heavy_list, Refi
                            b.heavy_scratch);
     }
     const int blocks = (ntiles + kGWaves - 1) / kGWaves, resident = device_cus() * kGainBlocksPerCU;
-    hipLaunchKernelGGL(k_gain_tile, dim3(blocks < resident ? blocks : resident), dim3(64 * kGWaves), 0, s, ga,
-                       ntiles);
+    const dim3 grid(blocks < resident ? blocks : resident);
+    if (ga.tile_g) hipLaunchKernelGGL(k_gain_tile<true>, grid, dim3(64 * kGWaves), 0, s, ga, ntiles);
+    else hipLaunchKernelGGL(k_gain_tile<false>, grid, dim3(64 * kGWaves), 0, s, ga, ntiles);
 }
 
 }  // namespace
