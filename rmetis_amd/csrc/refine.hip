@@ -85,6 +85,13 @@ constexpr int kGItems = kGCap / 64;      // staged edges per lane per batch
 constexpr int kGainBlocksPerCU = 4;      // resident gain blocks per CU (LDS-limited)
 static_assert(kTile == 64, "a gain tile is one wave");
 
+// Raw buffer resource over n int32 values at p (stride 0, range-checked in
+// bytes; the third dword is the gfx9 data-format word).  Loads at or past
+// the end return 0 and touch no memory.
+__device__ inline __amdgpu_buffer_rsrc_t edge_rsrc(const int32_t* p, int n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(p), (short)0, n * 4, 0x00020000);
+}
+
 __device__ inline int wave_excl_scan_i(int x, int* total) {
     const int lane = __lane_id();
     int inc = x;
@@ -121,8 +128,10 @@ __device__ inline int wave_excl_scan_i(int x, int* total) {
 // BATCH: the level is a union of graphs (tile_v0/tile_nv/tile_g describe the
 // tiles); instantiated separately so the single-graph kernel carries none of
 // the batch branches.
+// (At most 128 VGPRs: four waves per SIMD, the residency the LDS allows.)
 template <bool BATCH>
-__global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntiles) {
+__global__ __launch_bounds__(64 * kGWaves) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_gain_tile(GainArgs a,
+                                                                                                       int ntiles) {
     __shared__ int2 s_e_all[kGWaves][kGCap];        // staged edges: (part of the neighbour, weight)
     __shared__ int32_t s_loff_all[kGWaves][kTile + 1];
     __shared__ int32_t s_beg_all[kGWaves][kTile];
@@ -230,39 +239,53 @@ __global__ __launch_bounds__(64 * kGWaves) void k_gain_tile(GainArgs a, int ntil
             const uint64_t fit = __ballot(lane >= vs && loff + sdeg - P0 <= kGCap);
             const int ve = 64 - __builtin_clzll(fit);            // fit is lanes [vs, ve), ve > vs
             const int P1 = __shfl(loff + sdeg, ve - 1, 64);
-            int32_t uu[kGItems], ww[kGItems];
+            // All loads of a batch are kept in flight together: neighbour ids
+            // first and weights second, then the part[] gathers (which wait
+            // for the ids only), then the LDS writes.  Without heavy vertices
+            // the batch is the edge range [x0+P0, x0+P1): buffer loads over
+            // exactly that range (one offset register, lanes past the end
+            // read 0 without a memory access).  Items past the batch gather
+            // part[0].
+            if (P1 > P0) {
+                if (!anyheavy) {
+                    int32_t uu[kGItems], ww[kGItems], pu[kGItems];
+                    const int p0 = __builtin_amdgcn_readfirstlane(P0);
+                    const int ne = __builtin_amdgcn_readfirstlane(P1 - P0);
+                    const int64_t e0 = (int64_t)x0 + p0;
+                    const int32_t* wsrc = a.adjwgt ? a.adjwgt : a.adjncy;
+                    const int lb4 = lane * 4;   // the lane's byte offset; item i is a resource 64 edges on
 #pragma unroll
-            for (int i = 0; i < kGItems; ++i) {
-                const int p = P0 + lane + i * 64;
-                uu[i] = -1;
-                ww[i] = 1;
-                if (p < P1) {
-                    int32_t j;
-                    if (!anyheavy) {
-                        j = x0 + p;
-                    } else {
-                        int lo = vs, hi = ve - 1;   // last vertex whose staged range starts at or before p
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (s_loff[mid] <= p) lo = mid;
-                            else hi = mid - 1;
-                        }
-                        j = s_beg[lo] + (p - s_loff[lo]);
+                    for (int i = 0; i < kGItems; ++i)
+                        uu[i] = __builtin_amdgcn_raw_buffer_load_b32(
+                            edge_rsrc(a.adjncy + e0 + i * 64, max(ne - i * 64, 0)), lb4, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < kGItems; ++i)
+                        ww[i] = __builtin_amdgcn_raw_buffer_load_b32(
+                            edge_rsrc(wsrc + e0 + i * 64, max(ne - i * 64, 0)), lb4, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < kGItems; ++i) pu[i] = a.part[uu[i]];
+#pragma unroll
+                    for (int i = 0; i < kGItems; ++i) {
+                        const int p = P0 + lane + i * 64;
+                        if (p < P1) s_e[p - P0] = make_int2(pu[i], a.adjwgt ? ww[i] : 1);
                     }
-                    uu[i] = a.adjncy[j];
-                    if (a.adjwgt) ww[i] = a.adjwgt[j];
+                } else {
+                    // a tile with a heavy vertex (rare): item by item
+#pragma unroll 1
+                    for (int i = 0; i < kGItems; ++i) {
+                        const int p = P0 + lane + i * 64;
+                        if (p < P1) {
+                            int lo = vs, hi = ve - 1;   // last vertex whose staged range starts at or before p
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (s_loff[mid] <= p) lo = mid;
+                                else hi = mid - 1;
+                            }
+                            const int32_t j = s_beg[lo] + (p - s_loff[lo]);
+                            s_e[p - P0] = make_int2(a.part[a.adjncy[j]], a.adjwgt ? a.adjwgt[j] : 1);
+                        }
+                    }
                 }
-            }
-            // all gathers in flight before the first LDS write (a gather
-            // inside the per-item branch would be waited for one at a time);
-            // items past the batch read part[0]
-            int32_t pu[kGItems];
-#pragma unroll
-            for (int i = 0; i < kGItems; ++i) pu[i] = a.part[P0 + lane + i * 64 < P1 ? uu[i] : 0];
-#pragma unroll
-            for (int i = 0; i < kGItems; ++i) {
-                const int p = P0 + lane + i * 64;
-                if (p < P1) s_e[p - P0] = make_int2(pu[i], ww[i]);
             }
             wave_sync();
             if (lane >= vs && lane < ve && sdeg > 0) {
