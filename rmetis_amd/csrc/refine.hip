@@ -310,38 +310,88 @@ __global__ __launch_bounds__(64 * kGWaves) __attribute__((amdgpu_waves_per_eu(4,
                 if (ext && (a.mode == 1 || part_overweight(pwg, omg, a.ncon, pv))) {
                     const int32_t ed_own = (int32_t)internal;
                     const int32_t g0 = wsub(0, ed_own);
-                    // aggregate equal parts onto their first occurrence
-                    for (int e = lb; e < le; ++e) {
-                        const int32_t q = s_e[e].x;
-                        if (q < 0 || q == pv) continue;
-                        uint32_t sum = (uint32_t)s_e[e].y;
-                        for (int e2 = e + 1; e2 < le; ++e2)
-                            if (s_e[e2].x == q) {
-                                sum += (uint32_t)s_e[e2].y;
-                                s_e[e2].x = -1;
-                            }
-                        s_e[e].y = (int32_t)sum;
-                    }
                     int ncand = 0, nbefore = 0;
+                    // The distinct adjacent parts with their weight sums: up
+                    // to four in registers in one pass over the edges (the
+                    // usual case), ranked pairwise; more fall back to the
+                    // aggregation in LDS.
+                    int32_t q0 = -1, q1 = -1, q2 = -1, q3 = -1;
+                    uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+                    int nq = 0;
                     for (int e = lb; e < le; ++e) {
                         const int2 x = s_e[e];
-                        const int32_t q = x.x;
-                        if (q < 0 || q == pv || x.y == 0) continue;
-                        const int32_t g = wsub(x.y, ed_own);
-                        if (!gain_qualifies(a.mode, g)) continue;
-                        ++ncand;
-                        if (g > g0) ++nbefore;
-                        int rank = 0;
-                        for (int e2 = lb; e2 < le; ++e2) {
-                            const int2 x2 = s_e[e2];
-                            const int32_t q2 = x2.x;
-                            if (q2 < 0 || q2 == pv || x2.y == 0) continue;
-                            const int32_t g2 = wsub(x2.y, ed_own);
-                            if (!gain_qualifies(a.mode, g2)) continue;
-                            if (g2 > g || (g2 == g && q2 < q)) ++rank;
+                        if (x.x == pv) continue;
+                        const uint32_t w = (uint32_t)x.y;
+                        if (x.x == q0) w0 += w;
+                        else if (x.x == q1) w1 += w;
+                        else if (x.x == q2) w2 += w;
+                        else if (x.x == q3) w3 += w;
+                        else {
+                            if (nq == 0) q0 = x.x, w0 = w;
+                            else if (nq == 1) q1 = x.x, w1 = w;
+                            else if (nq == 2) q2 = x.x, w2 = w;
+                            else if (nq == 3) q3 = x.x, w3 = w;
+                            ++nq;
                         }
-                        a.pool[beg + rank] = q;
-                        if (a.gains) a.gains[beg + rank] = g;
+                    }
+                    if (nq <= 4) {
+                        // a slot is a candidate when its sum is nonzero and its
+                        // gain qualifies (a zero sum belongs to the zero group)
+                        int32_t g[4];
+                        bool c[4];
+                        const int32_t qs[4] = {q0, q1, q2, q3};
+                        const uint32_t ws[4] = {w0, w1, w2, w3};
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            g[j] = wsub((int32_t)ws[j], ed_own);
+                            c[j] = j < nq && ws[j] != 0 && gain_qualifies(a.mode, g[j]);
+                            ncand += c[j];
+                            nbefore += c[j] && g[j] > g0;
+                        }
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            int rank = 0;
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+                                rank += i != j && c[i] && (g[i] > g[j] || (g[i] == g[j] && qs[i] < qs[j]));
+                            if (c[j]) {
+                                a.pool[beg + rank] = qs[j];
+                                if (a.gains) a.gains[beg + rank] = g[j];
+                            }
+                        }
+                    } else {
+                        // aggregate equal parts onto their first occurrence
+                        for (int e = lb; e < le; ++e) {
+                            const int32_t q = s_e[e].x;
+                            if (q < 0 || q == pv) continue;
+                            uint32_t sum = (uint32_t)s_e[e].y;
+                            for (int e2 = e + 1; e2 < le; ++e2)
+                                if (s_e[e2].x == q) {
+                                    sum += (uint32_t)s_e[e2].y;
+                                    s_e[e2].x = -1;
+                                }
+                            s_e[e].y = (int32_t)sum;
+                        }
+                        for (int e = lb; e < le; ++e) {
+                            const int2 x = s_e[e];
+                            const int32_t q = x.x;
+                            if (q < 0 || q == pv || x.y == 0) continue;
+                            const int32_t gq = wsub(x.y, ed_own);
+                            if (!gain_qualifies(a.mode, gq)) continue;
+                            ++ncand;
+                            if (gq > g0) ++nbefore;
+                            int rank = 0;
+                            for (int e2 = lb; e2 < le; ++e2) {
+                                const int2 x2 = s_e[e2];
+                                const int32_t q2 = x2.x;
+                                if (q2 < 0 || q2 == pv || x2.y == 0) continue;
+                                const int32_t g2 = wsub(x2.y, ed_own);
+                                if (!gain_qualifies(a.mode, g2)) continue;
+                                if (g2 > gq || (g2 == gq && q2 < q)) ++rank;
+                            }
+                            a.pool[beg + rank] = q;
+                            if (a.gains) a.gains[beg + rank] = gq;
+                        }
                     }
                     const bool zflag = gain_qualifies(a.mode, g0);
                     if (a.g0_out) a.g0_out[v] = g0;
