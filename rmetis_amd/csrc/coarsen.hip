@@ -315,10 +315,11 @@ __global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* _
         const int tile = wt.base + it * wt.stride;
         const int64_t cv = (int64_t)tile * 64 + lane;
         const bool own = cv < cn;
-        int32_t ds = 0, xa = 0, xb = 0, da = 0;
+        int32_t ds = 0, xa = 0, xb = 0, da = 0, toff = 0;
         if (own) {
             const int32_t a = members[2 * cv], b = members[2 * cv + 1];
             ds = degsum[cv];
+            toff = tmp_off[cv];
             xa = xadj[a];
             da = xadj[a + 1] - xa;
             xb = b >= 0 ? xadj[b] : 0;
@@ -375,28 +376,51 @@ __global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* _
                 }
             }
             wave_sync();
-            if (lane >= vs && lane < ve && own && !big) {
-                const int lb = loff - P0, le = lb + ds;
+            // Each lane compacts its coarse vertex's unique neighbours in
+            // place, in first-occurrence order: an entry adds its weight to an
+            // earlier equal entry or is appended (internal edges dropped).
+            int pos = 0;
+            const bool mine_v = lane >= vs && lane < ve && own && !big;
+            const int lb = loff - P0;
+            if (mine_v) {
+                const int le = lb + ds;
                 const int32_t me = (int32_t)cv;
-                for (int e = lb; e < le; ++e)
-                    if (s_cu[e] == me) s_cu[e] = -1;                     // internal edge (graph/mod.rs:330-332)
-                const int32_t off = tmp_off[cv];
-                int32_t pos = 0;
                 for (int e = lb; e < le; ++e) {
                     const int32_t cu = s_cu[e];
-                    if (cu < 0) continue;
-                    uint32_t sum = (uint32_t)s_w[e];
-                    for (int e2 = e + 1; e2 < le; ++e2)
-                        if (s_cu[e2] == cu) {                             // merge (graph/mod.rs:335-339)
-                            sum += (uint32_t)s_w[e2];
-                            s_cu[e2] = -1;
-                        }
-                    tmp_adj[off + pos] = cu;
-                    tmp_w[off + pos] = (int32_t)sum;
-                    if ((int32_t)sum != 1) not_one = true;
-                    ++pos;
+                    const int32_t w = s_w[e];
+                    if (cu == me) continue;                                 // internal edge (graph/mod.rs:330-332)
+                    int k = lb;
+                    while (k < lb + pos && s_cu[k] != cu) ++k;
+                    if (k < lb + pos) {
+                        s_w[k] = (int32_t)((uint32_t)s_w[k] + (uint32_t)w);  // merge (graph/mod.rs:335-339)
+                    } else {
+                        s_cu[lb + pos] = cu;
+                        s_w[lb + pos] = w;
+                        ++pos;
+                    }
                 }
+                for (int k = lb; k < lb + pos; ++k)
+                    if (s_w[k] != 1) not_one = true;
                 cdeg[cv] = pos;
+            }
+            wave_sync();
+            // Without big vertices among the batch's lanes, staged position i
+            // is tmp position tmp_off[vs] + i (tmp_off is the prefix of the
+            // degree sums): the batch goes out in coalesced stores, the slots
+            // after each vertex's unique entries included (never read).
+            // Otherwise each lane writes its own entries.
+            const uint64_t bmask = ((ve >= 64 ? ~0ull : ((1ull << ve) - 1ull)) >> vs) << vs;
+            if ((bigm & bmask) == 0) {
+                const int32_t base = __shfl(toff, vs, 64);
+                for (int i = lane; i < P1 - P0; i += 64) {
+                    tmp_adj[base + i] = s_cu[i];
+                    tmp_w[base + i] = s_w[i];
+                }
+            } else if (mine_v) {
+                for (int k = 0; k < pos; ++k) {
+                    tmp_adj[toff + k] = s_cu[lb + k];
+                    tmp_w[toff + k] = s_w[lb + k];
+                }
             }
             wave_sync();
             vs = ve;
