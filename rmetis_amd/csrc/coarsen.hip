@@ -263,6 +263,7 @@ constexpr int kCWaves = 4;               // waves (independent tiles) per contra
 constexpr int kCCap = 1024;              // merged-sequence positions a wave stages per batch
 constexpr int kCItems = kCCap / 64;      // staged positions per lane per batch
 constexpr int kContractBlocksPerCU = 3;  // resident contraction blocks per CU (LDS- and VGPR-limited)
+constexpr int kCSlots = 16;              // unique coarse neighbours a lane merges in registers
 
 __device__ inline int wave_excl_scan_c(int x, int* total) {
     const int lane = __lane_id();
@@ -298,8 +299,12 @@ __global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* _
                                                                int32_t* __restrict__ tmp_w, int32_t* __restrict__ cdeg,
                                                                int32_t* __restrict__ big_list,
                                                                int32_t* __restrict__ counters) {
-    __shared__ int32_t s_cu_all[kCWaves][kCCap];
-    __shared__ int32_t s_w_all[kCWaves][kCCap];
+    // staged entries of the lane owning batch position q sit at q + lane:
+    // one slot of padding per lane spreads the lanes' ranges over the LDS
+    // banks (degree sums with a common factor would otherwise put every lane
+    // of the merge loops on a few banks)
+    __shared__ int32_t s_cu_all[kCWaves][kCCap + 64];
+    __shared__ int32_t s_w_all[kCWaves][kCCap + 64];
     __shared__ uint8_t s_own_all[kCWaves][kCCap];
     __shared__ int32_t s_loff_all[kCWaves][65], s_xa_all[kCWaves][64], s_xb_all[kCWaves][64], s_da_all[kCWaves][64];
     const int wv = threadIdx.x >> 6, lane = __lane_id();
@@ -348,14 +353,16 @@ __global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* _
             if (lane >= vs && lane < ve)
                 for (int e = 0; e < sds; ++e) s_own[loff - P0 + e] = (uint8_t)lane;
             wave_sync();
-            int32_t uu[kCItems], ww[kCItems];
+            int32_t uu[kCItems], ww[kCItems], oo[kCItems];
 #pragma unroll
             for (int i = 0; i < kCItems; ++i) {
                 const int p = P0 + lane + i * 64;
                 uu[i] = 0;
                 ww[i] = 1;
+                oo[i] = 0;
                 if (p < P1) {
                     const int o = s_own[p - P0];
+                    oo[i] = o;
                     const int32_t tl = p - s_loff[o];
                     const int32_t j = tl < s_da[o] ? s_xa[o] + tl : s_xb[o] + (tl - s_da[o]);
                     uu[i] = adjncy[j];
@@ -371,40 +378,67 @@ __global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* _
             for (int i = 0; i < kCItems; ++i) {
                 const int p = P0 + lane + i * 64;
                 if (p < P1) {
-                    s_cu[p - P0] = cc[i];
-                    s_w[p - P0] = ww[i];
+                    s_cu[p - P0 + oo[i]] = cc[i];
+                    s_w[p - P0 + oo[i]] = ww[i];
                 }
             }
             wave_sync();
-            // Each lane compacts its coarse vertex's unique neighbours in
-            // place, in first-occurrence order: an entry adds its weight to an
-            // earlier equal entry or is appended (internal edges dropped).
+            // Each lane compacts its coarse vertex's unique neighbours, in
+            // first-occurrence order (internal edges dropped): an entry adds
+            // its weight to an earlier equal entry or is appended.  The first
+            // kCSlots unique entries live in registers (compared with unrolled
+            // selects, constant indices only); more are appended in place in
+            // LDS after slot kCSlots of the lane's range and searched there.
+            // Appends never overtake the read position.  At the end the
+            // register slots go to the front of the range.
             int pos = 0;
             const bool mine_v = lane >= vs && lane < ve && own && !big;
-            const int lb = loff - P0;
+            const int lb = loff - P0 + lane;
             if (mine_v) {
                 const int le = lb + ds;
                 const int32_t me = (int32_t)cv;
+                int32_t sc[kCSlots];
+                uint32_t sw[kCSlots];
+#pragma unroll
+                for (int j = 0; j < kCSlots; ++j) sc[j] = -1, sw[j] = 0;
                 for (int e = lb; e < le; ++e) {
                     const int32_t cu = s_cu[e];
-                    const int32_t w = s_w[e];
+                    const uint32_t w = (uint32_t)s_w[e];
                     if (cu == me) continue;                                 // internal edge (graph/mod.rs:330-332)
-                    int k = lb;
-                    while (k < lb + pos && s_cu[k] != cu) ++k;
-                    if (k < lb + pos) {
-                        s_w[k] = (int32_t)((uint32_t)s_w[k] + (uint32_t)w);  // merge (graph/mod.rs:335-339)
+                    bool hit = false;
+#pragma unroll
+                    for (int j = 0; j < kCSlots; ++j)
+                        if (sc[j] == cu) sw[j] += w, hit = true;            // merge (graph/mod.rs:335-339)
+                    if (hit) continue;
+                    if (pos < kCSlots) {
+#pragma unroll
+                        for (int j = 0; j < kCSlots; ++j)
+                            if (j == pos) sc[j] = cu, sw[j] = w;
                     } else {
+                        int k = lb + kCSlots;
+                        while (k < lb + pos && s_cu[k] != cu) ++k;
+                        if (k < lb + pos) {
+                            s_w[k] = (int32_t)((uint32_t)s_w[k] + w);
+                            continue;
+                        }
                         s_cu[lb + pos] = cu;
-                        s_w[lb + pos] = w;
-                        ++pos;
+                        s_w[lb + pos] = (int32_t)w;
                     }
+                    ++pos;
                 }
-                for (int k = lb; k < lb + pos; ++k)
+#pragma unroll
+                for (int j = 0; j < kCSlots; ++j)
+                    if (j < pos) {
+                        s_cu[lb + j] = sc[j];
+                        s_w[lb + j] = (int32_t)sw[j];
+                        if (sw[j] != 1u) not_one = true;
+                    }
+                for (int k = lb + kCSlots; k < lb + pos; ++k)
                     if (s_w[k] != 1) not_one = true;
                 cdeg[cv] = pos;
             }
             wave_sync();
-            // Without big vertices among the batch's lanes, staged position i
+            // Without big vertices among the batch's lanes, batch position i
             // is tmp position tmp_off[vs] + i (tmp_off is the prefix of the
             // degree sums): the batch goes out in coalesced stores, the slots
             // after each vertex's unique entries included (never read).
@@ -413,8 +447,9 @@ __global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* _
             if ((bigm & bmask) == 0) {
                 const int32_t base = __shfl(toff, vs, 64);
                 for (int i = lane; i < P1 - P0; i += 64) {
-                    tmp_adj[base + i] = s_cu[i];
-                    tmp_w[base + i] = s_w[i];
+                    const int q = i + s_own[i];
+                    tmp_adj[base + i] = s_cu[q];
+                    tmp_w[base + i] = s_w[q];
                 }
             } else if (mine_v) {
                 for (int k = 0; k < pos; ++k) {
