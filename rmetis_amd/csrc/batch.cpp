@@ -222,7 +222,7 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
             gs.cgseg = cx.arena.get<int32_t>(L.g.n);
             {
                 ProfScope ps(kProfMatch, 0.0, L.g.n, s);
-                match_graph(L.g, o.ctype_shem, 0, kMatchRounds, B.cb, s, &gs);
+                match_graph(L.g, o.ctype_shem, 0, B.cb, s, &gs);
             }
             BLevel C;
             C.cmap = cx.arena.get<int32_t>(L.g.n);

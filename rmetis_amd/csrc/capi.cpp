@@ -574,7 +574,7 @@ int32_t rmetis_hip_match(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, co
         ContextGuard cx;
         DevGraph g = upload_graph(*cx.c, nvtxs, 1, xadj, adjncy, nullptr, adjwgt, nullptr);
         Buffers B = alloc_buffers(*cx.c, g.n, g.m, 2, 1);
-        match_graph(g, ctype != 0, seed, 24, B.cb, cx->stream);
+        match_graph(g, ctype != 0, seed, B.cb, cx->stream);
         download(*cx.c, matching, B.cb.match, nvtxs);
         RK_HIP(hipStreamSynchronize(cx->stream));
         return METIS_OK;

@@ -1,12 +1,15 @@
 // coarsen.hip -- matching and coarse-graph contraction on the GPU.
 //
 // Matching (coarsen/shem.rs:13-56, coarsen/rm.rs:13-43).  The reference's
-// matchings are sequential greedy scans.  On the GPU every unmatched vertex
-// proposes to its best unmatched neighbour under a strict total order on
-// edges (EdgeKey below) and mutual proposals become pairs (a
-// locally-dominant handshake).  Rounds repeat until nothing changes or the
-// round limit; leftovers are self-matched.  The result is symmetric,
-// deterministic for a given seed and independent of scheduling.
+// matchings are sequential greedy scans.  On the GPU the matching is the
+// greedy matching in descending order of a strict total order on edges
+// (EdgeKey below, ties broken by the endpoint ids), computed with the
+// suitor algorithm (Manne and Halappanavar, IPDPS 2014): every vertex tries
+// to become the suitor of its best neighbour whose current suitor it beats,
+// a displaced suitor searches again, and mutual suitors are matched.  The
+// result does not depend on the order of the attempts, so it is symmetric,
+// deterministic for a given seed and independent of scheduling; it needs no
+// rounds and one launch per level.
 //
 // Contraction (graph/mod.rs:284-384), bit-exact for a given symmetric matching:
 //  * coarse ids ascend with the pair's smaller vertex (288-297): exclusive
@@ -57,22 +60,29 @@ __device__ inline EdgeKey edge_key(int32_t v, int32_t u, int32_t w, int32_t dv, 
     return EdgeKey{((uint64_t)dpri << 32) | (uint64_t)((uint32_t)w ^ 0x80000000u), h};
 }
 
-// The rounds keep the unmatched vertices in per-block lists: block b of nseg
-// owns the vertices [b*S, (b+1)*S) (S = seg), proposes and accepts for them
-// only, and compacts its survivors into its own next list with a block scan,
-// so no global counter is shared.  The list order does not affect the
-// matching: every decision depends on the match state alone.
+// Strict total order on edges: the key, then the endpoint ids (graph-local,
+// smaller pair first).  For the edges of one vertex this prefers, among
+// equal keys, the smaller neighbour.
+__device__ inline bool edge_before(const EdgeKey& k1, int32_t a1, int32_t b1, const EdgeKey& k2, int32_t a2,
+                                   int32_t b2) {
+    if (k1.gt(k2)) return true;
+    if (!k1.eq(k2)) return false;
+    const int32_t l1 = min(a1, b1), l2 = min(a2, b2);
+    return l1 < l2 || (l1 == l2 && max(a1, b1) < max(a2, b2));
+}
+
+// suitor[u]: (suitor id << 32) | weight of the edge (suitor, u), or
+// kNoSuitor.  A suitor's key is recomputed from these (and the degrees).
+constexpr uint64_t kNoSuitor = 0xFFFFFFFF00000000ull;
+
 struct MatchArgs {
-    int32_t n, seg, rounds, shem;
+    int32_t n, shem;
     uint64_t seed;
     const int32_t* xadj;
     const int32_t* adjncy;
     const int32_t* adjwgt;
     int32_t* match;
-    int32_t* cand;
-    int32_t* mlist[2];
-    int32_t* seg_cnt;   // [2 * kMatchSegMax]
-    int32_t* flags;     // [kMaxMatchRounds] "round r changed the matching"
+    uint64_t* suitor;
     // batched levels (null for a single graph): graph of each vertex, and per
     // graph its first vertex, matching seed and whether it is matched here
     const int32_t* gseg;
@@ -81,137 +91,121 @@ struct MatchArgs {
     const uint8_t* gon;
 };
 
-__device__ inline void dev_match_init(const MatchArgs& a, int blk, int* s_tmp) {
-    const int64_t lo = (int64_t)blk * a.seg, hi = min(lo + a.seg, (int64_t)a.n);
-    int nout = 0;
-    for (int64_t vb = lo; vb < hi; vb += kBlock) {
-        const int64_t v = vb + threadIdx.x;
-        bool open = false;
-        if (v < hi) {
-            open = a.xadj[v + 1] != a.xadj[v] && (!a.gseg || a.gon[a.gseg[v]]);
-            a.match[v] = open ? -1 : (int32_t)v;
-            a.cand[v] = -1;
-        }
-        int tot;
-        const int pos = block_flag_scan(open, s_tmp, &tot);
-        if (open) a.mlist[0][lo + nout + pos] = (int32_t)v;
-        nout += tot;
-    }
-    if (threadIdx.x == 0) a.seg_cnt[blk] = nout;
+__device__ inline uint64_t ld_agent64(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Proposals of round r for block blk's list.  An open vertex whose previous
-// proposal is still unmatched proposes it again without a rescan: its
-// unmatched neighbours only ever lose members, so the best of them stays the
-// best while it is unmatched.
+__global__ __launch_bounds__(kBlock) void k_suitor_init(int32_t n, uint64_t* __restrict__ suitor) {
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock)
+        suitor[v] = kNoSuitor;
+}
+
+// G lanes per searching vertex.  A vertex v of the level starts a chain: the
+// current searcher scans its neighbours for the best one (edge order) whose
+// suitor it beats, and claims it with a 64-bit CAS on the suitor word it
+// compared against; on success the displaced suitor (if any) becomes the
+// searcher, on failure the scan is repeated.  Keys of a claimed word only
+// grow, so a stale read can only make a neighbour look easier to claim (the
+// CAS then fails and returns the current word, which is used for that
+// neighbour in the next scan): every retry sees a newer word, and the chain
+// ends.
 template <int G>
-__device__ inline void dev_match_propose(const MatchArgs& a, int r, int blk) {
+__global__ __launch_bounds__(kBlock) void k_suitor(MatchArgs a) {
     constexpr int NG = kBlock / G;
     const int gl = group_lane<G>();
-    const int32_t nl = a.seg_cnt[(r & 1) * kMatchSegMax + blk];
-    const int32_t* mine = a.mlist[r & 1] + (int64_t)blk * a.seg;
-    for (int i = threadIdx.x / G; i < nl; i += NG) {
-        const int32_t v = mine[i];
-        const int32_t prev = a.cand[v];
-        if (prev >= 0 && a.match[prev] == -1) continue;   // group-uniform
-        const int32_t beg = a.xadj[v], end = a.xadj[v + 1];
-        const int32_t dv = end - beg;
+    for (int64_t v = (int64_t)blockIdx.x * NG + threadIdx.x / G; v < a.n; v += (int64_t)gridDim.x * NG) {
+        if (a.xadj[v + 1] == a.xadj[v]) continue;                 // group-uniform
         // keys use graph-local ids and the graph's seed, so a graph of a batch
         // is matched exactly as it is alone
         uint64_t seed = a.seed;
         int32_t base = 0;
         if (a.gseg) {
             const int32_t gi = a.gseg[v];
+            if (!a.gon[gi]) continue;
             seed = a.gseed[gi];
             base = a.gv0[gi];
         }
-        EdgeKey best{0ull, 0u};
-        int32_t bu = -1;
-        for (int32_t j = beg + gl; j < end; j += G) {
-            const int32_t u = a.adjncy[j];
-            if (a.match[u] != -1) continue;
-            const int32_t du = a.xadj[u + 1] - a.xadj[u];
-            const EdgeKey key = edge_key(v - base, u - base, a.adjwgt ? a.adjwgt[j] : 1, dv, du, seed, a.shem);
-            if (bu < 0 || key.gt(best) || (key.eq(best) && u < bu)) {
-                best = key;
-                bu = u;
+        int32_t cur = (int32_t)v;
+        int32_t fu = -1;          // neighbour of the last failed claim and the word the CAS returned
+        uint64_t fword = 0;
+        while (cur >= 0) {
+            const int32_t beg = a.xadj[cur], end = a.xadj[cur + 1];
+            const int32_t dc = end - beg;
+            EdgeKey best{0ull, 0u};
+            int32_t bu = -1, bw = 0;
+            uint64_t bseen = 0;
+            for (int32_t j = beg + gl; j < end; j += G) {
+                const int32_t u = a.adjncy[j];
+                const int32_t w = a.adjwgt ? a.adjwgt[j] : 1;
+                uint64_t sv = ld_agent64(&a.suitor[u]);
+                if (u == fu && sv != fword) {
+                    // keep the newer of the loaded word and the CAS result
+                    const int32_t s1 = (int32_t)(sv >> 32);
+                    if (s1 < 0) sv = fword;
+                    else if ((int32_t)(fword >> 32) >= 0) {
+                        const int32_t du0 = a.xadj[u + 1] - a.xadj[u];
+                        const int32_t s2 = (int32_t)(fword >> 32);
+                        const EdgeKey k1 = edge_key(s1 - base, u - base, (int32_t)(uint32_t)sv,
+                                                    a.xadj[s1 + 1] - a.xadj[s1], du0, seed, a.shem);
+                        const EdgeKey k2 = edge_key(s2 - base, u - base, (int32_t)(uint32_t)fword,
+                                                    a.xadj[s2 + 1] - a.xadj[s2], du0, seed, a.shem);
+                        if (edge_before(k2, s2 - base, u - base, k1, s1 - base, u - base)) sv = fword;
+                    }
+                }
+                const int32_t sid = (int32_t)(sv >> 32);
+                if (sid == cur) continue;
+                const int32_t du = a.xadj[u + 1] - a.xadj[u];
+                const EdgeKey k = edge_key(cur - base, u - base, w, dc, du, seed, a.shem);
+                if (sid >= 0) {
+                    const EdgeKey ks = edge_key(sid - base, u - base, (int32_t)(uint32_t)sv,
+                                                a.xadj[sid + 1] - a.xadj[sid], du, seed, a.shem);
+                    if (!edge_before(k, cur - base, u - base, ks, sid - base, u - base)) continue;
+                }
+                if (bu < 0 || edge_before(k, cur - base, u - base, best, cur - base, bu - base)) {
+                    best = k;
+                    bu = u;
+                    bw = w;
+                    bseen = sv;
+                }
             }
-        }
 #pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            const EdgeKey ob{__shfl_xor(best.hi, o, G), __shfl_xor(best.lo, o, G)};
-            const int32_t ou = __shfl_xor(bu, o, G);
-            if (ou >= 0 && (bu < 0 || ob.gt(best) || (ob.eq(best) && ou < bu))) {
-                best = ob;
-                bu = ou;
+            for (int o = G / 2; o > 0; o >>= 1) {
+                const EdgeKey ob{__shfl_xor(best.hi, o, G), __shfl_xor(best.lo, o, G)};
+                const int32_t ou = __shfl_xor(bu, o, G);
+                const int32_t ow = __shfl_xor(bw, o, G);
+                const uint64_t os = __shfl_xor(bseen, o, G);
+                if (ou >= 0 && (bu < 0 || edge_before(ob, cur - base, ou - base, best, cur - base, bu - base))) {
+                    best = ob;
+                    bu = ou;
+                    bw = ow;
+                    bseen = os;
+                }
             }
-        }
-        if (gl == 0) a.cand[v] = bu >= 0 ? bu : -2;
-    }
-}
-
-// Accepts of round r for block blk: mutual proposals become pairs, vertices
-// without an unmatched neighbour (cand -2) retire, the rest of the list is
-// compacted into the block's next list.  Sets flags[r] if anything changed.
-__device__ inline void dev_match_accept(const MatchArgs& a, int r, int blk, int* s_tmp) {
-    const int32_t nl = a.seg_cnt[(r & 1) * kMatchSegMax + blk];
-    const int32_t* mine = a.mlist[r & 1] + (int64_t)blk * a.seg;
-    int32_t* out = a.mlist[(r + 1) & 1] + (int64_t)blk * a.seg;
-    bool ch = false;
-    int nout = 0;
-    for (int ib = 0; ib < nl; ib += kBlock) {
-        const int i = ib + threadIdx.x;
-        int32_t v = -1;
-        bool open = false;
-        if (i < nl) {
-            v = mine[i];
-            const int32_t u = a.cand[v];
-            if (u == -2) {
-                a.match[v] = v;
-                ch = true;
-            } else if (u >= 0 && a.cand[u] == v) {
-                a.match[v] = u;
-                ch = true;
+            if (bu < 0) break;                                      // nothing to claim: cur stays unmatched
+            uint64_t old = 0;
+            if (gl == 0)
+                old = atomicCAS((unsigned long long*)&a.suitor[bu], (unsigned long long)bseen,
+                                ((unsigned long long)(uint32_t)cur << 32) | (uint32_t)bw);
+            old = __shfl(old, 0, G);
+            if (old == bseen) {
+                cur = (int32_t)(bseen >> 32);                          // the displaced suitor searches again
+                fu = -1;
             } else {
-                open = true;
+                fu = bu;
+                fword = old;
             }
         }
-        int tot;
-        const int pos = block_flag_scan(open, s_tmp, &tot);
-        if (open) out[nout + pos] = v;
-        nout += tot;
     }
-    if (threadIdx.x == 0) a.seg_cnt[((r + 1) & 1) * kMatchSegMax + blk] = nout;
-    // one plain store per block that changed something (every writer stores 1)
-    if (__syncthreads_or(ch) && threadIdx.x == 0) a.flags[r] = 1;
 }
 
-__device__ inline void dev_match_finish(const MatchArgs& a, int blk) {
-    const int64_t lo = (int64_t)blk * a.seg, hi = min(lo + a.seg, (int64_t)a.n);
-    for (int64_t v = lo + threadIdx.x; v < hi; v += kBlock)
-        if (a.match[v] == -1) a.match[v] = (int32_t)v;
+// Mutual suitors are matched; every other vertex is matched to itself.
+__global__ __launch_bounds__(kBlock) void k_suitor_pairs(int32_t n, const uint64_t* __restrict__ suitor,
+                                                         int32_t* __restrict__ match) {
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
+        const int32_t s = (int32_t)(suitor[v] >> 32);
+        match[v] = (s >= 0 && (int32_t)(suitor[s] >> 32) == (int32_t)v) ? s : (int32_t)v;
+    }
 }
-
-// One launch per phase; round r runs only if round r-1 matched or retired
-// something.
-__global__ __launch_bounds__(kBlock) void k_match_init(MatchArgs a) {
-    __shared__ int s_tmp[kBlock / 64];
-    dev_match_init(a, blockIdx.x, s_tmp);
-}
-
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_match_propose(MatchArgs a, int r) {
-    if (r > 0 && a.flags[r - 1] == 0) return;
-    dev_match_propose<G>(a, r, blockIdx.x);
-}
-
-__global__ __launch_bounds__(kBlock) void k_match_accept(MatchArgs a, int r) {
-    __shared__ int s_tmp[kBlock / 64];
-    if (r > 0 && a.flags[r - 1] == 0) return;
-    dev_match_accept(a, r, blockIdx.x, s_tmp);
-}
-
-__global__ __launch_bounds__(kBlock) void k_match_finish_seg(MatchArgs a) { dev_match_finish(a, blockIdx.x); }
 
 // Batched levels: vertices of graphs not matched at this level (gseg/gon)
 // lead no coarse vertex and get cmap -1; they leave the hierarchy here.
@@ -582,40 +576,26 @@ inline int grid_cap(int64_t items, int per_block, int cap) {
     return (int)b;
 }
 
-template <int G>
-void match_rounds(MatchArgs& a, int nseg, hipStream_t s) {
-    hipLaunchKernelGGL(k_match_init, dim3(nseg), dim3(kBlock), 0, s, a);
-    for (int r = 0; r < a.rounds; ++r) {
-        hipLaunchKernelGGL(k_match_propose<G>, dim3(nseg), dim3(kBlock), 0, s, a, r);
-        hipLaunchKernelGGL(k_match_accept, dim3(nseg), dim3(kBlock), 0, s, a, r);
-    }
-    hipLaunchKernelGGL(k_match_finish_seg, dim3(nseg), dim3(kBlock), 0, s, a);
-}
-
 }  // namespace
 
-void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, CoarsenBufs& b, hipStream_t s,
+void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, CoarsenBufs& b, hipStream_t s,
                  const GraphSegs* gs) {
     if (g.n <= 0) return;
-    if (rounds > kMaxMatchRounds) rounds = kMaxMatchRounds;
-    const int nb = (g.n + kBlock - 1) / kBlock;
-    // one list segment per resident block (at most kMatchSegMax), each of seg vertices
-    int nseg = device_cus() * 8;
-    if (nseg > kMatchSegMax) nseg = kMatchSegMax;
-    if (nseg > nb) nseg = nb;
-    const int seg = (int)(((int64_t)g.n + nseg - 1) / nseg);
-    nseg = (int)(((int64_t)g.n + seg - 1) / seg);
-    RK_HIP(hipMemsetAsync(b.round_flags, 0, sizeof(int32_t) * kMaxMatchRounds, s));
-    MatchArgs a{g.n, seg, rounds, ctype_shem, seed, g.xadj, g.adjncy, g.adjwgt, b.match, b.cand,
-                {b.mlist[0], b.mlist[1]}, b.seg_cnt, b.round_flags,
-                gs ? gs->gseg : nullptr, gs ? gs->gv0 : nullptr, gs ? gs->gseed : nullptr, gs ? gs->gon : nullptr};
-    switch (pick_group(g.n, g.m)) {
-        case 4: match_rounds<4>(a, nseg, s); break;
-        case 8: match_rounds<8>(a, nseg, s); break;
-        case 16: match_rounds<16>(a, nseg, s); break;
-        case 32: match_rounds<32>(a, nseg, s); break;
-        default: match_rounds<64>(a, nseg, s); break;
+    const MatchArgs a{g.n, ctype_shem, seed, g.xadj, g.adjncy, g.adjwgt, b.match, b.suitor,
+                      gs ? gs->gseg : nullptr, gs ? gs->gv0 : nullptr, gs ? gs->gseed : nullptr,
+                      gs ? gs->gon : nullptr};
+    const int grid = grid_cap(g.n, kBlock, 4096);
+    hipLaunchKernelGGL(k_suitor_init, dim3(grid), dim3(kBlock), 0, s, g.n, b.suitor);
+    const int G = pick_group(g.n, g.m);
+    const int sgrid = grid_cap(g.n, kBlock / G, device_cus() * 8);
+    switch (G) {
+        case 4: hipLaunchKernelGGL(k_suitor<4>, dim3(sgrid), dim3(kBlock), 0, s, a); break;
+        case 8: hipLaunchKernelGGL(k_suitor<8>, dim3(sgrid), dim3(kBlock), 0, s, a); break;
+        case 16: hipLaunchKernelGGL(k_suitor<16>, dim3(sgrid), dim3(kBlock), 0, s, a); break;
+        case 32: hipLaunchKernelGGL(k_suitor<32>, dim3(sgrid), dim3(kBlock), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_suitor<64>, dim3(sgrid), dim3(kBlock), 0, s, a); break;
     }
+    hipLaunchKernelGGL(k_suitor_pairs, dim3(grid), dim3(kBlock), 0, s, g.n, b.suitor, b.match);
     RK_HIP(hipGetLastError());
 }
 

@@ -139,11 +139,8 @@ Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon
     c.hash_first = a.get<int32_t>(2 * (int64_t)m + n + 1);
     c.hash_sum = a.get<int32_t>(2 * (int64_t)m + n + 1);
     c.counters = a.get<int32_t>(16);
-    c.round_flags = a.get<int32_t>(kMaxMatchRounds);
-    c.seg_cnt = a.get<int32_t>(2 * kMatchSegMax);
     c.host_small = r.host_small;
-    c.mlist[0] = a.get<int32_t>((int64_t)n + kMatchSegMax);
-    c.mlist[1] = a.get<int32_t>((int64_t)n + kMatchSegMax);
+    c.suitor = a.get<uint64_t>(n);
     c.scan_tmp = r.scan_tmp;
     B.cut_acc = a.get<unsigned long long>(1);
     B.cut_out = a.get<int32_t>(1);
@@ -325,7 +322,7 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
             }
             {
                 ProfScope ps(kProfMatch, 0.0, cur->n, s);
-                match_graph(*cur, o.ctype_shem, rng.next_u64(), kMatchRounds, B.cb, s);
+                match_graph(*cur, o.ctype_shem, rng.next_u64(), B.cb, s);
             }
             int32_t* cmap = cx.arena.get<int32_t>(cur->n);
             int32_t* heavy = cx.arena.get<int32_t>(cur->n);

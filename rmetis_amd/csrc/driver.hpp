@@ -26,7 +26,6 @@ struct Options {
 };
 
 constexpr int kNOptions = 40;  // types.rs:170
-constexpr int kMatchRounds = 24;   // handshake rounds per level at most
 
 struct ArenaAlloc : CoarseAlloc {
     Arena& a;

@@ -176,15 +176,11 @@ struct CoarsenBufs {
     int32_t* hash_first = nullptr;
     int32_t* hash_sum = nullptr;
     int32_t* counters = nullptr;   // [16]: 0 cn, 1 cm, 2 nbig, 4 weight-not-one flag, 6-7 hash bump (u64), 8 nheavy
-    int32_t* round_flags = nullptr;  // [kMaxMatchRounds] "round r changed the matching"
-    int32_t* seg_cnt = nullptr;      // [2 * kMatchSegMax] per-block list lengths, current / next round
     int32_t* host_small = nullptr;   // pinned host scratch for the per-level counter read
-    int32_t* mlist[2] = {nullptr, nullptr};  // [n + kMatchSegMax] each: per-block lists of unmatched vertices
+    uint64_t* suitor = nullptr;      // [n] matching: (suitor id << 32) | edge weight
     void* scan_tmp = nullptr;
 };
 
-constexpr int kMaxMatchRounds = 64;
-constexpr int kMatchSegMax = 4096;   // matching blocks (list segments) at most
 constexpr int kContractBig = 64;   // coarse vertices of larger degree sum use the global-hash path
 
 // Deterministic seeded handshake matching (ctype 1 heavy edge, 0 random).
@@ -198,7 +194,7 @@ struct GraphSegs {
     int32_t* cgseg = nullptr;          // [n] out: graph of each coarse vertex
 };
 
-void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, int rounds, CoarsenBufs& b, hipStream_t s,
+void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, CoarsenBufs& b, hipStream_t s,
                  const GraphSegs* gs = nullptr);
 struct CoarseAlloc {
     virtual int32_t* alloc_i32(int64_t count) = 0;
