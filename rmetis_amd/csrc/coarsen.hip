@@ -182,11 +182,26 @@ __global__ __launch_bounds__(kBlock) void k_suitor(MatchArgs a) {
                 }
             }
             if (bu < 0) break;                                      // nothing to claim: cur stays unmatched
+            // Claim bu.  When the word changed meanwhile but its new suitor is
+            // still beaten, claim again at once: bu stays the best choice
+            // (the other neighbours can only have become harder to claim).
             uint64_t old = 0;
-            if (gl == 0)
-                old = atomicCAS((unsigned long long*)&a.suitor[bu], (unsigned long long)bseen,
-                                ((unsigned long long)(uint32_t)cur << 32) | (uint32_t)bw);
+            if (gl == 0) {
+                const unsigned long long mine = ((unsigned long long)(uint32_t)cur << 32) | (uint32_t)bw;
+                const int32_t du = a.xadj[bu + 1] - a.xadj[bu];
+                for (;;) {
+                    old = atomicCAS((unsigned long long*)&a.suitor[bu], (unsigned long long)bseen, mine);
+                    if (old == bseen) break;
+                    const int32_t sid = (int32_t)(old >> 32);
+                    if (sid == cur) break;
+                    const EdgeKey ks = edge_key(sid - base, bu - base, (int32_t)(uint32_t)old,
+                                                a.xadj[sid + 1] - a.xadj[sid], du, seed, a.shem);
+                    if (!edge_before(best, cur - base, bu - base, ks, sid - base, bu - base)) break;
+                    bseen = old;                                        // still beaten: claim against it
+                }
+            }
             old = __shfl(old, 0, G);
+            bseen = __shfl(bseen, 0, G);
             if (old == bseen) {
                 cur = (int32_t)(bseen >> 32);                          // the displaced suitor searches again
                 fu = -1;
@@ -586,7 +601,7 @@ void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, CoarsenBufs& 
                       gs ? gs->gon : nullptr};
     const int grid = grid_cap(g.n, kBlock, 4096);
     hipLaunchKernelGGL(k_suitor_init, dim3(grid), dim3(kBlock), 0, s, g.n, b.suitor);
-    const int G = pick_group(g.n, g.m);
+    const int G = pick_group(g.n, g.m) > 8 ? 8 : pick_group(g.n, g.m);   // short scans: chains dominate
     const int sgrid = grid_cap(g.n, kBlock / G, device_cus() * 8);
     switch (G) {
         case 4: hipLaunchKernelGGL(k_suitor<4>, dim3(sgrid), dim3(kBlock), 0, s, a); break;
