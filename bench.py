@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-pool", type=int, default=48, help="serve: distinct graphs the requests draw from")
     ap.add_argument("--slots", type=int, default=3, help="serve: batched calls in flight at once")
+    ap.add_argument("--serve-max-batch", type=int, default=0,
+                    help="serve: vertices per batched call at most (0: the server's default)")
     ap.add_argument("--plain-sample", type=int, default=40, help="serve: requests timed on the plain path")
     ap.add_argument("--plain-stride", type=int, default=3,
                     help="batch workloads: the plain path is timed on every n-th graph (odd: grids and RGGs alike)")
@@ -422,7 +424,7 @@ def bench_serve(args):
     maxn = max(g.n for g in pool)
     ring = 256
     outs = [DA(count=maxn, dtype=np.int32) for _ in range(ring)]
-    srv = R.Server(slots=args.slots)
+    srv = R.Server(slots=args.slots, max_batch_vertices=args.serve_max_batch)
     # warm-up (untimed): every pool graph once, so arenas and staging are grown
     wids = [srv.submit_device(g.n, 2, *d, 8, outs[i % ring], ubvec=ub, options=opts, seed=1)
             for i, (g, d) in enumerate(zip(pool, dev)) if i < ring]

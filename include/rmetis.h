@@ -119,11 +119,11 @@ int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, cons
 
 /* Serving (no counterpart in the reference): a request loop with continuous
  * batching.  Each request is one graph with its own nparts and seed; a
- * server runs up to `slots` batched calls at once (one HIP stream each),
+ * server runs up to `slots` (1..3) batched calls at once (one HIP stream each),
  * every idle slot taking the oldest pending request plus the pending
  * requests that can share its batch (same ncon, ubvec and options but the
  * seed) within the device-memory budget (<= 0: 80 % of the free memory at
- * creation) and max_batch_vertices (<= 0: 2^23).  A request's result equals
+ * creation) and max_batch_vertices (<= 0: 2^25).  A request's result equals
  * that of RMETIS_PartGraphKway with its seed, whatever it was batched with.
  * Returns NULL on error (rmetis_hip_last_error()). */
 typedef struct rmetis_hip_server rmetis_hip_server;

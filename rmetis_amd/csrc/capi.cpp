@@ -297,7 +297,11 @@ int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, cons
 rmetis_hip_server* rmetis_hip_server_create(int32_t slots, int64_t memory_budget, int64_t max_batch_vertices) {
     rmetis_hip_server* out = nullptr;
     const int32_t rc = guarded([&]() -> int32_t {
-        if (slots < 1 || slots > 16) return METIS_ERROR_INPUT;
+        // at most kMaxServeSlots: a run with 4 slots stalled (cause not
+        // found), so larger servers are refused rather than run
+        if (slots < 1) return METIS_ERROR_INPUT;
+        if (slots > kMaxServeSlots) throw Error(METIS_ERROR_INPUT, "a server has at most " +
+                                                               std::to_string(kMaxServeSlots) + " slots");
         int cnt = 0;
         if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0)
             throw Error(-4, "no HIP device is available (the k-way path runs only on the GPU)");
@@ -308,7 +312,9 @@ rmetis_hip_server* rmetis_hip_server_create(int32_t slots, int64_t memory_budget
             RK_HIP(hipMemGetInfo(&free_b, &total_b));
             memory_budget = (int64_t)(free_b / 10 * 8);
         }
-        if (max_batch_vertices <= 0) max_batch_vertices = 1ll << 23;
+        // large batches keep more graphs' admissions (one workgroup each) on
+        // the device at once; the memory budget still bounds what runs
+        if (max_batch_vertices <= 0) max_batch_vertices = 1ll << 25;
         auto* h = new rmetis_hip_server();
         h->device = dev;
         h->s = std::make_unique<Server>(dev, slots, memory_budget, max_batch_vertices);

@@ -14,6 +14,8 @@
 
 namespace rk {
 
+constexpr int kMaxServeSlots = 3;   // batched calls a server runs at once, at most
+
 // One k-way request: a copy of the caller's graph, its part count, shared
 // options and its own seed; the result is filled in by the worker that ran it.
 struct ServeRequest {
