@@ -181,9 +181,11 @@ void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* d_
     part_weights(g, d_part, k, B.rb.pw, s);                                          // greedy.rs:50
     RK_HIP(hipMemsetAsync(B.rb.done, 0, sizeof(int32_t), s));
     refine_pass(g, k, 0, nheavy, heavy_list, B.rb, d_part, s);                       // greedy.rs:60-123
-    // cut rounds (126-190); after the first, only the tiles around the last
-    // round's moves are recomputed
-    for (int it = 0; it < niter; ++it) refine_pass(g, k, 1, nheavy, heavy_list, B.rb, d_part, s, it > 0);
+    // cut rounds (126-190); on large levels, after the first, only the tiles
+    // around the last round's moves are recomputed
+    const bool inc = g.n >= kIncrementalMinN;
+    for (int it = 0; it < niter; ++it)
+        refine_pass(g, k, 1, nheavy, heavy_list, B.rb, d_part, s, inc && it > 0, inc && it + 1 < niter);
     if (st) {
         RK_HIP(hipEventRecord(e1, s));
         int32_t passes = 0;

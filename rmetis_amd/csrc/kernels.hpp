@@ -118,9 +118,14 @@ int heavy_list_sync(const DevGraph& g, int32_t* list, RefineBufs& b, hipStream_t
 // One greedy pass: mode 0 balance repair, mode 1 cut optimisation.  Reads
 // b.pw (running weights) and updates it, part, b.moves/b.nmoves; a
 // cut-optimisation pass with no move sets *b.done, which turns every later
-// pass of the level into a no-op.
+// pass of the level into a no-op.  incremental: recompute only the tiles the
+// previous pass marked; mark_next: mark the tiles around this pass's moves
+// for an incremental next pass.
 void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32_t* heavy_list, RefineBufs& b,
-                 int32_t* part, hipStream_t s, bool incremental = false);
+                 int32_t* part, hipStream_t s, bool incremental = false, bool mark_next = false);
+// Levels from this size up run their cut passes after the first
+// incrementally; below it a full gain launch costs less than marking.
+constexpr int32_t kIncrementalMinN = 1 << 18;
 // ---- admit.hip ----
 // Ordered admission of the candidates that the gain kernels left in
 // b.tile_rec / b.tile_cnt (one workgroup; updates b.pw, part, b.moves).

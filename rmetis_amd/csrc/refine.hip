@@ -608,7 +608,7 @@ int heavy_list_sync(const DevGraph& g, int32_t* list, RefineBufs& b, hipStream_t
 }
 
 void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32_t* heavy_list, RefineBufs& b,
-                 int32_t* part, hipStream_t s, bool incremental) {
+                 int32_t* part, hipStream_t s, bool incremental, bool mark_next) {
     if (g.n <= 0) return;
     const int ntiles = (g.n + kTile - 1) / kTile;
     GainArgs ga{g.n, g.ncon, k, mode, g.xadj, g.adjncy, g.adjwgt, g.vwgt, part, b.pw, b.omax,
@@ -628,7 +628,7 @@ void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32
         ProfScope ps(kProfAdmit, 0.0, g.n, s);
         admit_pass(g, k, mode, ntiles, b, part, s);
     }
-    if (mode == 1 && b.tile_dirty && b.full)   // one wave per move, not on the admission's chain
+    if (mark_next && mode == 1 && b.tile_dirty && b.full)   // one wave per move, not on the admission's chain
         hipLaunchKernelGGL(k_mark_moves, dim3(256), dim3(kBlock), 0, s, b.moves, b.nmoves, g.xadj, g.adjncy,
                            b.tile_dirty, b.full, kMarkMaxDeg);
     RK_HIP(hipGetLastError());

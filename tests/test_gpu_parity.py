@@ -244,6 +244,34 @@ def test_greedy_refine(ncon, k):
     assert np.array_equal(p_g, p_o)
 
 
+def test_greedy_refine_large_level_incremental():
+    # a level of at least 2^18 vertices runs its later cut passes
+    # incrementally (only the tiles around the previous pass's moves): the
+    # result must still equal the oracle's full passes
+    from rmetis_amd import synth
+    s = synth.grid(640, 440)
+    n = s.n
+    rng = np.random.default_rng(5)
+    src = np.repeat(np.arange(n), np.diff(s.xadj))
+    lo = np.minimum(src, s.adjncy).astype(np.int64)
+    hi = np.maximum(src, s.adjncy).astype(np.int64)
+    _, inv = np.unique(lo * n + hi, return_inverse=True)
+    aw = rng.integers(1, 9, size=inv.max() + 1).astype(np.int32)[inv]
+    vw = rng.integers(1, 4, n).astype(np.int32)
+    k = 16
+    part = (np.arange(n) * k // n).astype(np.int32)
+    flip = rng.random(n) < 0.2
+    part[flip] = rng.integers(0, k, int(flip.sum()))
+    tpw = np.full(k, vw.sum() / k)
+    ub = np.array([1.03])
+    g = _graph(s.xadj, s.adjncy, vwgt=vw, adjwgt=aw)
+    p_o, cut_o = O.greedy_refine(1, s.xadj, s.adjncy, vw, aw, part, k, tpw, ub, 10)
+    p_g, cut_g = R.greedy_refine(g, part, k, tpw, ub, 10)
+    assert n >= 1 << 18
+    assert cut_g == cut_o
+    assert np.array_equal(p_g, p_o)
+
+
 def test_greedy_reference_cases():
     # tests/parallel.rs:91-116: cycle4 with tpwgts [2,2], ub 1.03
     x, a = cycle4()
