@@ -224,11 +224,19 @@ __global__ __launch_bounds__(kBlock) void k_suitor_pairs(int32_t n, const uint64
 
 // Batched levels: vertices of graphs not matched at this level (gseg/gon)
 // lead no coarse vertex and get cmap -1; they leave the hierarchy here.
+// Also clears the contraction's accumulators (degsum, cdeg; counters 2..9)
+// in place of separate fills.
 __global__ __launch_bounds__(kBlock) void k_leader(int32_t n, const int32_t* __restrict__ match,
                                                    int32_t* __restrict__ leader, const int32_t* __restrict__ gseg,
-                                                   const uint8_t* __restrict__ gon) {
+                                                   const uint8_t* __restrict__ gon, int32_t* __restrict__ degsum,
+                                                   int32_t* __restrict__ cdeg, int32_t* __restrict__ counters) {
     const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (v < n) leader[v] = (match[v] >= (int32_t)v && (!gseg || gon[gseg[v]])) ? 1 : 0;
+    if (v < n) {
+        leader[v] = (match[v] >= (int32_t)v && (!gseg || gon[gseg[v]])) ? 1 : 0;
+        degsum[v] = 0;
+        cdeg[v] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x >= 2 && threadIdx.x < 10) counters[threadIdx.x] = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_cmap(int32_t n, int32_t ncon, const int32_t* __restrict__ xadj,
@@ -630,11 +638,9 @@ ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, 
     cg.adjwgt = al.alloc_i32(g.m > 0 ? g.m : 1);
     const int32_t* gseg = gs ? gs->gseg : nullptr;
     const uint8_t* gon = gs ? gs->gon : nullptr;
-    hipLaunchKernelGGL(k_leader, dim3(nb), dim3(kBlock), 0, s, n, match, b.leader, gseg, gon);
+    hipLaunchKernelGGL(k_leader, dim3(nb), dim3(kBlock), 0, s, n, match, b.leader, gseg, gon, b.degsum, b.cdeg,
+                       b.counters);
     exclusive_scan_i32(b.leader, b.cid, n, cn_dev, b.scan_tmp, s);      // coarse ids (graph/mod.rs:288-297)
-    RK_HIP(hipMemsetAsync(b.degsum, 0, sizeof(int32_t) * n, s));
-    RK_HIP(hipMemsetAsync(b.cdeg, 0, sizeof(int32_t) * n, s));
-    RK_HIP(hipMemsetAsync(b.counters + 2, 0, 8 * sizeof(int32_t), s));
     hipLaunchKernelGGL(k_cmap, dim3(nb), dim3(kBlock), 0, s, n, g.ncon, g.xadj, g.vwgt, g.vsize, match, b.cid, cmap,
                        b.members, b.degsum, cg.vwgt, cg.vsize, gseg, gon, gs ? gs->cgseg : nullptr);
     exclusive_scan_i32(b.degsum, b.tmp_off, n, nullptr, b.scan_tmp, s);
@@ -651,7 +657,7 @@ ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, 
     exclusive_scan_i32(b.cdeg, cg.xadj, n, cg.xadj + n, b.scan_tmp, s);   // cxadj[cn..n] = cm
     hipLaunchKernelGGL(k_contract_copy, dim3(grid_cap(n, kBlock / 16, 65536)), dim3(kBlock), 0, s, n, cn_dev,
                        b.tmp_off, b.tmp_adj, b.tmp_w, cg.xadj, cg.adjncy, cg.adjwgt, b.counters);
-    if (heavy_list) heavy_list_build(n, cn_dev, cg.xadj, heavy_list, b.counters + 8, s);
+    if (heavy_list) heavy_list_build(n, cn_dev, cg.xadj, heavy_list, b.counters + 8, s, true);
     // one read of counters[0..8] (cn, cm, ..., the weight flag, nheavy)
     int32_t* hc = b.host_small;
     RK_HIP(hipMemcpyAsync(hc, b.counters, 9 * sizeof(int32_t), hipMemcpyDeviceToHost, s));

@@ -111,7 +111,7 @@ struct AdmitArgs {
 // Heavy-vertex list (degree > kGainHeavy) of a level with n = *n_dev (or n_max
 // when n_dev is null) vertices; the count lands in *count (no host sync).
 void heavy_list_build(int32_t n_max, const int32_t* n_dev, const int32_t* xadj, int32_t* list, int32_t* count,
-                      hipStream_t s);
+                      hipStream_t s, bool count_zeroed = false);
 // heavy_list_build for a level whose size is known on the host, returning the
 // count (synchronises).
 int heavy_list_sync(const DevGraph& g, int32_t* list, RefineBufs& b, hipStream_t s);

@@ -592,8 +592,8 @@ void gain_table(const DevGraph& g, int32_t k, int nheavy, const int32_t* heavy_l
 }
 
 void heavy_list_build(int32_t n_max, const int32_t* n_dev, const int32_t* xadj, int32_t* list, int32_t* count,
-                      hipStream_t s) {
-    RK_HIP(hipMemsetAsync(count, 0, sizeof(int32_t), s));
+                      hipStream_t s, bool count_zeroed) {
+    if (!count_zeroed) RK_HIP(hipMemsetAsync(count, 0, sizeof(int32_t), s));
     if (n_max > 0)
         hipLaunchKernelGGL(k_heavy_list, dim3((n_max + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n_max, n_dev,
                            xadj, kGainHeavy, list, count);
