@@ -164,7 +164,8 @@ std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffer
 }
 
 void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* d_fmax, const long long* d_omax,
-                  int niter, int32_t* d_part, int nheavy, const int32_t* heavy_list, Buffers& B, LevelStat* st) {
+                  int niter, int32_t* d_part, int nheavy, const int32_t* heavy_list, Buffers& B, LevelStat* st,
+                  bool cleared) {
     // No host synchronisation unless statistics are requested: the
     // thresholds already live on the device and the pass count is fixed
     // (a cut pass that moves nothing turns the remaining passes into no-ops).
@@ -178,8 +179,8 @@ void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* d_
         RK_HIP(hipEventRecord(e0, s));
         RK_HIP(hipMemsetAsync(B.rb.passes, 0, sizeof(int32_t), s));
     }
-    part_weights(g, d_part, k, B.rb.pw, s);                                          // greedy.rs:50
-    RK_HIP(hipMemsetAsync(B.rb.done, 0, sizeof(int32_t), s));
+    part_weights(g, d_part, k, B.rb.pw, s, cleared);                                 // greedy.rs:50
+    if (!cleared) RK_HIP(hipMemsetAsync(B.rb.done, 0, sizeof(int32_t), s));
     refine_pass(g, k, 0, nheavy, heavy_list, B.rb, d_part, s);                       // greedy.rs:60-123
     // cut rounds (126-190); on large levels, after the first, only the tiles
     // around the last round's moves are recomputed
@@ -408,9 +409,9 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
             const int32_t fheavy = i == 0 ? fin_nheavy : levels[i - 1].nheavy;
             const int32_t* flist = i == 0 ? fin_heavy : levels[i - 1].heavy;
             int32_t* pf = cx.arena.get<int32_t>(fine.n);
-            project(fine.n, levels[i].cmap, pc, pf, s);
+            project(fine.n, levels[i].cmap, pc, pf, s, B.rb.pw, kc, B.rb.done, 1);
             refine_level(cx, fine, k, d_thr + 2 * kc * i, d_thr + 2 * kc * i + kc, o.niter, pf, fheavy, flist, B,
-                         st ? &lst[i] : nullptr);
+                         st ? &lst[i] : nullptr, true);
             pc = pf;
         }
         edge_cut(g, pc, B.cut_acc, B.cut_out, pick_group(g.n, g.m), s);    // kway.rs:94

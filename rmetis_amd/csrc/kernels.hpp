@@ -44,8 +44,12 @@ void fill_tiles(const int4* desc, int32_t ndesc, int32_t* tile_v0, int32_t* tile
 int pick_group(int32_t n, int32_t m);
 void edge_cut(const DevGraph& g, const int32_t* part, unsigned long long* acc, int32_t* cut_out, int G,
               hipStream_t s);
-void part_weights(const DevGraph& g, const int32_t* part, int32_t k, int32_t* pw, hipStream_t s);
-void project(int32_t n_fine, const int32_t* cmap, const int32_t* part_c, int32_t* part_f, hipStream_t s);
+// zeroed: pw already holds zeros (a clear folded into an earlier kernel)
+void part_weights(const DevGraph& g, const int32_t* part, int32_t k, int32_t* pw, hipStream_t s,
+                  bool zeroed = false);
+// part_f[v] = part_c[cmap[v]]; also clears clear_a[0..na) and clear_b[0..nb)
+void project(int32_t n_fine, const int32_t* cmap, const int32_t* part_c, int32_t* part_f, hipStream_t s,
+             int32_t* clear_a = nullptr, int64_t na = 0, int32_t* clear_b = nullptr, int32_t nb = 0);
 // out[c] = sum over v of vwgt[v*ncon+c] as int64 (g.vwgt must be present).
 void column_sums(const DevGraph& g, long long* out, hipStream_t s);
 

@@ -76,14 +76,21 @@ __global__ __launch_bounds__(kBlock) void k_part_weights_global(int32_t n, int32
 }
 
 // Vertices with cmap -1 (graphs of a batch whose coarsest level this is)
-// keep the part already stored for them.
+// keep the part already stored for them.  Block 0 also clears clear_a[0..na)
+// and clear_b[0..nb) for the level's refinement (part weights, done flag)
+// in place of separate fills.
 __global__ __launch_bounds__(kBlock) void k_project(int32_t n, const int32_t* __restrict__ cmap,
                                                     const int32_t* __restrict__ part_c,
-                                                    int32_t* __restrict__ part_f) {
+                                                    int32_t* __restrict__ part_f, int32_t* __restrict__ clear_a,
+                                                    int64_t na, int32_t* __restrict__ clear_b, int32_t nb) {
     int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v < n) {
         const int32_t c = cmap[v];
         if (c >= 0) part_f[v] = part_c[c];
+    }
+    if (blockIdx.x == 0) {
+        for (int64_t i = threadIdx.x; i < na; i += kBlock) clear_a[i] = 0;
+        for (int32_t i = threadIdx.x; i < nb; i += kBlock) clear_b[i] = 0;
     }
 }
 
@@ -315,9 +322,9 @@ void edge_cut(const DevGraph& g, const int32_t* part, unsigned long long* acc, i
     RK_HIP(hipGetLastError());
 }
 
-void part_weights(const DevGraph& g, const int32_t* part, int32_t k, int32_t* pw, hipStream_t s) {
+void part_weights(const DevGraph& g, const int32_t* part, int32_t k, int32_t* pw, hipStream_t s, bool zeroed) {
     const int64_t kc = (int64_t)k * g.ncon;
-    RK_HIP(hipMemsetAsync(pw, 0, kc * sizeof(int32_t), s));
+    if (!zeroed) RK_HIP(hipMemsetAsync(pw, 0, kc * sizeof(int32_t), s));
     if (kc <= kPwLds) {
         hipLaunchKernelGGL(k_part_weights_lds, dim3(grid_for(g.n, kPwVerts, 1 << 30)), dim3(kBlock), 0, s,
                            g.n, g.ncon, g.vwgt, part, (int32_t)kc, pw);
@@ -328,10 +335,11 @@ void part_weights(const DevGraph& g, const int32_t* part, int32_t k, int32_t* pw
     RK_HIP(hipGetLastError());
 }
 
-void project(int32_t n_fine, const int32_t* cmap, const int32_t* part_c, int32_t* part_f, hipStream_t s) {
+void project(int32_t n_fine, const int32_t* cmap, const int32_t* part_c, int32_t* part_f, hipStream_t s,
+             int32_t* clear_a, int64_t na, int32_t* clear_b, int32_t nb) {
     if (n_fine <= 0) return;
     hipLaunchKernelGGL(k_project, dim3((n_fine + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n_fine, cmap,
-                       part_c, part_f);
+                       part_c, part_f, clear_a, clear_a ? na : 0, clear_b, clear_b ? nb : 0);
     RK_HIP(hipGetLastError());
 }
 
