@@ -145,6 +145,13 @@ __device__ inline int block_flag_scan(bool flag, int* s_tmp, int* total) {
 }
 
 // Inclusive wave scan of int64 values.
+// Raw buffer resource over n int32 values at p (stride 0, range-checked in
+// bytes; the third dword is the gfx9 data-format word).  Loads at or past
+// the end return 0 and touch no memory.
+__device__ inline __amdgpu_buffer_rsrc_t edge_rsrc(const int32_t* p, int n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(p), (short)0, n * 4, 0x00020000);
+}
+
 __device__ inline long long wave_incl_scan(long long x) {
     const int lane = __lane_id();
 #pragma unroll

@@ -12,35 +12,116 @@ namespace rk {
 
 namespace {
 
-// G lanes per vertex, blocks stride over vertices.  Each lane keeps an int64
-// partial; the block sum is added to one 64-bit accumulator (integer addition
-// is order independent, so the result is deterministic).
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_edge_cut(int32_t n, const int32_t* __restrict__ xadj,
-                                                     const int32_t* __restrict__ adjncy,
-                                                     const int32_t* __restrict__ adjwgt,
-                                                     const int32_t* __restrict__ part,
-                                                     unsigned long long* __restrict__ acc) {
-    constexpr int NG = kBlock / G;
-    __shared__ long long s_w[kBlock / 64];
-    const int gl = group_lane<G>();
-    const int grp = threadIdx.x / G;
-    long long sum = 0;
-    for (int64_t v = (int64_t)blockIdx.x * NG + grp; v < n; v += (int64_t)gridDim.x * NG) {
-        const int32_t pv = part[v];
-        const int32_t beg = xadj[v], end = xadj[v + 1];
-        for (int32_t j = beg + gl; j < end; j += G) {
-            if (part[adjncy[j]] != pv) sum += adjwgt ? adjwgt[j] : 1;
+constexpr int kCutWaves = 4;              // waves (independent tiles) per edge-cut block
+constexpr int kCutCap = 1024;             // entries a wave stages per batch
+constexpr int kCutItems = kCutCap / 64;
+constexpr int kCutBlocksPerCU = 4;
+
+// Edge cut, staged like the gain kernel: every wave takes tiles of 64
+// consecutive vertices (lane = vertex, XCD-contiguous tile regions).  A
+// batch of whole vertices is one contiguous entry range: neighbour ids and
+// weights by range-checked buffer loads, then all part[] gathers in flight,
+// then (part, weight) pairs in LDS; each lane sums the weights of its entries
+// into other parts.  A tile holding a vertex of more than kCutCap entries is
+// summed directly (a lane per vertex, the whole wave for each such vertex).
+// Sums are exact in 64 bits and added once per wave (order independent).
+__global__ __launch_bounds__(64 * kCutWaves) void k_edge_cut_tile(int32_t n, const int32_t* __restrict__ xadj,
+                                                                  const int32_t* __restrict__ adjncy,
+                                                                  const int32_t* __restrict__ adjwgt,
+                                                                  const int32_t* __restrict__ part,
+                                                                  unsigned long long* __restrict__ acc) {
+    __shared__ int2 s_e_all[kCutWaves][kCutCap];
+    const int wv = threadIdx.x >> 6, lane = __lane_id();
+    int2* s_e = s_e_all[wv];
+    const int ntiles = (int)(((int64_t)n + 63) / 64);
+    const WaveTiles wt = wave_tiles(ntiles, kCutWaves);
+    unsigned long long sum = 0;
+    auto w_at = [&](int64_t j) __attribute__((always_inline)) -> uint32_t {
+        return adjwgt ? (uint32_t)adjwgt[j] : 1u;
+    };
+    for (int it = 0; it < wt.count; ++it) {
+        const int tile = wt.base + it * wt.stride;
+        const int64_t v = (int64_t)tile * 64 + lane;
+        const bool own = v < n;
+        int32_t beg = 0, end = 0, pv = -1;
+        if (own) {
+            beg = xadj[v];
+            end = xadj[v + 1];
+            pv = part[v];
+        }
+        const int32_t deg = end - beg;
+        uint64_t bigm = __ballot(deg > kCutCap);
+        if (bigm) {
+            if (deg <= kCutCap)
+                for (int32_t j = beg; j < end; ++j)
+                    if (part[adjncy[j]] != pv) sum += w_at(j);
+            while (bigm) {
+                const int b = __builtin_ctzll(bigm);
+                bigm &= bigm - 1;
+                const int32_t bb = __shfl(beg, b, 64), be = __shfl(end, b, 64), bp = __shfl(pv, b, 64);
+                for (int32_t j = bb + lane; j < be; j += 64)
+                    if (part[adjncy[j]] != bp) sum += w_at(j);
+            }
+            continue;
+        }
+        // exclusive prefix of the degrees over the tile
+        int inc = deg;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        const int loff = inc - deg;
+        const int32_t x0 = __builtin_amdgcn_readfirstlane(beg);   // lane 0 owns v0 < n
+        for (int vs = 0; vs < 64;) {
+            const int P0 = __shfl(loff, vs, 64);
+            const uint64_t fit = __ballot(lane >= vs && loff + deg - P0 <= kCutCap);
+            const int ve = 64 - __builtin_clzll(fit);               // fit is lanes [vs, ve), ve > vs
+            const int P1 = __shfl(loff + deg, ve - 1, 64);
+            if (P1 > P0) {
+                const int p0 = __builtin_amdgcn_readfirstlane(P0);
+                const int ne = __builtin_amdgcn_readfirstlane(P1 - P0);
+                const int64_t e0 = (int64_t)x0 + p0;
+                const int32_t* wsrc = adjwgt ? adjwgt : adjncy;
+                const int lb4 = lane * 4;
+                int32_t uu[kCutItems], ww[kCutItems], pu[kCutItems];
+#pragma unroll
+                for (int i = 0; i < kCutItems; ++i)
+                    uu[i] = __builtin_amdgcn_raw_buffer_load_b32(edge_rsrc(adjncy + e0 + i * 64, max(ne - i * 64, 0)),
+                                                                 lb4, 0, 0);
+#pragma unroll
+                for (int i = 0; i < kCutItems; ++i)
+                    ww[i] = __builtin_amdgcn_raw_buffer_load_b32(edge_rsrc(wsrc + e0 + i * 64, max(ne - i * 64, 0)),
+                                                                 lb4, 0, 0);
+#pragma unroll
+                for (int i = 0; i < kCutItems; ++i) pu[i] = part[uu[i]];
+#pragma unroll
+                for (int i = 0; i < kCutItems; ++i) {
+                    const int p = P0 + lane + i * 64;
+                    if (p < P1) s_e[p - P0] = make_int2(pu[i], adjwgt ? ww[i] : 1);
+                }
+            }
+            wave_sync();
+            if (lane >= vs && lane < ve && deg > 0) {
+                const int lb = loff - P0, le = lb + deg;
+                int e = lb;
+                for (; e + 4 <= le; e += 4) {
+                    const int2 a0 = s_e[e], a1 = s_e[e + 1], a2 = s_e[e + 2], a3 = s_e[e + 3];
+                    sum += (a0.x != pv ? (uint32_t)a0.y : 0u) + (uint64_t)(a1.x != pv ? (uint32_t)a1.y : 0u) +
+                           (uint64_t)(a2.x != pv ? (uint32_t)a2.y : 0u) + (uint64_t)(a3.x != pv ? (uint32_t)a3.y : 0u);
+                }
+                for (; e < le; ++e) {
+                    const int2 a0 = s_e[e];
+                    if (a0.x != pv) sum += (uint32_t)a0.y;
+                }
+            }
+            wave_sync();
+            vs = ve;
         }
     }
-    long long inc = wave_incl_scan(sum);
-    if (__lane_id() == 63) s_w[threadIdx.x >> 6] = inc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        long long t = 0;
-        for (int w = 0; w < kBlock / 64; ++w) t += s_w[w];
-        atomicAdd(acc, (unsigned long long)t);
-    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if (lane == 0 && sum) atomicAdd(acc, sum);
 }
 
 constexpr int kPwVerts = 4096;   // vertices per block in the part-weight kernel
@@ -309,15 +390,12 @@ inline int grid_for(int64_t items, int per_block, int cap = 8192) {
 
 void edge_cut(const DevGraph& g, const int32_t* part, unsigned long long* acc, int32_t* cut_out,
               int G, hipStream_t s) {
+    (void)G;
     RK_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long), s));
-    const int grid = grid_for(g.n, kBlock / G, 16384);
-    switch (G) {
-        case 4: hipLaunchKernelGGL(k_edge_cut<4>, dim3(grid), dim3(kBlock), 0, s, g.n, g.xadj, g.adjncy, g.adjwgt, part, acc); break;
-        case 8: hipLaunchKernelGGL(k_edge_cut<8>, dim3(grid), dim3(kBlock), 0, s, g.n, g.xadj, g.adjncy, g.adjwgt, part, acc); break;
-        case 16: hipLaunchKernelGGL(k_edge_cut<16>, dim3(grid), dim3(kBlock), 0, s, g.n, g.xadj, g.adjncy, g.adjwgt, part, acc); break;
-        case 32: hipLaunchKernelGGL(k_edge_cut<32>, dim3(grid), dim3(kBlock), 0, s, g.n, g.xadj, g.adjncy, g.adjwgt, part, acc); break;
-        default: hipLaunchKernelGGL(k_edge_cut<64>, dim3(grid), dim3(kBlock), 0, s, g.n, g.xadj, g.adjncy, g.adjwgt, part, acc); break;
-    }
+    const int64_t ntiles = ((int64_t)g.n + 63) / 64;
+    const int64_t blocks = (ntiles + kCutWaves - 1) / kCutWaves, resident = (int64_t)device_cus() * kCutBlocksPerCU;
+    hipLaunchKernelGGL(k_edge_cut_tile, dim3((unsigned)(blocks < resident ? blocks : resident)),
+                       dim3(64 * kCutWaves), 0, s, g.n, g.xadj, g.adjncy, g.adjwgt, part, acc);
     hipLaunchKernelGGL(k_finish_cut, dim3(1), dim3(1), 0, s, acc, cut_out);
     RK_HIP(hipGetLastError());
 }

@@ -85,13 +85,6 @@ constexpr int kGItems = kGCap / 64;      // staged edges per lane per batch
 constexpr int kGainBlocksPerCU = 4;      // resident gain blocks per CU (LDS-limited)
 static_assert(kTile == 64, "a gain tile is one wave");
 
-// Raw buffer resource over n int32 values at p (stride 0, range-checked in
-// bytes; the third dword is the gfx9 data-format word).  Loads at or past
-// the end return 0 and touch no memory.
-__device__ inline __amdgpu_buffer_rsrc_t edge_rsrc(const int32_t* p, int n) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(p), (short)0, n * 4, 0x00020000);
-}
-
 __device__ inline int wave_excl_scan_i(int x, int* total) {
     const int lane = __lane_id();
     int inc = x;

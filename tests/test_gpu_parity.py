@@ -41,6 +41,38 @@ def test_edge_cut(case, k):
         assert g.edge_cut(part) == O.edge_cut(x, a, aw, part)
 
 
+def test_edge_cut_hubs_and_batches():
+    # hubs of more than 1,024 entries (summed by a whole wave), tiles whose
+    # entries span several staged batches, isolated vertices, large weights
+    # (the i32 wrap of the reference's sum)
+    rng = np.random.default_rng(3)
+    n = 9000
+    edges = set()
+    for hub, cnt in ((0, 3000), (4500, 1500), (8999, 1100)):
+        for u in rng.choice(n, cnt, replace=False):
+            if u != hub:
+                edges.add((min(hub, u), max(hub, u)))
+    for v in range(100, 4000):                       # dense rows: several batches per tile
+        for d in (1, 2, 3, 5, 8, 13, 21, 34, 55):
+            if v + d < 4400:
+                edges.add((v, v + d))
+    src = np.array([e[0] for e in edges] + [e[1] for e in edges], np.int64)
+    dst = np.array([e[1] for e in edges] + [e[0] for e in edges], np.int64)
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    x = np.zeros(n + 1, np.int32)
+    np.add.at(x, src + 1, 1)
+    x = np.cumsum(x).astype(np.int32)
+    a = dst.astype(np.int32)
+    lo, hi = np.minimum(src, dst), np.maximum(src, dst)
+    _, inv = np.unique(lo * n + hi, return_inverse=True)
+    aw = rng.integers(1, 1 << 28, inv.max() + 1).astype(np.int32)[inv]
+    for k in (2, 37):
+        part = rng.integers(0, k, n).astype(np.int32)
+        assert _graph(x, a, adjwgt=aw).edge_cut(part) == O.edge_cut(x, a, aw, part)
+        assert _graph(x, a).edge_cut(part) == O.edge_cut(x, a, None, part)
+
+
 def test_edge_cut_reference_values():
     x, a = cycle4()
     g = _graph(x, a)
