@@ -192,7 +192,8 @@ struct CoarsenBufs {
 
 constexpr int kContractBig = 64;   // coarse vertices of larger degree sum use the global-hash path
 
-// Deterministic seeded handshake matching (ctype 1 heavy edge, 0 random).
+// Deterministic seeded matching (ctype 1 heavy edge, 0 random): the greedy
+// matching in edge-key order, computed by the suitor algorithm.
 // The graphs of a batched level, laid out one after another in one CSR
 // (null for a single graph).
 struct GraphSegs {

@@ -15,7 +15,7 @@ enum ProfTag {
     kProfGain = 0,      // gain launches over every tile (boundary, gain tables, candidates)
     kProfCompact = 1,   // tile-count scan + ordered compaction
     kProfAdmit = 2,     // ordered admission (one workgroup)
-    kProfMatch = 3,     // matching rounds
+    kProfMatch = 3,     // matching (suitor init, chains, pairing)
     kProfContract = 4,  // coarse numbering + contraction
     kProfOther = 5,     // part weights, projection, edge cut
     kProfGainInc = 6,   // incremental gain launches (only tiles near the last pass's moves)

@@ -6,7 +6,7 @@
 // updated after every recorded move; the moves are applied after the scan.
 // The GPU reproduces that bit-exactly in two launches:
 //
-//  1. k_gain_tile (256-vertex tiles) / k_gain_heavy (a block per vertex of
+//  1. k_gain_tile (64-vertex tiles, a wave each) / k_gain_heavy (a block per vertex of
 //     degree > kGainHeavy): boundary test and the neighbour-part gain table
 //     ed[p] = sum of edge weights to part p (compute_move_gain,
 //     greedy.rs:197-218, gives gain(v,pv,p) = ed[p] - ed[pv]).  Every vertex
@@ -15,7 +15,7 @@
 //     in which the reference's strict `gain > best_gain` scan over p = 0..k-1
 //     picks the first feasible part.  Parts with ed == 0 (all non-adjacent
 //     parts) share one gain value and are kept implicit ("zero group").
-//     Records are compacted per 256-vertex tile in vertex order.
+//     Records are compacted per tile in vertex order.
 //  2. admission (admit.hip): one workgroup replays the ordered scan.
 #include <climits>
 #include <vector>
