@@ -396,6 +396,64 @@ def test_gpu_matching_properties():
     assert not np.array_equal(m1, m3)                  # seeded
 
 
+_M32 = 0xFFFFFFFF
+
+
+def _mix32(x):
+    x &= _M32
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & _M32
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & _M32
+    x ^= x >> 16
+    return x
+
+
+def _greedy_matching(x, a, aw, seed, shem):
+    """The matching the GPU computes, by its definition: the sequential greedy
+    matching over the edges in descending order of (key, then the smaller and
+    larger endpoint ascending), key = (2^32-1 minus the smaller endpoint
+    degree, edge weight, seeded lowbias32 hash) for heavy-edge matching and the
+    hash alone for random matching."""
+    n = len(x) - 1
+    deg = np.diff(x)
+    edges = []
+    for v in range(n):
+        for j in range(x[v], x[v + 1]):
+            u = int(a[j])
+            if v < u:
+                w = int(aw[j]) if aw is not None else 1
+                h = _mix32(v ^ _mix32(u ^ (seed & _M32)) ^ ((seed >> 32) & _M32))
+                if shem:
+                    hi = ((_M32 - min(int(deg[v]), int(deg[u]))) << 32) | ((w & _M32) ^ 0x80000000)
+                else:
+                    hi = 0
+                edges.append((-hi, -h, v, u))
+    edges.sort()
+    m = np.arange(n, dtype=np.int32)
+    free = np.ones(n, bool)
+    for _, _, v, u in edges:
+        if free[v] and free[u]:
+            m[v], m[u] = u, v
+            free[v] = free[u] = False
+    return m
+
+
+@pytest.mark.parametrize("shem", [1, 0])
+def test_gpu_matching_is_the_greedy_matching(shem):
+    from rmetis_amd import synth
+    cases = [synth.rgg(3000, 11), synth.grid(40, 37)]
+    x2, a2, aw2 = random_graph(1500, 0.01, 4, wmax=3)           # many equal weights
+    for s in cases:
+        g = R.Graph(s.n, 1, s.xadj, s.adjncy, s.vwgt, s.adjwgt)
+        for seed in (1, 0x1234567890ABCDEF):
+            ctype = R.CType.Shem if shem else R.CType.Rm
+            assert np.array_equal(R.gpu_matching(g, seed, ctype), _greedy_matching(s.xadj, s.adjncy, s.adjwgt, seed, shem))
+    g2 = _graph(x2, a2, adjwgt=aw2)
+    ctype = R.CType.Shem if shem else R.CType.Rm
+    assert np.array_equal(R.gpu_matching(g2, 7, ctype), _greedy_matching(x2, a2, aw2, 7, shem))
+
+
 def test_kway_reference_cases():
     # tests/correctness.rs:119-128 (greedy, ncuts 5, balanced)
     x, a = grid_sorted(4, 4)
