@@ -243,8 +243,13 @@ int32_t rmetis_hip_device_synchronize(void);
  * the previous pass's moves; bytes not modelled).
  * enable(on) clears the counters; read returns launches, total ms and the
  * algorithmic HBM bytes of the timed launches (0 where not modelled), over
- * all levels (size < 0) or the levels with exactly `size` vertices. */
+ * all levels (size < 0) or the levels with exactly `size` vertices.
+ * filter(tag, size) restricts the bracketing to launches of one tag (and,
+ * with size >= 0, to levels of that many vertices), so that the events of
+ * the other launches stay out of the stream; tag < 0 brackets every launch
+ * (the default).  Each bracket adds two event records to the stream. */
 void rmetis_hip_profile_enable(int32_t on);
+void rmetis_hip_profile_filter(int32_t tag, int64_t size);
 int32_t rmetis_hip_profile_read(int32_t tag, int64_t size, int64_t* launches, double* ms, double* bytes);
 
 /* Message of the last failed call on this thread ("" if none). */

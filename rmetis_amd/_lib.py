@@ -67,6 +67,7 @@ SIGNATURES = {
     "rmetis_hip_memcpy": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]),
     "rmetis_hip_device_synchronize": (C.c_int32, []),
     "rmetis_hip_profile_enable": (None, [C.c_int32]),
+    "rmetis_hip_profile_filter": (None, [C.c_int32, C.c_int64]),
     "rmetis_hip_profile_read": (C.c_int32, [C.c_int32, C.c_int64, C.POINTER(C.c_int64),
                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "rmetis_hip_last_error": (C.c_char_p, []),

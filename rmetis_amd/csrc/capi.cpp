@@ -685,6 +685,8 @@ int32_t rmetis_hip_device_synchronize(void) {
 
 void rmetis_hip_profile_enable(int32_t on) { Profiler::get().enable(on != 0); }
 
+void rmetis_hip_profile_filter(int32_t tag, int64_t size) { Profiler::get().filter(tag, (long long)size); }
+
 int32_t rmetis_hip_profile_read(int32_t tag, int64_t size, int64_t* launches, double* ms, double* bytes) {
     if (!launches || !ms || !bytes || tag < 0 || tag >= kProfTags) return METIS_ERROR_INPUT;
     long long l = 0;

@@ -29,6 +29,16 @@ public:
         return p;
     }
     bool on() const { return on_; }
+    // Records only launches of tag (and, with size >= 0, on levels of that
+    // many vertices); tag < 0 records every launch.
+    void filter(int tag, long long size) {
+        std::lock_guard<std::mutex> lk(mu_);
+        ftag_ = tag;
+        fsize_ = size;
+    }
+    bool wants(int tag, long long size) const {
+        return on_ && (ftag_ < 0 || (tag == ftag_ && (fsize_ < 0 || size == fsize_)));
+    }
     void enable(bool v) {
         std::lock_guard<std::mutex> lk(mu_);
         harvest_locked();
@@ -100,6 +110,8 @@ private:
     }
     std::mutex mu_;
     bool on_ = false;
+    int ftag_ = -1;
+    long long fsize_ = -1;
     std::vector<Rec> pending_;
     std::vector<hipEvent_t> free_;
     std::map<std::pair<int, long long>, Acc> acc_;
@@ -113,7 +125,7 @@ struct ProfScope {
     hipStream_t s;
     hipEvent_t e0 = nullptr;
     ProfScope(int t, double b, long long sz, hipStream_t st) : tag(t), bytes(b), size(sz), s(st) {
-        if (Profiler::get().on()) e0 = Profiler::get().start(s);
+        if (Profiler::get().wants(t, sz)) e0 = Profiler::get().start(s);
     }
     ~ProfScope() {
         if (e0) Profiler::get().stop(tag, e0, bytes, size, s);
