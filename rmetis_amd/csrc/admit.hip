@@ -669,7 +669,9 @@ RK_INL void admit_body(const AdmitArgs& a, const int mode) {
             }
         }
     }
-    __threadfence();
+    // tile_list, mov and gscratch are read back only by this workgroup: the
+    // barrier's workgroup-scope release/acquire orders them (an agent-scope
+    // fence would write back and invalidate the whole L2 of the XCD)
     __syncthreads();
     int32_t* ids = uni;             // window of listed tiles
     int32_t* offs = uni + kCntWin;  // their record counts, then offsets
@@ -707,7 +709,6 @@ RK_INL void admit_body(const AdmitArgs& a, const int mode) {
         nmov += wtot;
         __syncthreads();
     }
-    __threadfence();
     __syncthreads();
     // 2. windows of kWin movers: stage in LDS, walk with wave 0, then all
     //    threads write the window's moves and new parts in vertex order
@@ -792,7 +793,6 @@ RK_INL void admit_body(const AdmitArgs& a, const int mode) {
         nout += tot;
         __syncthreads();
     }
-    if constexpr (!L) __threadfence();
     __syncthreads();
     for (int i = tid; i < kc; i += kAT) a.pw[i] = fm[i] - slack[i];
     if (tid == 0) {

@@ -366,6 +366,33 @@ def test_build_coarse_graph(case, ncon, weighted):
     _coarse_equal(x, a, vw, aw, vs, ncon, match)
 
 
+@pytest.mark.parametrize("rows,cols", [(128, 129), (256, 256), (257, 256)])
+def test_build_coarse_graph_scan_sizes(rows, cols):
+    # the contraction's prefix sums: one workgroup in several rounds up to
+    # 65,536 vertices (16,512 and 65,536 here), three launches above (65,792)
+    x, a = grid_sorted(rows, cols)
+    n = len(x) - 1
+    rng = np.random.default_rng(n)
+    vw = rng.integers(1, 6, n).astype(np.int32)
+    aw = rng.integers(1, 9, len(a)).astype(np.int32)
+    aw = _symmetric_weights(x, a, aw)
+    match = O.shem_matching(x, a, aw, O.rng(n))
+    _coarse_equal(x, a, vw, aw, None, 1, match)
+
+
+def _symmetric_weights(x, a, aw):
+    """Edge weights with w(u,v) == w(v,u): the weight of the smaller-id end's entry."""
+    n = len(x) - 1
+    src = np.repeat(np.arange(n), np.diff(x))
+    key = {}
+    for i in range(len(a)):
+        u, v = int(src[i]), int(a[i])
+        if u < v:
+            key[(u, v)] = int(aw[i])
+    return np.array([key[(min(int(src[i]), int(a[i])), max(int(src[i]), int(a[i])))] for i in range(len(a))],
+                    np.int32)
+
+
 def test_build_coarse_graph_hub():
     # degree sums far above the per-group LDS capacity -> global-hash path
     n = 3000
