@@ -51,13 +51,18 @@ struct EdgeKey {
     __device__ bool eq(const EdgeKey& o) const { return hi == o.hi && lo == o.lo; }
 };
 
-__device__ inline EdgeKey edge_key(int32_t v, int32_t u, int32_t w, int32_t dv, int32_t du, uint64_t seed,
-                                   int shem) {
+// Key of edge (v, u) of weight w whose endpoints' smaller degree is mdeg.
+__device__ inline EdgeKey edge_key_m(int32_t v, int32_t u, int32_t w, int32_t mdeg, uint64_t seed, int shem) {
     const uint32_t lo = (uint32_t)min(v, u), hi = (uint32_t)max(v, u);
     const uint32_t h = mix32(lo ^ mix32(hi ^ (uint32_t)seed) ^ (uint32_t)(seed >> 32));
     if (!shem) return EdgeKey{0ull, h};
-    const uint32_t dpri = 0xFFFFFFFFu - (uint32_t)min(dv, du);
+    const uint32_t dpri = 0xFFFFFFFFu - (uint32_t)mdeg;
     return EdgeKey{((uint64_t)dpri << 32) | (uint64_t)((uint32_t)w ^ 0x80000000u), h};
+}
+
+__device__ inline EdgeKey edge_key(int32_t v, int32_t u, int32_t w, int32_t dv, int32_t du, uint64_t seed,
+                                   int shem) {
+    return edge_key_m(v, u, w, min(dv, du), seed, shem);
 }
 
 // Strict total order on edges: the key, then the endpoint ids (graph-local,
@@ -71,9 +76,35 @@ __device__ inline bool edge_before(const EdgeKey& k1, int32_t a1, int32_t b1, co
     return l1 < l2 || (l1 == l2 && max(a1, b1) < max(a2, b2));
 }
 
-// suitor[u]: (suitor id << 32) | weight of the edge (suitor, u), or
-// kNoSuitor.  A suitor's key is recomputed from these (and the degrees).
+// suitor[u]: kNoSuitor, or the suitor's id in the high word and the weight
+// w of the edge (suitor, u) in the low word.  When 0 <= w < 2^16 and the
+// smaller degree m of the two endpoints is below 2^15, the word is compact:
+// bit 31 of the high word is set (ids are below 2^31) and the low word is
+// m << 16 | w, so a reader rebuilds the suitor's key without loading the
+// suitor's degree.  The word of a pair is a function of the pair alone.
 constexpr uint64_t kNoSuitor = 0xFFFFFFFF00000000ull;
+
+__device__ inline uint64_t suitor_word(int32_t s, int32_t w, int32_t mdeg) {
+    if ((uint32_t)w < 0x10000u && (uint32_t)mdeg < 0x8000u)
+        return ((uint64_t)((uint32_t)s | 0x80000000u) << 32) | ((uint64_t)(uint32_t)mdeg << 16) | (uint32_t)w;
+    return ((uint64_t)(uint32_t)s << 32) | (uint32_t)w;
+}
+
+__device__ inline int32_t suitor_id(uint64_t word) {
+    const uint32_t hi = (uint32_t)(word >> 32);
+    return hi == 0xFFFFFFFFu ? -1 : (int32_t)(hi & 0x7FFFFFFFu);
+}
+
+// Key of the edge (s, u) recorded by word (s = suitor_id(word) >= 0; du =
+// degree of u); ids in the keys are graph-local (minus base).
+__device__ inline EdgeKey suitor_key(uint64_t word, int32_t s, int32_t u, int32_t du, const int32_t* xadj,
+                                     uint64_t seed, int shem, int32_t base) {
+    const uint32_t lo = (uint32_t)word;
+    if ((uint32_t)(word >> 32) >> 31)
+        return edge_key_m(s - base, u - base, (int32_t)(lo & 0xFFFFu), (int32_t)(lo >> 16), seed, shem);
+    const int32_t ds = shem ? xadj[s + 1] - xadj[s] : 0;
+    return edge_key(s - base, u - base, (int32_t)lo, ds, du, seed, shem);
+}
 
 struct MatchArgs {
     int32_t n, shem;
@@ -132,39 +163,36 @@ __global__ __launch_bounds__(kBlock) void k_suitor(MatchArgs a) {
             const int32_t beg = a.xadj[cur], end = a.xadj[cur + 1];
             const int32_t dc = end - beg;
             EdgeKey best{0ull, 0u};
-            int32_t bu = -1, bw = 0;
+            int32_t bu = -1, bw = 0, bd = 0;
             uint64_t bseen = 0;
             for (int32_t j = beg + gl; j < end; j += G) {
                 const int32_t u = a.adjncy[j];
                 const int32_t w = a.adjwgt ? a.adjwgt[j] : 1;
                 uint64_t sv = ld_agent64(&a.suitor[u]);
+                const int32_t du = a.xadj[u + 1] - a.xadj[u];
                 if (u == fu && sv != fword) {
                     // keep the newer of the loaded word and the CAS result
-                    const int32_t s1 = (int32_t)(sv >> 32);
+                    const int32_t s1 = suitor_id(sv);
+                    const int32_t s2 = suitor_id(fword);
                     if (s1 < 0) sv = fword;
-                    else if ((int32_t)(fword >> 32) >= 0) {
-                        const int32_t du0 = a.xadj[u + 1] - a.xadj[u];
-                        const int32_t s2 = (int32_t)(fword >> 32);
-                        const EdgeKey k1 = edge_key(s1 - base, u - base, (int32_t)(uint32_t)sv,
-                                                    a.xadj[s1 + 1] - a.xadj[s1], du0, seed, a.shem);
-                        const EdgeKey k2 = edge_key(s2 - base, u - base, (int32_t)(uint32_t)fword,
-                                                    a.xadj[s2 + 1] - a.xadj[s2], du0, seed, a.shem);
+                    else if (s2 >= 0) {
+                        const EdgeKey k1 = suitor_key(sv, s1, u, du, a.xadj, seed, a.shem, base);
+                        const EdgeKey k2 = suitor_key(fword, s2, u, du, a.xadj, seed, a.shem, base);
                         if (edge_before(k2, s2 - base, u - base, k1, s1 - base, u - base)) sv = fword;
                     }
                 }
-                const int32_t sid = (int32_t)(sv >> 32);
+                const int32_t sid = suitor_id(sv);
                 if (sid == cur) continue;
-                const int32_t du = a.xadj[u + 1] - a.xadj[u];
                 const EdgeKey k = edge_key(cur - base, u - base, w, dc, du, seed, a.shem);
                 if (sid >= 0) {
-                    const EdgeKey ks = edge_key(sid - base, u - base, (int32_t)(uint32_t)sv,
-                                                a.xadj[sid + 1] - a.xadj[sid], du, seed, a.shem);
+                    const EdgeKey ks = suitor_key(sv, sid, u, du, a.xadj, seed, a.shem, base);
                     if (!edge_before(k, cur - base, u - base, ks, sid - base, u - base)) continue;
                 }
                 if (bu < 0 || edge_before(k, cur - base, u - base, best, cur - base, bu - base)) {
                     best = k;
                     bu = u;
                     bw = w;
+                    bd = du;
                     bseen = sv;
                 }
             }
@@ -173,11 +201,13 @@ __global__ __launch_bounds__(kBlock) void k_suitor(MatchArgs a) {
                 const EdgeKey ob{__shfl_xor(best.hi, o, G), __shfl_xor(best.lo, o, G)};
                 const int32_t ou = __shfl_xor(bu, o, G);
                 const int32_t ow = __shfl_xor(bw, o, G);
+                const int32_t od = __shfl_xor(bd, o, G);
                 const uint64_t os = __shfl_xor(bseen, o, G);
                 if (ou >= 0 && (bu < 0 || edge_before(ob, cur - base, ou - base, best, cur - base, bu - base))) {
                     best = ob;
                     bu = ou;
                     bw = ow;
+                    bd = od;
                     bseen = os;
                 }
             }
@@ -187,15 +217,13 @@ __global__ __launch_bounds__(kBlock) void k_suitor(MatchArgs a) {
             // (the other neighbours can only have become harder to claim).
             uint64_t old = 0;
             if (gl == 0) {
-                const unsigned long long mine = ((unsigned long long)(uint32_t)cur << 32) | (uint32_t)bw;
-                const int32_t du = a.xadj[bu + 1] - a.xadj[bu];
+                const unsigned long long mine = suitor_word(cur, bw, min(dc, bd));
                 for (;;) {
                     old = atomicCAS((unsigned long long*)&a.suitor[bu], (unsigned long long)bseen, mine);
                     if (old == bseen) break;
-                    const int32_t sid = (int32_t)(old >> 32);
+                    const int32_t sid = suitor_id(old);
                     if (sid == cur) break;
-                    const EdgeKey ks = edge_key(sid - base, bu - base, (int32_t)(uint32_t)old,
-                                                a.xadj[sid + 1] - a.xadj[sid], du, seed, a.shem);
+                    const EdgeKey ks = suitor_key(old, sid, bu, bd, a.xadj, seed, a.shem, base);
                     if (!edge_before(best, cur - base, bu - base, ks, sid - base, bu - base)) break;
                     bseen = old;                                        // still beaten: claim against it
                 }
@@ -203,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_suitor(MatchArgs a) {
             old = __shfl(old, 0, G);
             bseen = __shfl(bseen, 0, G);
             if (old == bseen) {
-                cur = (int32_t)(bseen >> 32);                          // the displaced suitor searches again
+                cur = suitor_id(bseen);                                // the displaced suitor searches again
                 fu = -1;
             } else {
                 fu = bu;
@@ -217,8 +245,8 @@ __global__ __launch_bounds__(kBlock) void k_suitor(MatchArgs a) {
 __global__ __launch_bounds__(kBlock) void k_suitor_pairs(int32_t n, const uint64_t* __restrict__ suitor,
                                                          int32_t* __restrict__ match) {
     for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
-        const int32_t s = (int32_t)(suitor[v] >> 32);
-        match[v] = (s >= 0 && (int32_t)(suitor[s] >> 32) == (int32_t)v) ? s : (int32_t)v;
+        const int32_t s = suitor_id(suitor[v]);
+        match[v] = (s >= 0 && suitor_id(suitor[s]) == (int32_t)v) ? s : (int32_t)v;
     }
 }
 

@@ -481,6 +481,21 @@ def test_gpu_matching_is_the_greedy_matching(shem):
     assert np.array_equal(R.gpu_matching(g2, 7, ctype), _greedy_matching(x2, a2, aw2, 7, shem))
 
 
+@pytest.mark.parametrize("shem", [1, 0])
+def test_gpu_matching_wide_suitor_words(shem):
+    # suitor words hold (smaller degree, weight) only for weights below 2^16;
+    # heavier edges take the other encoding, which reloads the suitor's degree
+    ctype = R.CType.Shem if shem else R.CType.Rm
+    x, a, aw = random_graph(2000, 0.006, 31, wmax=3)
+    remap = np.array([0, 65535, 65536, 1], np.int32)            # 65535 compact, 65536 not
+    aw = remap[aw]
+    x2, a2, aw2 = random_graph(1500, 0.008, 32, wmax=200000)     # mostly above 2^16
+    for xx, aa, ww in ((x, a, aw), (x2, a2, aw2)):
+        g = _graph(xx, aa, adjwgt=ww)
+        for seed in (3, 0xFEDCBA9876543210):
+            assert np.array_equal(R.gpu_matching(g, seed, ctype), _greedy_matching(xx, aa, ww, seed, shem))
+
+
 def test_kway_reference_cases():
     # tests/correctness.rs:119-128 (greedy, ncuts 5, balanced)
     x, a = grid_sorted(4, 4)
