@@ -186,7 +186,7 @@ struct CoarsenBufs {
     int32_t* hash_sum = nullptr;
     int32_t* counters = nullptr;   // [16]: 0 cn, 1 cm, 2 nbig, 4 weight-not-one flag, 6-7 hash bump (u64), 8 nheavy
     int32_t* host_small = nullptr;   // pinned host scratch for the per-level counter read
-    uint64_t* suitor = nullptr;      // [n] matching: (suitor id << 32) | edge weight
+    uint64_t* suitor = nullptr;      // [n] matching: suitor id and the pair's key inputs (suitor_word)
     void* scan_tmp = nullptr;
 };
 
