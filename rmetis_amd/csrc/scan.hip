@@ -2,8 +2,11 @@
 //
 // Used for the contraction's coarse numbering (ascending-leader order of
 // graph/mod.rs:288-297), coarse CSR offsets and the ordered compaction of
-// greedy move candidates.  Three launches: per-chunk sums, a single-block scan
-// of the chunk sums, and a per-chunk rescan that adds the chunk offset.
+// greedy move candidates.  Up to kScanOneMax elements one workgroup of 1024
+// threads scans the input in one launch (the coarser levels are small and
+// launch-latency bound).  Larger
+// inputs take three launches: per-chunk sums, a single-block scan of the
+// chunk sums, and a per-chunk rescan that adds the chunk offset.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -14,6 +17,13 @@ namespace {
 constexpr int kScanItems = 8;
 constexpr int kScanChunk = kBlock * kScanItems;   // 2048 elements per block
 
+constexpr int kOneThreads = 1024;
+constexpr int kOneRows = 16;
+constexpr int64_t kScanOneMax = (int64_t)kOneThreads * kOneRows;   // 16,384
+
+// Exclusive scan over the NT threads of the block; *total = the block's sum.
+// Both barriers are inside, so consecutive calls may reuse s_w.
+template <int NT = kBlock>
 __device__ inline long long block_excl_scan_ll(long long x, long long* s_w, long long* total) {
     const int lane = __lane_id();
     const int wave = threadIdx.x >> 6;
@@ -22,7 +32,7 @@ __device__ inline long long block_excl_scan_ll(long long x, long long* s_w, long
     __syncthreads();
     long long off = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
+    for (int w = 0; w < NT / 64; ++w) {
         long long c = s_w[w];
         if (w < wave) off += c;
         tot += c;
@@ -85,6 +95,53 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const int32_t* __restrict_
     }
 }
 
+// n <= kScanOneMax, one block of kOneThreads, one round.  The input is read
+// as rows of kOneThreads elements (thread t loads element j*kOneThreads + t
+// of every row j, so each load is coalesced); each wave scans its 64-element
+// piece of every row, and one block scan over the pieces' totals in (row,
+// wave) order -- which is element order -- gives each piece's offset.  Sums
+// are taken modulo 2^32: the outputs are int32 truncations of the exact
+// prefix sums, so the low 32 bits are all that matter.  Every load precedes
+// every store, so in == out is allowed.
+__global__ __launch_bounds__(kOneThreads) void k_scan_one(const int32_t* in, int n, int32_t* out,
+                                                          int32_t* total_out) {
+    __shared__ uint32_t s_seg[kOneThreads];   // piece (row j, wave w) at j * 16 + w
+    __shared__ long long s_w[kOneThreads / 64];
+    const int t = threadIdx.x, lane = __lane_id(), wave = t >> 6;
+    const int rows = (n + kOneThreads - 1) / kOneThreads;   // <= kOneRows
+    uint32_t x[kOneRows];
+#pragma unroll
+    for (int j = 0; j < kOneRows; ++j) {
+        const int idx = j * kOneThreads + t;
+        x[j] = j < rows && idx < n ? (uint32_t)in[idx] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kOneRows; ++j) {
+        if (j < rows) {
+            uint32_t inc = x[j];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += y;
+            }
+            if (lane == 63) s_seg[j * (kOneThreads / 64) + wave] = inc;
+            x[j] = inc - x[j];   // exclusive within the piece
+        }
+    }
+    __syncthreads();
+    const uint32_t piece = t < rows * (kOneThreads / 64) ? s_seg[t] : 0u;
+    long long tot;
+    const long long off = block_excl_scan_ll<kOneThreads>((long long)piece, s_w, &tot);
+    s_seg[t] = (uint32_t)off;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kOneRows; ++j) {
+        const int idx = j * kOneThreads + t;
+        if (j < rows && idx < n) out[idx] = (int32_t)(s_seg[j * (kOneThreads / 64) + wave] + x[j]);
+    }
+    if (t == 0 && total_out) *total_out = (int32_t)tot;
+}
+
 }  // namespace
 
 size_t scan_temp_bytes(int64_t n) {
@@ -96,6 +153,11 @@ void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* tot
                         hipStream_t s) {
     if (n <= 0) {
         if (total_dev) RK_HIP(hipMemsetAsync(total_dev, 0, sizeof(int32_t), s));
+        return;
+    }
+    if (n <= kScanOneMax) {
+        hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kOneThreads), 0, s, in, (int)n, out, total_dev);
+        RK_HIP(hipGetLastError());
         return;
     }
     int nchunks = (int)((n + kScanChunk - 1) / kScanChunk);

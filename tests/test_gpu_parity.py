@@ -366,10 +366,11 @@ def test_build_coarse_graph(case, ncon, weighted):
     _coarse_equal(x, a, vw, aw, vs, ncon, match)
 
 
-@pytest.mark.parametrize("rows,cols", [(128, 129), (256, 256), (257, 256)])
+@pytest.mark.parametrize("rows,cols", [(64, 63), (128, 128), (128, 129), (257, 256)])
 def test_build_coarse_graph_scan_sizes(rows, cols):
-    # the contraction's prefix sums: one workgroup in several rounds up to
-    # 65,536 vertices (16,512 and 65,536 here), three launches above (65,792)
+    # the contraction's prefix sums: one workgroup in one launch up to 16,384
+    # elements (4,032 and 16,384 vertices here), three launches above (16,512
+    # and 65,792)
     x, a = grid_sorted(rows, cols)
     n = len(x) - 1
     rng = np.random.default_rng(n)
