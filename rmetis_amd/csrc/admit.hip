@@ -399,6 +399,7 @@ RK_INL void resolve_chunk(const Ctx& x, S& st, int s0, int cnt, int first = 0) {
 }
 
 constexpr int kSpecRounds = 8;   // speculative rounds per chunk; the walk finishes the rest
+constexpr int kSpecMinPerPart = 24;   // vertices per part below which a level is not speculated
 constexpr int kSpecMinRun = 4;   // shortest run of slots between barriers worth a speculative call
 
 // Net weight moved into a part by the lanes of masks mi (in) and mo (out):
@@ -745,8 +746,11 @@ RK_INL void admit_body(const AdmitArgs& a, const int mode) {
         }
         __syncthreads();
         // a speculative call that runs out of rounds (many moves conflict)
-        // turns speculation off for the rest of the window
-        bool spec_on = mode == 1;
+        // turns speculation off for the rest of the window.  Levels with
+        // fewer than kSpecMinPerPart vertices per part (the coarsest, whose
+        // vertices weigh a large share of a part's slack) conflict so often
+        // that the walk alone is faster there; both are exact.
+        bool spec_on = mode == 1 && (int64_t)a.ntiles * 64 >= (int64_t)kSpecMinPerPart * a.k;
         if ((tid >> 6) == 0)
             for (int s0 = 0; s0 < wn; s0 += 64) {
                 const int cnt = min(64, wn - s0);
