@@ -165,7 +165,7 @@ std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffer
 
 void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* d_fmax, const long long* d_omax,
                   int niter, int32_t* d_part, int nheavy, const int32_t* heavy_list, Buffers& B, LevelStat* st,
-                  bool cleared) {
+                  bool cleared, bool pw_valid) {
     // No host synchronisation unless statistics are requested: the
     // thresholds already live on the device and the pass count is fixed
     // (a cut pass that moves nothing turns the remaining passes into no-ops).
@@ -179,7 +179,7 @@ void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* d_
         RK_HIP(hipEventRecord(e0, s));
         RK_HIP(hipMemsetAsync(B.rb.passes, 0, sizeof(int32_t), s));
     }
-    part_weights(g, d_part, k, B.rb.pw, s, cleared);                                 // greedy.rs:50
+    if (!pw_valid) part_weights(g, d_part, k, B.rb.pw, s, cleared);                  // greedy.rs:50
     if (!cleared) RK_HIP(hipMemsetAsync(B.rb.done, 0, sizeof(int32_t), s));
     refine_pass(g, k, 0, nheavy, heavy_list, B.rb, d_part, s);                       // greedy.rs:60-123
     // cut rounds (126-190); on large levels, after the first, only the tiles
@@ -404,14 +404,21 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
             refine_level(cx, cg, k, d_thr + 2 * kc * li, d_thr + 2 * kc * li + kc, o.niter, pc,
                          levels.back().nheavy, levels.back().heavy, B, st ? &cstat : nullptr);
         }
+        // With explicit vertex weights a coarse vertex weighs the sum of its
+        // fine vertices (graph/mod.rs:284-384), so the projected partition has
+        // exactly the part weights the coarser level's last pass left in
+        // B.rb.pw (int32 sums wrap alike in any order): the table is carried
+        // over instead of recomputed.  Unit weights (vwgt absent) count
+        // vertices, which differ from level to level.
+        const bool keep_pw = g.vwgt != nullptr;
         for (int64_t i = (int64_t)levels.size() - 1; i >= 0; --i) {     // kway.rs:76-92
             const DevGraph& fine = i == 0 ? g : levels[i - 1].g;
             const int32_t fheavy = i == 0 ? fin_nheavy : levels[i - 1].nheavy;
             const int32_t* flist = i == 0 ? fin_heavy : levels[i - 1].heavy;
             int32_t* pf = cx.arena.get<int32_t>(fine.n);
-            project(fine.n, levels[i].cmap, pc, pf, s, B.rb.pw, kc, B.rb.done, 1);
+            project(fine.n, levels[i].cmap, pc, pf, s, keep_pw ? nullptr : B.rb.pw, kc, B.rb.done, 1);
             refine_level(cx, fine, k, d_thr + 2 * kc * i, d_thr + 2 * kc * i + kc, o.niter, pf, fheavy, flist, B,
-                         st ? &lst[i] : nullptr, true);
+                         st ? &lst[i] : nullptr, true, keep_pw);
             pc = pf;
         }
         edge_cut(g, pc, B.cut_acc, B.cut_out, pick_group(g.n, g.m), s);    // kway.rs:94

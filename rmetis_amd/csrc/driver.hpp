@@ -78,10 +78,13 @@ Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon
 std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffers& B);
 
 // One level's greedy refinement (greedy.rs:36-193) on the device partition.
-// cleared: B.rb.pw and B.rb.done were already zeroed (by the projection)
+// cleared: B.rb.pw and B.rb.done were already zeroed (by the projection).
+// pw_valid: B.rb.pw already holds partition_weights(d_part) (the coarser
+// level's final table carried over a projection with explicit vertex
+// weights), so it is not recomputed.
 void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* d_fmax, const long long* d_omax,
                   int niter, int32_t* d_part, int nheavy, const int32_t* heavy_list, Buffers& B, LevelStat* st,
-                  bool cleared = false);
+                  bool cleared = false, bool pw_valid = false);
 
 // Full k-way partition of a device-resident graph (partition/kway.rs:20-141);
 // writes d_part and returns objval.
