@@ -227,6 +227,30 @@ int32_t rmetis_hip_validate_csr(idx_t nvtxs, idx_t ncon, const idx_t* xadj, int6
  * dbglvl. */
 int32_t rmetis_hip_parse_options(const idx_t* options, int64_t* out);
 
+/* grow_partition, src/initial/grow.rs:12-103, with the attempt RNG
+ * SmallRng::seed_from_u64(seed) (src/partition/kway.rs:62): the BFS growing
+ * the k-way path runs on its coarsest level.  tpwgts[nparts*ncon] are the
+ * f64 target weights (already scaled to the total, as kway.rs:67-69 passes
+ * them).  Host only; returns METIS_OK or METIS_ERROR_INPUT (also for
+ * nparts > nvtxs, where the reference's seed loop would not end). */
+int32_t rmetis_hip_grow_partition(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                                  const idx_t* vwgt, idx_t nparts, const double* tpwgts, uint64_t seed,
+                                  idx_t* part);
+
+/* The integer thresholds the greedy kernels compare with: for target
+ * t = (tpwgts[p*ncon+c] as f64, or 1/nparts) * total[c] (Graph::
+ * compute_target_weights, src/graph/mod.rs:392-410) and ub = ubvec[c] as f64
+ * or 1 + ufactor/1000 (src/partition/kway.rs:44-49), fmax = floor(t*ub) with
+ * w <= fmax <=> (w as f64 <= t*ub) and omax with w > omax <=> (w as f64 >
+ * t*ub) (src/refine/greedy.rs:70,87,144).  Host only. */
+int32_t rmetis_hip_level_thresholds(idx_t nparts, idx_t ncon, const real_t* tpwgts, const idx_t* total,
+                                    const real_t* ubvec, idx_t ufactor, int64_t* fmax, int64_t* omax);
+
+/* Test toggles (no counterpart in the reference): bit 0 makes the k-way path
+ * recompute the part-weight table after each projection instead of carrying
+ * the coarser level's table over; results must not change. */
+void rmetis_hip_debug_flags(int32_t flags);
+
 /* Device-memory plumbing for callers that hold no other HIP runtime handle
  * (no counterpart in the reference): plain hipMalloc/hipFree/hipMemcpy on the
  * current device; kind 1 = host->device, 2 = device->host, 3 = device->device. */

@@ -18,11 +18,25 @@
  *                            contractions, weight-conservation invariants.
  *   matchings             -- symmetry m[m[v]] == v (shem.rs:96-99, rm.rs:56-64),
  *                            maximality, heavy-edge preference on the shem.rs star.
- *   RNG (SmallRng)        -- published SplitMix64 and xoshiro256++ vectors.
- *   grow / part_graph_kway end to end -- parity unpinned: the exact values depend
- *                            on rand 0.8.5's streams, which no reference test
- *                            fixes; pinned only by the pipeline invariants of
- *                            tests/correctness.rs and tests/parallel.rs.
+ *   RNG (SmallRng)        -- published SplitMix64 and xoshiro256++ vectors;
+ *                            gen_range by its closed form for power-of-two
+ *                            ranges (rand 0.8.5's zone (range << lz) - 1 is
+ *                            2^63 - 1 there: a draw is kept iff bit 63-j is
+ *                            clear, the result is its top j bits) and by exact
+ *                            rational arithmetic for range 3.
+ *   grow_partition        -- hand-derived results: k = n gives the seed draw
+ *                            order; a path grown from both ends splits at
+ *                            ceil(n/2); a popped vertex claims every free
+ *                            neighbour (star); leftovers go, in ascending order,
+ *                            to the first part lightest on constraint 0.
+ *   part_graph_kway       -- an independent Python transcription of the driver
+ *                            (kway.rs:20-141) over the pieces pinned above:
+ *                            RNG consumption order, f32 stop rule, projection,
+ *                            targets per level, strictly-better attempt.  The
+ *                            Rust crate itself cannot be run here (no cargo,
+ *                            rand 0.8.5 not vendored), so the composed values
+ *                            rest on the restated rand algorithms, not on
+ *                            outputs of the crate.
  */
 #include "oracle.h"
 

@@ -260,20 +260,24 @@ int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, cons
     return guarded([&]() -> int32_t {
         if (!gvtx || !xadj || !adjncy || !nparts || !objval || !part) return METIS_ERROR_INPUT;
         if (ngraphs <= 0 || ncon <= 0) return METIS_ERROR_INPUT;
-        // graph/validate.rs:16-142 per graph: every row stays inside its graph
-        if (gvtx[0] != 0) return METIS_ERROR_INPUT;
+        // graph/validate.rs:16-142 per graph: offsets first (xadj[0] == 0 and
+        // monotone over the whole union, so that no row loop reads outside
+        // adjncy[0..xadj[n])), then every row stays inside its graph
+        if (gvtx[0] != 0 || xadj[0] != 0) return METIS_ERROR_INPUT;
+        for (int32_t i = 0; i < ngraphs; ++i)
+            if (gvtx[i + 1] <= gvtx[i]) return METIS_ERROR_INPUT;
+        for (int32_t v = 0; v < gvtx[ngraphs]; ++v)
+            if (xadj[v] > xadj[v + 1]) return METIS_ERROR_INPUT;
         for (int32_t i = 0; i < ngraphs; ++i) {
             const int32_t v0 = gvtx[i], v1 = gvtx[i + 1];
             if (v1 <= v0) return METIS_ERROR_INPUT;
             if (nparts[i] < 2 || nparts[i] > v1 - v0) return METIS_ERROR_INPUT;
             for (int32_t v = v0; v < v1; ++v) {
-                if (xadj[v] > xadj[v + 1]) return METIS_ERROR_INPUT;
                 for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j)
                     if (adjncy[j] < v0 || adjncy[j] >= v1 || adjncy[j] == v) return METIS_ERROR_INPUT;
             }
         }
         const int32_t n = gvtx[ngraphs];
-        if (xadj[0] != 0) return METIS_ERROR_INPUT;
         std::vector<int32_t> cnt(n, 0);
         for (int64_t j = 0; j < xadj[n]; ++j) cnt[adjncy[j]]++;
         for (int32_t v = 0; v < n; ++v)
@@ -659,6 +663,39 @@ int32_t rmetis_hip_parse_options(const idx_t* options, int64_t* out) {
     for (int i = 0; i < 13; ++i) out[i] = v[i];
     return METIS_OK;
 }
+
+int32_t rmetis_hip_grow_partition(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                                  const idx_t* vwgt, idx_t nparts, const double* tpwgts, uint64_t seed,
+                                  idx_t* part) {
+    // initial/grow.rs:12-103 with SmallRng::seed_from_u64(seed) (kway.rs:62)
+    return guarded([&]() -> int32_t {
+        if (!xadj || !adjncy || !tpwgts || !part || nvtxs <= 0 || ncon <= 0 || nparts < 1) return METIS_ERROR_INPUT;
+        if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt)) return METIS_ERROR_INPUT;
+        if (nparts > nvtxs) return METIS_ERROR_INPUT;   // the reference's seed loop would not end
+        Rng rng(seed);
+        if (!grow_host(nvtxs, ncon, xadj, adjncy, vwgt, nparts, tpwgts, rng, part)) return METIS_ERROR_INPUT;
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_level_thresholds(idx_t nparts, idx_t ncon, const real_t* tpwgts, const idx_t* total,
+                                    const real_t* ubvec, idx_t ufactor, int64_t* fmax, int64_t* omax) {
+    // graph/mod.rs:392-410 targets, kway.rs:44-49 imbalance, greedy.rs:70,87,144 compares
+    return guarded([&]() -> int32_t {
+        if (!total || !fmax || !omax || nparts < 1 || ncon < 1) return METIS_ERROR_INPUT;
+        std::vector<double> ub(ncon);
+        for (int32_t c = 0; c < ncon; ++c) ub[c] = ubvec ? (double)ubvec[c] : 1.0 + (double)ufactor / 1000.0;
+        std::vector<long long> f((size_t)nparts * ncon), o((size_t)nparts * ncon);
+        level_thresholds(nparts, ncon, tpwgts, total, ub.data(), f.data(), o.data());
+        for (size_t i = 0; i < f.size(); ++i) {
+            fmax[i] = f[i];
+            omax[i] = o[i];
+        }
+        return METIS_OK;
+    });
+}
+
+void rmetis_hip_debug_flags(int32_t flags) { set_debug_flags(flags); }
 
 void* rmetis_hip_device_alloc(int64_t bytes) {
     void* p = nullptr;

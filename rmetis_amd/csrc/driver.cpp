@@ -10,6 +10,7 @@
 #include "profiler.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -54,6 +55,13 @@ void default_options_raw(int32_t* arr) {
     arr[17] = 0;   // NUMBERING
     arr[5] = 0;    // DBGLVL
 }
+
+namespace {
+std::atomic<int32_t> g_debug_flags{0};
+}  // namespace
+
+void set_debug_flags(int32_t flags) { g_debug_flags.store(flags, std::memory_order_relaxed); }
+int32_t debug_flags() { return g_debug_flags.load(std::memory_order_relaxed); }
 
 namespace {
 
@@ -410,7 +418,7 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
         // B.rb.pw (int32 sums wrap alike in any order): the table is carried
         // over instead of recomputed.  Unit weights (vwgt absent) count
         // vertices, which differ from level to level.
-        const bool keep_pw = g.vwgt != nullptr;
+        const bool keep_pw = g.vwgt != nullptr && !(debug_flags() & kDebugNoPwCarry);
         for (int64_t i = (int64_t)levels.size() - 1; i >= 0; --i) {     // kway.rs:76-92
             const DevGraph& fine = i == 0 ? g : levels[i - 1].g;
             const int32_t fheavy = i == 0 ? fin_nheavy : levels[i - 1].nheavy;

@@ -102,6 +102,13 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t ngraphs, const in
                        const float* ubvec, const Options& o, const uint64_t* seeds, int32_t* d_part, int32_t* objval,
                        KwayStats* st);
 
+// Test toggles (no counterpart in the reference): bit 0 recomputes the
+// part-weight table after every projection instead of carrying the coarser
+// level's table over (the two must give identical partitions).
+constexpr int32_t kDebugNoPwCarry = 1;
+void set_debug_flags(int32_t flags);
+int32_t debug_flags();
+
 // initial/grow.rs:12-103 on the host; false when n < k.
 bool grow_host(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
                int32_t k, const double* tpw, Rng& rng, int32_t* part);

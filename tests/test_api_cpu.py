@@ -3,6 +3,7 @@ include/rmetis.h declares, and its host-side boundary logic (CSR validation,
 option parsing, argument checks, error codes) follows src/ffi/c_api.rs and
 src/graph/validate.rs.  No compute call reaches a GPU here."""
 import ctypes as C
+import math
 import os
 import re
 
@@ -12,6 +13,7 @@ import pytest
 import rmetis_amd as R
 from rmetis_amd import _lib, synth
 from graphs import cycle4
+import pyoracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -208,3 +210,80 @@ def test_server_without_gpu_fails_loudly():
         R.Server(slots=2)
     assert _lib.lib().rmetis_hip_server_create(0, 0, 0) is None      # slots < 1
     assert _lib.lib().rmetis_hip_server_create(4, 0, 0) is None      # slots > 3
+
+
+def test_product_grow_equals_oracle():
+    # rmetis_hip_grow_partition is the k-way path's BFS growing of its
+    # coarsest level (initial/grow.rs:12-103, host); bit-exact against the
+    # pinned oracle for the same attempt seed, including disconnected graphs,
+    # two constraints and k = n
+    from graphs import random_graph
+    L = _lib.lib()
+    rng = np.random.default_rng(3)
+    cases = 0
+    for trial in range(40):
+        n = int(rng.integers(4, 300))
+        x, a, aw = random_graph(n, float(rng.choice([0.005, 0.02, 0.1])), trial, wmax=3)
+        ncon = int(rng.integers(1, 3))
+        vw = rng.integers(1, 6, n * ncon).astype(np.int32) if trial % 3 else None
+        k = int(min(n, rng.choice([2, 3, 8, 17, n])))
+        tot = (vw.reshape(n, ncon).sum(0) if vw is not None else np.full(ncon, n)).astype(np.float64)
+        tpw = np.tile(tot / k, k)
+        seed = int(rng.integers(1, 2**40))
+        part = np.zeros(n, np.int32)
+        rc = L.rmetis_hip_grow_partition(n, ncon, x.ctypes.data_as(_lib.i32p), a.ctypes.data_as(_lib.i32p),
+                                         vw.ctypes.data_as(_lib.i32p) if vw is not None else None, k,
+                                         tpw.ctypes.data_as(_lib.f64p), seed, part.ctypes.data_as(_lib.i32p))
+        assert rc == 1
+        want = O.grow_partition(ncon, x, a, vw, k, tpw, O.rng(seed))
+        assert np.array_equal(part, want), trial
+        cases += 1
+    assert cases == 40
+    # nparts > nvtxs is an input error (the reference's seed loop never ends)
+    x, a = cycle4_arrays()
+    part = np.zeros(4, np.int32)
+    tpw = np.ones(5)
+    assert L.rmetis_hip_grow_partition(4, 1, x.ctypes.data_as(_lib.i32p), a.ctypes.data_as(_lib.i32p), None, 5,
+                                       tpw.ctypes.data_as(_lib.f64p), 1, part.ctypes.data_as(_lib.i32p)) == -2
+
+
+def cycle4_arrays():
+    return cycle4()
+
+
+def test_level_thresholds():
+    # floor(target * ub) with target = (tpwgts as f64 or 1/k) * total
+    # (graph/mod.rs:392-410) and ub = ubvec as f64 or 1 + ufactor/1000
+    # (kway.rs:44-49): w <= fmax <=> w as f64 <= target*ub (greedy.rs:87,144)
+    L = _lib.lib()
+    cases = [
+        (8, 1, None, [65536], None, 30),
+        (64, 1, None, [1048576], None, 30),
+        (3, 2, None, [1000, 999], [1.05, 1.05], 30),
+        (4, 1, [0.1, 0.2, 0.3, 0.4], [12345], None, 1),
+        (5, 1, None, [1000], [1.0], 30),          # exact integer targets
+        (2, 1, [0.3, 0.7], [10], None, 0),        # 0.7f * 10 is just below 7
+        (7, 2, None, [2**30 - 1, 3], [1.001, 1.5], 30),
+    ]
+    for k, ncon, tp, total, ubv, uf in cases:
+        tpa = None if tp is None else np.array(tp, np.float32)
+        tot = np.array(total, np.int32)
+        uba = None if ubv is None else np.array(ubv, np.float32)
+        fm = np.zeros(k * ncon, np.int64)
+        om = np.zeros(k * ncon, np.int64)
+        rc = L.rmetis_hip_level_thresholds(k, ncon, tpa.ctypes.data_as(_lib.f32p) if tpa is not None else None,
+                                           tot.ctypes.data_as(_lib.i32p),
+                                           uba.ctypes.data_as(_lib.f32p) if uba is not None else None, uf,
+                                           fm.ctypes.data_as(C.POINTER(C.c_int64)),
+                                           om.ctypes.data_as(C.POINTER(C.c_int64)))
+        assert rc == 1
+        tgt = [((float(tpa[p * ncon + c]) if tpa is not None else 1.0 / k) * float(tot[c]))
+               for p in range(k) for c in range(ncon)]
+        ub = [float(uba[c]) if uba is not None else 1.0 + uf / 1000.0 for c in range(ncon)]
+        for p in range(k):
+            for c in range(ncon):
+                t = tgt[p * ncon + c] * ub[c]
+                f = int(math.floor(t))
+                assert fm[p * ncon + c] == f and om[p * ncon + c] == f
+                # the defining property on the integers either side
+                assert float(f) <= t < float(f + 1)

@@ -38,19 +38,51 @@ def test_next_u32_is_upper_half():
         assert O.next_u32(r1) == O.next_u64(r2) >> 32
 
 
-def test_range_is_unbiased_widening_multiply():
-    # gen_range(0..n): hi 64 bits of v*n with rejection zone (n << lz) - 1
-    r1, r2 = O.rng(11), O.rng(11)
-    for n in (1, 2, 3, 10, 1000, 2**33 + 5):
-        got = O.range_incl(r1, 0, n - 1)
-        zone = ((n << (64 - n.bit_length())) - 1) & M64
+def test_range_power_of_two_closed_form():
+    # rand 0.8.5 UniformInt<usize>::sample_single_inclusive keeps a draw v
+    # iff lo(v * range) <= (range << lz) - 1.  For range = 2^j that zone is
+    # 2^63 - 1 whatever j is (the "conservative but fast approximation", not
+    # the exact 2^64 - 1 that a power of two would allow), and lo = v << j,
+    # so v is kept iff its bit 63 - j is clear, and the result is then v's top
+    # j bits.  j = 0 (gen_range(0..1)) keeps only draws below 2^63 and returns
+    # 0.  Both the result and the number of draws consumed are checked.
+    rejected = 0
+    for j in (0, 1, 3, 8, 17, 33, 62):
+        n = 1 << j
+        r1, r2 = O.rng(100 + j), O.rng(100 + j)
+        for _ in range(48):
+            got = O.range_incl(r1, 0, n - 1)
+            while True:
+                v = O.next_u64(r2)
+                if (v >> (63 - j)) & 1 == 0:
+                    break
+                rejected += 1
+            assert got == v >> (64 - j)
+        assert O.next_u64(r1) == O.next_u64(r2)      # same number of draws consumed
+    assert rejected > 100                            # about half of all draws
+
+
+def test_range_small_odd_bounds():
+    # range 3: zone (3 << 62) - 1; the result is floor(3 v / 2^64), i.e. 0, 1
+    # or 2 as v falls below 2^64/3, 2*2^64/3 or above, and a draw is rejected
+    # iff 3 v mod 2^64 >= 3 * 2^62, which for v in [0, 2^64) is the top quarter
+    # of each third: v mod ceil(2^64/3) lands in [2^62, ...).  Checked against
+    # exact rational arithmetic on the raw stream.
+    from fractions import Fraction
+    r1, r2 = O.rng(5), O.rng(5)
+    third = Fraction(1 << 64, 3)
+    for _ in range(200):
+        got = O.range_incl(r1, 0, 2)
         while True:
             v = O.next_u64(r2)
-            p = v * n
-            if (p & M64) <= zone:
-                assert got == p >> 64
+            q = int(Fraction(v) / third)                 # floor(3v / 2^64)
+            frac = Fraction(v) - q * third               # position inside its third
+            if frac * 3 < (3 << 62):                    # lo = 3 * frac < 3 * 2^62
                 break
-        assert 0 <= got < n
+        assert got == q
+    # gen_range(a..b) shifts by a: low + hi
+    r1, r2 = O.rng(9), O.rng(9)
+    assert O.range_incl(r1, 1000, 1000 + 7) == 1000 + O.range_incl(r2, 0, 7)
 
 
 # ---------------------------------------------------------------- reductions
@@ -301,6 +333,74 @@ def test_shem_star_prefers_heavy_edge():
         assert (m != np.arange(4)).sum() >= 2
 
 
+# ---------------------------------------------------------------- grow
+def _distinct_draws(seed, n, k):
+    """The seed vertices grow.rs:24-32 picks: distinct gen_range(0..n) draws."""
+    r = O.rng(seed)
+    out = []
+    while len(out) < k:
+        v = O.range_incl(r, 0, n - 1)
+        if v not in out:
+            out.append(v)
+    return out
+
+
+def _seed_for(n, want, limit=200000):
+    """An attempt seed whose first distinct draws over 0..n are `want`."""
+    for s in range(1, limit):
+        if _distinct_draws(s, n, len(want)) == list(want):
+            return s
+    raise AssertionError(f"no seed draws {want}")
+
+
+def test_grow_k_equals_n_is_the_draw_order():
+    # every vertex is a seed and at its target (1/k * n = 1): part[seed_p] = p
+    x, a = grid_sorted(3, 3)
+    for seed in (1, 2, 77):
+        order = _distinct_draws(seed, 9, 9)
+        part = O.grow_partition(1, x, a, None, 9, np.ones(9), O.rng(seed))
+        assert [int(part[v]) for v in order] == list(range(9))
+
+
+@pytest.mark.parametrize("n", [7, 8])
+def test_grow_path_from_both_ends(n):
+    # seeds 0 (part 0) and n-1 (part 1); each round part 0 claims the next
+    # vertex to the right, then part 1 the next to the left, until a part
+    # reaches its target n/2: part 0 ends with [0, ceil(n/2)), because it
+    # moves first in every round, and part 1 with the rest
+    x, a = path(n)
+    seed = _seed_for(n, [0, n - 1])
+    part = O.grow_partition(1, x, a, None, 2, np.full(2, n / 2.0), O.rng(seed))
+    h = (n + 1) // 2
+    assert part.tolist() == [0] * h + [1] * (n - h)
+
+
+def test_grow_claims_every_free_neighbour_of_a_popped_vertex():
+    # star centre 0, leaves 1..5; seeds 0 and 3: part 0's first pop claims
+    # all free leaves at once (past its target 3), part 1 never grows
+    x, a = build_undirected(6, [(0, i) for i in range(1, 6)])
+    seed = _seed_for(6, [0, 3])
+    part = O.grow_partition(1, x, a, None, 2, np.full(2, 3.0), O.rng(seed))
+    assert part.tolist() == [0, 0, 0, 1, 0, 0]
+
+
+def test_grow_leftovers_go_to_the_first_lightest_part_on_constraint_0():
+    # edge (0, 1) plus isolated vertices; seeds 0 and 1 block each other, so
+    # every other vertex is a leftover, assigned in ascending order to the
+    # part lightest on constraint 0, the lower index on ties
+    x, a = build_undirected(8, [(0, 1)])
+    seed = _seed_for(8, [0, 1])
+    part = O.grow_partition(1, x, a, None, 2, np.full(2, 4.0), O.rng(seed))
+    assert part.tolist() == [0, 1, 0, 1, 0, 1, 0, 1]
+    # two constraints: constraint 1 would prefer the other part every time
+    x, a = build_undirected(6, [(0, 1)])
+    seed = _seed_for(6, [0, 1])
+    vw = np.array([1, 9, 2, 9, 5, 1, 1, 1, 1, 1, 1, 1], np.int32)
+    part = O.grow_partition(2, x, a, vw, 2, np.full(4, 100.0), O.rng(seed))
+    # pw0 = 1, pw1 = 2 on constraint 0: v2 -> 0 (pw0 6), v3, v4, v5 -> 1 (3, 4, 5)
+    assert part.tolist() == [0, 1, 0, 1, 1, 1]
+
+
 # ---------------------------------------------------------------- pipeline
 def _balanced(xadj, part, k, ufactor, vwgt=None):
     n = len(xadj) - 1
@@ -331,3 +431,67 @@ def test_pipeline_reference_cases():
     x, a = cycle4()
     assert O.part_graph_kway(x, a, 10)[0] == -2
     assert O.part_graph_kway(x, a, 1)[0] == -2
+
+
+def _kway_composed(x, a, k, seed, vwgt=None, adjwgt=None, ncon=1, ncuts=1, niter=10, ufactor=30,
+                   tpwgts=None, shem=True):
+    """partition/kway.rs:20-141 composed in Python from the pinned oracle
+    pieces (matching, contraction, targets, grow, greedy refine, cut): one
+    attempt RNG per attempt drawn in the driver's order (coarsen/mod.rs:49-69
+    matchings, then grow.rs seeds), the f32 stop rule, projection, strictly
+    better cut kept."""
+    ub = np.full(ncon, 1.0 + ufactor / 1000.0)
+    best = None
+    for att in range(ncuts):
+        r = O.rng(seed + att)
+        cur = (x, a, vwgt, adjwgt)
+        levels = []
+        limit = max(20 * k, 20)
+        while True:
+            cx, ca, cvw, caw = cur
+            m = O.shem_matching(cx, ca, caw, r) if shem else O.random_matching(cx, ca, r)
+            cg = O.build_coarse_graph(ncon, cx, ca, cvw, caw, None, m)
+            nxt = (cg["cxadj"], cg["cadjncy"], cg["cvwgt"], cg["cadjwgt"] if cg["adjwgt_present"] else None)
+            levels.append((cur, cg["cmap"]))
+            ratio = np.float32(cg["n"]) / np.float32(len(cx) - 1)
+            cur = nxt
+            if cg["n"] <= limit or ratio > np.float32(0.95):
+                break
+        cx, ca, cvw, caw = cur
+        tpw = O.target_weights(len(cx) - 1, ncon, cvw, k, tpwgts)
+        part = O.grow_partition(ncon, cx, ca, cvw, k, tpw, r)
+        part, _ = O.greedy_refine(ncon, cx, ca, cvw, caw, part, k, tpw, ub, niter)
+        for (fx, fa, fvw, faw), cmap in reversed(levels):
+            part = part[cmap]
+            tpw = O.target_weights(len(fx) - 1, ncon, fvw, k, tpwgts)
+            part, _ = O.greedy_refine(ncon, fx, fa, fvw, faw, part, k, tpw, ub, niter)
+        obj = O.edge_cut(x, a, adjwgt, part)
+        if best is None or obj < best[0]:
+            best = (obj, part)
+    return best
+
+
+@pytest.mark.parametrize("case", ["grid", "weighted_rgg", "rm", "ncuts", "tpwgts"])
+def test_part_graph_kway_is_the_composition_of_its_pinned_pieces(case):
+    # the driver's composition and RNG consumption order, checked against an
+    # independent transcription over the functions pinned above
+    import rmetis_amd.synth as S
+    kw = {}
+    if case == "grid":
+        x, a = grid_sorted(24, 20)
+        k, vw, aw = 6, None, None
+    else:
+        g = S.rgg(1500, 3, vwgt_max=3)
+        x, a, vw, aw, k = g.xadj, g.adjncy, g.vwgt, g.adjwgt, 8
+        if case == "rm":
+            kw["shem"] = False
+        if case == "ncuts":
+            kw["ncuts"] = 3
+        if case == "tpwgts":
+            kw["tpwgts"] = np.array([0.05, 0.1, 0.15, 0.2, 0.1, 0.1, 0.2, 0.1], np.float32)
+    for seed in (1, 5):
+        obj, part = _kway_composed(x, a, k, seed, vwgt=vw, adjwgt=aw, **kw)
+        rc, o_obj, o_part = O.part_graph_kway(x, a, k, vwgt=vw, adjwgt=aw, seed=seed,
+                                              ctype_shem=kw.get("shem", True), ncuts=kw.get("ncuts", 1),
+                                              tpwgts=kw.get("tpwgts"))
+        assert rc == 1 and o_obj == obj and np.array_equal(o_part, part)
