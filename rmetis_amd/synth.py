@@ -50,11 +50,18 @@ def _csr_from_undirected(n, u, v, w=None):
     """u < v undirected edge list (no duplicates) -> symmetric CSR, sorted rows."""
     src = np.concatenate([u, v]).astype(np.int64)
     dst = np.concatenate([v, u]).astype(np.int64)
-    ww = None if w is None else np.concatenate([w, w])
-    order = np.lexsort((dst, src))
-    src, dst = src[order], dst[order]
-    if ww is not None:
-        ww = ww[order].astype(np.int32)
+    # one int64 key per entry (row-major): a plain sort when there are no
+    # weights to carry, else a stable argsort of the keys
+    key = src * np.int64(n) + dst
+    ww = None
+    if w is None:
+        key.sort()
+    else:
+        order = np.argsort(key, kind="stable")
+        key = key[order]
+        ww = np.concatenate([w, w])[order].astype(np.int32)
+    src = key // np.int64(n)
+    dst = key - src * np.int64(n)
     counts = np.bincount(src, minlength=n)
     xadj = np.zeros(n + 1, np.int64)
     np.cumsum(counts, out=xadj[1:])
@@ -146,32 +153,56 @@ def rgg(n, seed, r=None, wmax=10, vwgt_max=1, ncon=1, name=None, rng=None):
     return CSR(xadj, adj, vw, ww, ncon=ncon, name=name or f"rgg{n}")
 
 
-def rmat(scale, ef, seed, a=0.57, b=0.19, c=0.19, name=None, chunk=1 << 23):
+def _rmat_chunk(args):
+    """Unique (lo * n + hi) keys of one chunk of R-MAT pairs.  The chunk's
+    draws are the ones a single sequential stream would give it: scale
+    uniform doubles per pair and level, level by level, after all earlier
+    chunks (PCG64 advanced past them)."""
+    seed, scale, off, cnt, a, b, c = args
+    bg = np.random.PCG64(seed)
+    bg.advance(scale * off)
+    rng = np.random.Generator(bg)
+    src = np.zeros(cnt, np.int64)
+    dst = np.zeros(cnt, np.int64)
+    ab, abc = a + b, a + b + c
+    for lvl in range(scale):
+        rr = rng.random(cnt)
+        # quadrant 0 (a), 1 (b: right), 2 (c: down), 3 (d: down-right)
+        q = (rr >= a).view(np.uint8) + (rr >= ab).view(np.uint8) + (rr >= abc).view(np.uint8)
+        dst <<= 1
+        dst |= q & 1
+        src <<= 1
+        src |= q >> 1
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    lo, hi = np.minimum(src, dst), np.maximum(src, dst)
+    return np.unique(lo * (np.int64(1) << scale) + hi)
+
+
+def rmat(scale, ef, seed, a=0.57, b=0.19, c=0.19, name=None, chunk=1 << 23, workers=None):
     """R-MAT (scale, edge factor) graph: symmetrised, deduplicated, no
-    self-loops, vertex ids randomly permuted, unit weights."""
-    rng = np.random.default_rng(seed)
+    self-loops, vertex ids randomly permuted, unit weights.  Chunks of pairs
+    are drawn in parallel processes from positions of one PCG64 stream, so
+    the graph does not depend on the number of workers."""
+    import os
     n = 1 << scale
     total = ef * n
-    keys = []
-    for off in range(0, total, chunk):
-        cnt = min(chunk, total - off)
-        src = np.zeros(cnt, np.int64)
-        dst = np.zeros(cnt, np.int64)
-        for lvl in range(scale):
-            rr = rng.random(cnt)
-            bit = np.int64(1) << np.int64(scale - 1 - lvl)
-            right = (rr >= a) & (rr < a + b) | (rr >= a + b + c)
-            down = rr >= a + b
-            dst |= np.where(right, bit, 0)
-            src |= np.where(down, bit, 0)
-        keep = src != dst
-        src, dst = src[keep], dst[keep]
-        lo, hi = np.minimum(src, dst), np.maximum(src, dst)
-        keys.append(np.unique(lo * n + hi))
+    jobs = [(seed, scale, off, min(chunk, total - off), a, b, c) for off in range(0, total, chunk)]
+    if workers is None:
+        workers = min(16, os.cpu_count() or 1, len(jobs))
+    if workers <= 1 or len(jobs) == 1:
+        keys = [_rmat_chunk(j) for j in jobs]
+    else:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            keys = pool.map(_rmat_chunk, jobs, chunksize=1)
     key = np.unique(np.concatenate(keys))
-    u, v = key // n, key % n
-    perm = rng.permutation(n)
-    u, v = perm[u], perm[v]
+    del keys
+    bg = np.random.PCG64(seed)
+    bg.advance(scale * total)
+    perm = np.random.Generator(bg).permutation(n)
+    u, v = perm[key >> scale], perm[key & (n - 1)]
+    del key
     lo, hi = np.minimum(u, v), np.maximum(u, v)
     xadj, adj, _ = _csr_from_undirected(n, lo, hi)
     return CSR(xadj, adj, name=name or f"rmat{scale}")

@@ -33,32 +33,37 @@ SEEDS = range(1, 9)
 CUT_FACTOR = 1.03
 
 
-def _over_limit(g, part, k, ubvec):
-    """Number of (part, constraint) pairs over (1/k) * total * ub."""
+def _over_limit(g, part, k, ubvec, tpwgts=None):
+    """Number of (part, constraint) pairs over target * ub, target =
+    (tpwgts[p*ncon+c] as f64, or 1/k) * total[c] (graph/mod.rs:392-410)."""
     ncon = g.ncon
     vw = np.ones((g.n, ncon), np.int64) if g.vwgt is None else g.vwgt.reshape(g.n, ncon).astype(np.int64)
     pw = np.zeros((k, ncon), np.int64)
     np.add.at(pw, part, vw)
     total = vw.sum(axis=0)
     ub = np.array([1.0 + 30 / 1000.0] * ncon if ubvec is None else [float(np.float32(u)) for u in ubvec])
-    lim = (1.0 / k) * total.astype(np.float64) * ub
-    return int((pw.astype(np.float64) > lim[None, :]).sum())
+    frac = (np.full((k, ncon), 1.0 / k) if tpwgts is None
+            else np.asarray(tpwgts, np.float32).astype(np.float64).reshape(k, ncon))
+    lim = frac * total.astype(np.float64)[None, :] * ub[None, :]
+    return int((pw.astype(np.float64) > lim).sum())
 
 
-def _run(g, k, ubvec=None):
-    """(gpu cuts, oracle cuts, gpu over-limit count, oracle over-limit count) over SEEDS."""
+def _run(g, k, ubvec=None, tpwgts=None, ctype=R.CType.Shem, seeds=SEEDS):
+    """(gpu cuts, oracle cuts, gpu over-limit count, oracle over-limit count) over the seeds."""
     rg = R.Graph(g.n, g.ncon, g.xadj, g.adjncy, g.vwgt, g.adjwgt)
     gc, oc, gov, oov = [], [], 0, 0
-    for s in SEEDS:
-        r = R.part_graph_kway(rg, k, ubvec=ubvec, options=R.Options.for_kway(seed=s, rtype=R.RType.Greedy))
+    for s in seeds:
+        r = R.part_graph_kway(rg, k, tpwgts=tpwgts, ubvec=ubvec,
+                              options=R.Options.for_kway(seed=s, rtype=R.RType.Greedy, ctype=ctype))
         assert r.objval == rg.edge_cut(r.part)
         rc, obj, opart = O.part_graph_kway(g.xadj, g.adjncy, k, vwgt=g.vwgt, adjwgt=g.adjwgt, ncon=g.ncon,
-                                           ubvec=ubvec, seed=s)
+                                           tpwgts=tpwgts, ubvec=ubvec, seed=s,
+                                           ctype_shem=ctype == R.CType.Shem)
         assert rc == 1
         gc.append(r.objval)
         oc.append(obj)
-        gov += _over_limit(g, r.part, k, ubvec)
-        oov += _over_limit(g, opart, k, ubvec)
+        gov += _over_limit(g, r.part, k, ubvec, tpwgts)
+        oov += _over_limit(g, opart, k, ubvec, tpwgts)
     return gc, oc, gov, oov
 
 
