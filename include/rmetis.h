@@ -88,6 +88,25 @@ int32_t rmetis_hip_kway_device(idx_t nvtxs, idx_t ncon, const idx_t* d_xadj, con
                                const real_t* ubvec, const idx_t* options, idx_t* objval, idx_t* d_part,
                                rmetis_hip_stats* stats);
 
+/* No counterpart in the reference: rmetis_hip_kway_device for callers that
+ * cannot allocate (code that forbids hipMalloc, such as tight serving loops
+ * and harnesses that pre-plan all device memory).  Same arguments, same
+ * result, but no device or pinned memory is allocated and no stream is
+ * created: every device buffer of the call comes from a fixed workspace of
+ * 128 MiB per GPU (a static device array of the library), the kernels run
+ * on the null stream and host staging uses heap memory.  A graph whose work
+ * buffers and first coarse level do not fit is rejected with
+ * METIS_ERROR_MEMORY before any kernel runs (part and objval unwritten); a
+ * hierarchy that outgrows the workspace at a deeper level also ends with
+ * METIS_ERROR_MEMORY, after the call's queued work has finished (objval
+ * unwritten; d_part may then hold an earlier attempt's partition when
+ * NCUTS > 1).  Calls on one GPU share the workspace and are serialised by a
+ * lock; the workspace keeps the last call's contents. */
+int32_t rmetis_hip_kway_device_noalloc(idx_t nvtxs, idx_t ncon, const idx_t* d_xadj, const idx_t* d_adjncy,
+                                       const idx_t* d_vwgt, const idx_t* d_adjwgt, idx_t nparts,
+                                       const real_t* tpwgts, const real_t* ubvec, const idx_t* options,
+                                       idx_t* objval, idx_t* d_part);
+
 /* Many graphs in one call (no single counterpart in the reference: each
  * graph gets what RMETIS_PartGraphKway / partition_kway, kway.rs:20-141,
  * returns for it alone).  The graphs are laid out one after another in one

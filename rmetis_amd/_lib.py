@@ -34,6 +34,8 @@ SIGNATURES = {
     "rmetis_hip_kway_device": (C.c_int32, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_void_p, C.c_int32, f32p, f32p, i32p, i32p, C.c_void_p,
                                            C.POINTER(HipStats)]),
+    "rmetis_hip_kway_device_noalloc": (C.c_int32, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                   C.c_void_p, C.c_int32, f32p, f32p, i32p, i32p, C.c_void_p]),
     "rmetis_hip_kway_batch_device": (C.c_int32, [C.c_int32, i32p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                                  C.c_void_p, i32p, f32p, i32p, C.POINTER(C.c_uint64), i32p,
                                                  C.c_void_p, C.POINTER(HipStats)]),

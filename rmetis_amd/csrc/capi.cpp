@@ -160,6 +160,31 @@ void fill_stats(const KwayStats& ks, rmetis_hip_stats* out) {
     }
 }
 
+// Fixed device workspace of the allocation-free k-way entry point: one
+// array per GPU (each device holds its own copy from the library's load to
+// the end of the process).  Host code never uses the array's name as an
+// address: its device address is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = 128ull << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu[kMaxDevices];      // one call at a time per device
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on device dev, the current device (caller
+// holds g_scratch_mu[dev]).
+void* scratch_address(int dev) {
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(-4, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace
 }  // namespace rk
 
@@ -224,6 +249,58 @@ int32_t rmetis_hip_kway_device(idx_t nvtxs, idx_t ncon, const idx_t* d_xadj, con
         KwayStats ks;
         const int32_t obj = kway_device(*cx.c, g, nparts, tpwgts, ubvec, o, d_part, stats ? &ks : nullptr);
         if (stats) fill_stats(ks, stats);
+        if (objval) *objval = obj;
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_kway_device_noalloc(idx_t nvtxs, idx_t ncon, const idx_t* d_xadj, const idx_t* d_adjncy,
+                                       const idx_t* d_vwgt, const idx_t* d_adjwgt, idx_t nparts,
+                                       const real_t* tpwgts, const real_t* ubvec, const idx_t* options,
+                                       idx_t* objval, idx_t* d_part) {
+    // rmetis_hip_kway_device with every device buffer carved from `scratch`
+    // (no hipMalloc, hipFree or pinned allocation; the null stream; host
+    // staging in heap memory)
+    return guarded([&]() -> int32_t {
+        if (!d_xadj || !d_adjncy || !d_part) return METIS_ERROR_INPUT;
+        if (nvtxs <= 0 || ncon <= 0 || nparts < 2) return METIS_ERROR_INPUT;
+        const Options o = parse_options(options);
+        int cnt = 0;
+        if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0)
+            throw Error(-4, "no HIP device is available (the k-way path runs only on the GPU)");
+        int dev = 0;
+        RK_HIP(hipGetDevice(&dev));
+        if (dev < 0 || dev >= kMaxDevices) throw Error(-4, "device index out of range");
+        std::lock_guard<std::mutex> lk(g_scratch_mu[dev]);
+        void* ws = scratch_address(dev);
+        DevGraph g;
+        g.n = nvtxs;
+        g.ncon = ncon;
+        RK_HIP(hipMemcpy(&g.m, d_xadj + nvtxs, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (g.m < 0) return METIS_ERROR_INPUT;
+        const size_t need = kway_first_level_bytes(nvtxs, g.m, nparts, ncon);
+        if (need > kScratchBytes)
+            throw Error(-3, "graph needs at least " + std::to_string((need + (1 << 20) - 1) >> 20) +
+                                " MiB of device workspace; the allocation-free entry point has " +
+                                std::to_string(kScratchBytes >> 20) + " MiB");
+        g.xadj = (int32_t*)d_xadj;
+        g.adjncy = (int32_t*)d_adjncy;
+        g.vwgt = (int32_t*)d_vwgt;
+        g.adjwgt = (int32_t*)d_adjwgt;
+        Context cx;
+        cx.device = dev;
+        cx.stream = nullptr;   // the null stream: no stream is created
+        cx.pageable = true;
+        cx.arena.attach(ws, kScratchBytes);
+        int32_t obj = 0;
+        try {
+            obj = kway_device(cx, g, nparts, tpwgts, ubvec, o, d_part, nullptr);
+        } catch (...) {
+            // work already queued still uses the workspace: let it finish
+            // before the next call may take the lock
+            (void)hipStreamSynchronize(nullptr);
+            throw;
+        }
         if (objval) *objval = obj;
         return METIS_OK;
     });

@@ -4,7 +4,9 @@
 // reference's entry points are reentrant, c_api.rs:55-81) and repeated calls
 // reuse their memory instead of calling hipMalloc again.
 #pragma once
+#include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -17,6 +19,13 @@ public:
         size_t chunk, off;
     };
     ~Arena() { release(); }
+    // Carves every allocation from [base, base + bytes), memory the arena
+    // does not own; running out is then an error, never a hipMalloc.
+    void attach(void* base, size_t bytes) {
+        release();
+        chunks_.push_back({(char*)base, bytes});
+        external_ = true;
+    }
     void* alloc(size_t bytes) {
         bytes = (bytes + 255) & ~(size_t)255;
         if (bytes == 0) bytes = 256;
@@ -29,6 +38,9 @@ public:
             ++cur_;
             off_ = 0;
         }
+        if (external_)
+            throw Error(-3, "the fixed device workspace (" + std::to_string(chunks_.empty() ? 0 : chunks_[0].size >> 20) +
+                                " MiB) is too small for this graph's hierarchy");
         size_t sz = bytes > kChunk ? bytes : kChunk;
         void* p = nullptr;
         RK_HIP(hipMalloc(&p, sz));
@@ -65,9 +77,18 @@ public:
         off_ = 0;
     }
     void release() {
-        for (auto& c : chunks_) (void)hipFree(c.base);
+        if (!external_)
+            for (auto& c : chunks_) (void)hipFree(c.base);
         chunks_.clear();
+        external_ = false;
         reset();
+    }
+    // Bytes handed out so far (the high-water mark of the current chunk
+    // chain up to the cursor).
+    size_t used() const {
+        size_t t = 0;
+        for (size_t i = 0; i < cur_ && i < chunks_.size(); ++i) t += chunks_[i].size;
+        return t + off_;
     }
     size_t reserved() const {
         size_t t = 0;
@@ -83,12 +104,16 @@ private:
     };
     std::vector<Chunk> chunks_;
     size_t cur_ = 0, off_ = 0;
+    bool external_ = false;
 };
 
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     Arena arena;
+    // pageable: host staging is ordinary heap memory instead of pinned
+    // hipHostMalloc buffers (for callers that allow no HIP allocation)
+    bool pageable = false;
     void* pinned = nullptr;
     size_t pinned_size = 0;
 
@@ -97,7 +122,10 @@ struct Context {
     int32_t* small = nullptr;   // pinned scratch for device-to-host reads of a few counters
 
     int32_t* host_small() {
-        if (!small) RK_HIP(hipHostMalloc((void**)&small, kSmallBytes, hipHostMallocDefault));
+        if (!small) {
+            if (pageable) small = (int32_t*)heap_block(kSmallBytes);
+            else RK_HIP(hipHostMalloc((void**)&small, kSmallBytes, hipHostMallocDefault));
+        }
         return small;
     }
     static constexpr size_t kSmallBytes = 4096;
@@ -106,6 +134,13 @@ struct Context {
     // after the stream has been synchronised past its last copy.
     void* host_stage(size_t bytes) { return grow(pinned, pinned_size, bytes); }
     void* host_stage2(size_t bytes) { return grow(pinned2, pinned2_size, bytes); }
+    // Heap blocks of a pageable context; kept until the context ends, so a
+    // pointer handed out stays valid while copies from it may be in flight.
+    std::vector<std::unique_ptr<char[]>> heap;
+    void* heap_block(size_t bytes) {
+        heap.emplace_back(new char[bytes > 0 ? bytes : 1]);
+        return heap.back().get();
+    }
     // Pinned bump area for many small uploads within one call: nothing is
     // reused before upload_reset(), which the caller issues only after the
     // stream has been synchronised past the last copy.  Chunks are kept for
@@ -123,7 +158,8 @@ struct Context {
         }
         const size_t sz = bytes > kUpChunk ? bytes : kUpChunk;
         void* p = nullptr;
-        RK_HIP(hipHostMalloc(&p, sz, hipHostMallocDefault));
+        if (pageable) p = heap_block(sz);
+        else RK_HIP(hipHostMalloc(&p, sz, hipHostMallocDefault));
         up_chunks.push_back({(char*)p, sz});
         up_cur = up_chunks.size() - 1;
         up_off = bytes;
@@ -137,21 +173,27 @@ struct Context {
     std::vector<std::pair<char*, size_t>> up_chunks;
     size_t up_cur = 0, up_off = 0;
 
-    static void* grow(void*& p, size_t& cap, size_t bytes) {
+    void* grow(void*& p, size_t& cap, size_t bytes) {
         if (bytes > cap) {
-            if (p) (void)hipHostFree(p);
-            p = nullptr;
             size_t sz = bytes < (1u << 20) ? (1u << 20) : bytes;
-            RK_HIP(hipHostMalloc(&p, sz, hipHostMallocDefault));
+            if (pageable) {
+                p = heap_block(sz);     // the old block stays owned by `heap`
+            } else {
+                if (p) (void)hipHostFree(p);
+                p = nullptr;
+                RK_HIP(hipHostMalloc(&p, sz, hipHostMallocDefault));
+            }
             cap = sz;
         }
         return p;
     }
     ~Context() {
-        if (pinned) (void)hipHostFree(pinned);
-        if (pinned2) (void)hipHostFree(pinned2);
-        if (small) (void)hipHostFree(small);
-        for (auto& c : up_chunks) (void)hipHostFree(c.first);
+        if (!pageable) {
+            if (pinned) (void)hipHostFree(pinned);
+            if (pinned2) (void)hipHostFree(pinned2);
+            if (small) (void)hipHostFree(small);
+            for (auto& c : up_chunks) (void)hipHostFree(c.first);
+        }
         arena.release();
         if (stream) (void)hipStreamDestroy(stream);
     }

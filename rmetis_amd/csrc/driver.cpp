@@ -100,60 +100,101 @@ void thresholds_from_targets(int32_t k, int32_t ncon, const double* tpw, const d
         }
 }
 
-Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon, int64_t tiles, int64_t kc_total) {
-    Arena& a = cx.arena;
-    const hipStream_t s = cx.stream;
+namespace {
+
+// The per-call work buffers, laid out through A: the arena when allocating,
+// a byte counter when sizing (buffers_bytes), so that both see one list.
+template <class A>
+Buffers layout_buffers(A& a, int32_t n, int32_t m, int32_t k, int32_t ncon, int64_t ntiles, int64_t kc) {
     Buffers B;
-    const int64_t ntiles = tiles >= 0 ? tiles : ((int64_t)n + kTile - 1) / kTile;
-    const int64_t kc = kc_total >= 0 ? kc_total : (int64_t)k * ncon;
     RefineBufs& r = B.rb;
-    r.tile_rec = a.get<int4>(ntiles * kTile);
-    r.tile_cnt = a.get<int32_t>(ntiles);
-    r.tile_off = a.get<int32_t>(ntiles);
-    r.tile_bits = (uint32_t*)a.get<int32_t>(2 * ((ntiles + 31) / 32));
+    r.tile_rec = a.template get<int4>(ntiles * kTile);
+    r.tile_cnt = a.template get<int32_t>(ntiles);
+    r.tile_off = a.template get<int32_t>(ntiles);
+    r.tile_bits = (uint32_t*)a.template get<int32_t>(2 * ((ntiles + 31) / 32));
     r.tile_dirty = r.tile_bits + (ntiles + 31) / 32;
-    RK_HIP(hipMemsetAsync(r.tile_bits, 0, (size_t)(2 * ((ntiles + 31) / 32)) * sizeof(uint32_t), s));
-    r.mov = a.get<int4>(n);
-    r.pool = a.get<int32_t>(m);
-    r.heavy_rec = a.get<int4>(n);
-    r.heavy_scratch = k > 4096 ? a.get<int32_t>((int64_t)kHeavyGlobalBlocks * (3 * (int64_t)k + 2)) : nullptr;
-    r.moves = a.get<int2>(n);
-    r.pw = a.get<int32_t>(kc);
-    r.fmax = a.get<long long>(kc);
-    r.omax = a.get<long long>(kc);
-    r.adm_scratch = a.get<int32_t>(4 * kc);
-    int32_t* cnt = a.get<int32_t>(64);
+    r.mov = a.template get<int4>(n);
+    r.pool = a.template get<int32_t>(m);
+    r.heavy_rec = a.template get<int4>(n);
+    r.heavy_scratch = k > 4096 ? a.template get<int32_t>((int64_t)kHeavyGlobalBlocks * (3 * (int64_t)k + 2)) : nullptr;
+    r.moves = a.template get<int2>(n);
+    r.pw = a.template get<int32_t>(kc);
+    r.fmax = a.template get<long long>(kc);
+    r.omax = a.template get<long long>(kc);
+    r.adm_scratch = a.template get<int32_t>(4 * kc);
+    int32_t* cnt = a.template get<int32_t>(64);
     r.heavy_cnt = cnt + 0;
     r.nmov = cnt + 1;
     r.nmoves = cnt + 2;
     r.done = cnt + 3;
     r.passes = cnt + 4;
     r.full = cnt + 5;
-    r.host_small = cx.host_small();
     r.scan_tmp = a.alloc(scan_temp_bytes(std::max<int64_t>(ntiles, n)));
     CoarsenBufs& c = B.cb;
-    c.match = a.get<int32_t>(n);
-    c.cand = a.get<int32_t>(n);
-    c.leader = a.get<int32_t>(n);
-    c.cid = a.get<int32_t>(n);
-    c.members = a.get<int32_t>(2 * (int64_t)n);
-    c.degsum = a.get<int32_t>(n);
-    c.tmp_off = a.get<int32_t>(n);
-    c.tmp_adj = a.get<int32_t>(m);
-    c.tmp_w = a.get<int32_t>(m);
-    c.cdeg = a.get<int32_t>(n);
-    c.big_list = a.get<int32_t>(n);
-    c.hash_keys = a.get<int32_t>(2 * (int64_t)m + n + 1);
-    c.hash_first = a.get<int32_t>(2 * (int64_t)m + n + 1);
-    c.hash_sum = a.get<int32_t>(2 * (int64_t)m + n + 1);
-    c.counters = a.get<int32_t>(16);
-    c.host_small = r.host_small;
-    c.suitor = a.get<uint64_t>(n);
+    c.match = a.template get<int32_t>(n);
+    c.cand = a.template get<int32_t>(n);
+    c.leader = a.template get<int32_t>(n);
+    c.cid = a.template get<int32_t>(n);
+    c.members = a.template get<int32_t>(2 * (int64_t)n);
+    c.degsum = a.template get<int32_t>(n);
+    c.tmp_off = a.template get<int32_t>(n);
+    c.tmp_adj = a.template get<int32_t>(m);
+    c.tmp_w = a.template get<int32_t>(m);
+    c.cdeg = a.template get<int32_t>(n);
+    c.big_list = a.template get<int32_t>(n);
+    c.hash_keys = a.template get<int32_t>(2 * (int64_t)m + n + 1);
+    c.hash_first = a.template get<int32_t>(2 * (int64_t)m + n + 1);
+    c.hash_sum = a.template get<int32_t>(2 * (int64_t)m + n + 1);
+    c.counters = a.template get<int32_t>(16);
+    c.suitor = a.template get<uint64_t>(n);
     c.scan_tmp = r.scan_tmp;
-    B.cut_acc = a.get<unsigned long long>(1);
-    B.cut_out = a.get<int32_t>(1);
-    B.wsum = a.get<long long>(std::max(ncon, 1));
+    B.cut_acc = a.template get<unsigned long long>(1);
+    B.cut_out = a.template get<int32_t>(1);
+    B.wsum = a.template get<long long>(std::max(ncon, 1));
     return B;
+}
+
+// Sizes allocations as Arena does (256-byte granules) without memory.
+struct ByteCounter {
+    size_t total = 0;
+    void* alloc(size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        total += bytes == 0 ? 256 : bytes;
+        return nullptr;
+    }
+    template <typename T>
+    T* get(int64_t count) {
+        return (T*)alloc((size_t)(count > 0 ? count : 1) * sizeof(T));
+    }
+};
+
+}  // namespace
+
+Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon, int64_t tiles, int64_t kc_total) {
+    const int64_t ntiles = tiles >= 0 ? tiles : ((int64_t)n + kTile - 1) / kTile;
+    const int64_t kc = kc_total >= 0 ? kc_total : (int64_t)k * ncon;
+    Buffers B = layout_buffers(cx.arena, n, m, k, ncon, ntiles, kc);
+    RK_HIP(hipMemsetAsync(B.rb.tile_bits, 0, (size_t)(2 * ((ntiles + 31) / 32)) * sizeof(uint32_t), cx.stream));
+    B.rb.host_small = cx.host_small();
+    B.cb.host_small = B.rb.host_small;
+    return B;
+}
+
+size_t kway_first_level_bytes(int32_t n, int32_t m, int32_t k, int32_t ncon) {
+    ByteCounter c;
+    const int64_t ntiles = ((int64_t)n + kTile - 1) / kTile;
+    const int64_t kc = (int64_t)k * ncon;
+    layout_buffers(c, n, m, k, ncon, ntiles, kc);
+    c.get<int32_t>(n);                          // finest heavy-vertex list
+    c.get<int32_t>(n);                          // first level: cmap
+    c.get<int32_t>(n);                          //              heavy list
+    c.get<int32_t>((int64_t)n * ncon);          //              coarse vwgt
+    c.get<int32_t>((int64_t)n + 1);             //              coarse xadj
+    c.get<int32_t>(m > 0 ? m : 1);              //              coarse adjncy
+    c.get<int32_t>(m > 0 ? m : 1);              //              coarse adjwgt
+    c.get<long long>(4 * kc);                   //              thresholds of two levels
+    c.get<int32_t>(n);                          // finest part
+    return c.total;
 }
 
 std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffers& B) {

@@ -73,6 +73,11 @@ struct Buffers {
 Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon, int64_t tiles = -1,
                       int64_t kc_total = -1);
 
+// Device bytes kway_device takes from its arena before the second
+// coarsening level: the work buffers, the finest heavy-vertex list and the
+// first coarse level (deeper levels take more, which the arena reports).
+size_t kway_first_level_bytes(int32_t n, int32_t m, int32_t k, int32_t ncon);
+
 // Column sums of vwgt (or n per constraint when vwgt is absent), checked
 // against the int32 range the reference's sums live in.
 std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffers& B);
