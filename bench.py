@@ -1,17 +1,25 @@
 #!/usr/bin/env python3
 """k-way partitioning throughput on one MI355X.
 
-A step is one full multilevel k-way partition (coarsening, initial
-partition, uncoarsening with greedy refinement) of the synthetic graph, whose
-CSR is already resident in HBM when the timed region starts.  Default
-workload: random geometric graph, 2^20 points in the unit square, r =
-sqrt(12/(pi n)), Morton-ordered ids, vertex weight 1, edge weights U{1..10},
-k = 64, SHEM + greedy, ufactor 30, niter 10, ncuts 1, seed 1.
+Default workload (the headline): the uneven batch of 1,024 graphs (n
+log-uniform in 2^10..2^18, half 2D grids with a random aspect ratio, half
+random geometric graphs of mean degree 8-16, vwgt U{1..4}, adjwgt U{1..10},
+k from {2,4,8,16,32,64}), 8 attempts (seeds) per graph, all packed into one
+segmented CSR resident in HBM.  A step is one batched call that partitions
+every graph (multilevel k-way: SHEM coarsening, BFS growing of the coarsest
+level, greedy refinement, ufactor 30, niter 10).  The same graphs through the
+plain one-graph-per-call GPU path give `plain`, and their ratio
+`batched_vs_plain`.
 
-Prints one JSON line: graphs/s (value), per-pass edge rates, the roofline of
-the gain-table kernel on the finest level (HIP events on the library's
-stream), and a CPU baseline (the oracle restatement of the reference on the
-same graph, one host core).
+The JSON line also carries the roofline of the step's dominant kernel (HIP
+events on the library's stream over the timed steps, algorithmic bytes per
+launch), per-kernel rates at the finest level, the CPU baseline (the
+oracle's restatement of the reference on a bounded sample, one core and a
+pool over the host cores) and, as an extra key, the single-graph workload
+(`rgg20`: 2^20-point random geometric graph, k = 64).
+
+Other workloads: --workload rgg20 | grid256 | rmat22 | rggNN | rmatNN | batchN
+| serve.
 """
 import argparse
 import ctypes as C
@@ -19,28 +27,176 @@ import json
 import math
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "graphs/s + level edges/s (%HBM peak), batched vs plain GPU, cut <= ref+3%"
+# profiler tags of the library (rmetis_hip_profile_read)
+TAGS = {"gain": 0, "gain_inc": 6, "compact": 1, "admit": 2, "match": 3, "contract": 4}
+# kernel names behind each tag with an algorithmic byte model
+TAG_KERNELS = {"gain": "k_gain_tile (full launches)", "match": "k_suitor_init + k_suitor + k_suitor_pairs",
+               "contract": "k_leader + scans + k_cmap + k_contract_tile/big/copy"}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _prof(L, tag, size=-1):
+    ln, ms, by = C.c_int64(), C.c_double(), C.c_double()
+    L.rmetis_hip_profile_read(tag, size, C.byref(ln), C.byref(ms), C.byref(by))
+    return ln.value, ms.value, by.value
+
+
+def profiled_step(L, step, size):
+    """One untimed step with every kernel family bracketed: per tag, the
+    total ms over the step and (launches, ms, bytes) at the level of `size`
+    vertices."""
+    L.rmetis_hip_profile_filter(-1, -1)
+    L.rmetis_hip_profile_enable(1)
+    step(True)
+    L.rmetis_hip_device_synchronize()
+    total = {name: _prof(L, t)[1] for name, t in TAGS.items()}
+    finest = {name: _prof(L, t, size) for name, t in TAGS.items()}
+    L.rmetis_hip_profile_enable(0)
+    return total, finest
+
+
+def level_kernels(finest):
+    out = {}
+    for name, (ln, ms, by) in finest.items():
+        if ln == 0:
+            continue
+        e = {"launches": ln, "avg_us": round(1e3 * ms / ln, 2)}
+        if by > 0 and ms > 0:
+            gbs = by / (ms * 1e-3) / 1e9
+            e.update({"bytes_per_launch": int(by / ln), "achieved_gbs": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)})
+        out[name] = e
+    return out
+
+
+def dominant(total):
+    """The tag with the most kernel time among those with a byte model."""
+    modelled = [t for t in ("gain", "match", "contract") if total.get(t, 0) > 0]
+    return max(modelled, key=lambda t: total[t])
+
+
+def timed_steps(L, step, steps, tag, size):
+    """Times `steps` steps; only the launches of `tag` (at every level when
+    size < 0) are bracketed by events.  Returns (elapsed s, launches, ms,
+    bytes) of the bracketed launches."""
+    L.rmetis_hip_profile_filter(TAGS[tag], size)
+    L.rmetis_hip_profile_enable(1)
+    L.rmetis_hip_device_synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(False)
+    L.rmetis_hip_device_synchronize()
+    elapsed = time.perf_counter() - t0
+    ln, ms, by = _prof(L, TAGS[tag], size)
+    L.rmetis_hip_profile_enable(0)
+    L.rmetis_hip_profile_filter(-1, -1)
+    return elapsed, ln, ms, by
+
+
+def roofline(tag, ln, ms, by, workload):
+    avg_ms = ms / max(ln, 1)
+    per = by / max(ln, 1)
+    achieved = per / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic, src = None, None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tj = json.load(fh)
+        if tj.get("tag") == tag:
+            traffic, src = tj.get("traffic_bytes_per_launch"), tj.get("summary")
+    return {"bound": "hbm", "kernel": TAG_KERNELS.get(tag, tag), "tag": tag, "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": src, "bytes_per_launch": int(per), "avg_launch_us": round(avg_ms * 1e3, 2),
+            "launches": ln}
+
+
+def cpu_baseline(jobs, seconds_one, seconds_pool, what):
+    """The oracle (the C++ restatement of the reference) on host cores: jobs
+    run one after another on one core for about seconds_one, then on a pool
+    of threads over the host cores for about seconds_pool (the oracle releases
+    the GIL inside each call).  Returns the cpu_baseline object."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    O.build()
+    runs1, t1 = 0, 0.0
+    for j in jobs:
+        if t1 >= seconds_one:
+            break
+        t = time.perf_counter()
+        j(O)
+        t1 += time.perf_counter() - t
+        runs1 += 1
+    threads = host_threads()
+    done = [0]
+    lock = threading.Lock()
+    stop_at = time.perf_counter() + seconds_pool
+    nxt = [0]
+
+    def worker():
+        while time.perf_counter() < stop_at:
+            with lock:
+                i = nxt[0]
+                nxt[0] += 1
+            jobs[i % len(jobs)](O)
+            with lock:
+                done[0] += 1
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=worker) for _ in range(threads)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    tp = time.perf_counter() - t0
+    return {"value": round(done[0] / tp, 4), "unit": "graphs/s", "cores": threads, "kind": "port",
+            "sample": f"{what}: {done[0]} calls on a pool of {threads} host threads in {tp:.1f}s "
+                      f"(independent calls; the reference has no parallelism inside one call)",
+            "single_core": {"value": round(runs1 / t1, 4) if t1 > 0 else None, "unit": "graphs/s", "cores": 1,
+                            "calls": runs1, "seconds": round(t1, 1)},
+            "cpu_model": cpu_model()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rgg20",
-                    help="rgg20 | grid256 | rmat22 | rggNN | rmatNN | batch1024 (uneven batch of that many graphs)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0, help="serve: length of the arrival stream")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="batch1024",
+                    help="batch1024 (default: uneven batch of that many graphs) | rgg20 | grid256 | rmat22 | "
+                         "rggNN | rmatNN | serve")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="serve: length of the arrival stream")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="serve: Poisson arrival rate (requests/s)")
-    ap.add_argument("--serve-pool", type=int, default=48, help="serve: distinct graphs the requests draw from")
+    ap.add_argument("--serve-pool", type=int, default=192, help="serve: distinct graphs the requests draw from")
     ap.add_argument("--slots", type=int, default=3, help="serve: batched calls in flight at once")
     ap.add_argument("--serve-max-batch", type=int, default=0,
                     help="serve: vertices per batched call at most (0: the server's default)")
@@ -48,49 +204,67 @@ def main():
     ap.add_argument("--plain-stride", type=int, default=3,
                     help="batch workloads: the plain path is timed on every n-th graph (odd: grids and RGGs alike)")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-budget-s", type=float, default=20.0)
+    ap.add_argument("--cpu-budget-s", type=float, default=12.0, help="seconds of the one-core CPU baseline")
+    ap.add_argument("--cpu-pool-s", type=float, default=12.0, help="seconds of the all-cores CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="batch workloads: skip the rgg20 extra key")
     ap.add_argument("--no-host-api", action="store_true",
                     help="skip timing the host-buffer C entry point (PCIe-inclusive rate)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="issue one device-to-device copy of this many bytes first (a kernel of known "
                          "traffic, to scale FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--roofline-tag", default="",
+                    help="kernel family of the roofline (gain|match|contract; default: the dominant one)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (the path would shard "
+                                   "batches and attempts across GPUs with one all-reduce, not built)",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
 
+    import rmetis_amd as R
+    from rmetis_amd import _lib
+
+    L = _lib.lib()
+    if R.device_count() < 1:
+        raise SystemExit("no HIP device")
+    if args.copy_bytes > 0:
+        src = L.rmetis_hip_device_alloc(args.copy_bytes)
+        dst = L.rmetis_hip_device_alloc(args.copy_bytes)
+        if not src or not dst or L.rmetis_hip_memcpy(dst, src, args.copy_bytes, 3) != 1:
+            raise RuntimeError(f"calibration copy failed: {_lib.last_error()}")
+        L.rmetis_hip_device_synchronize()
+        L.rmetis_hip_device_free(src)
+        L.rmetis_hip_device_free(dst)
+        log(f"calibration copy of {args.copy_bytes} bytes done")
+    if args.workload.startswith("batch"):
+        out = bench_batch(args)
+    elif args.workload == "serve":
+        out = bench_serve(args)
+    else:
+        out = bench_single(args, args.workload)
+    print(json.dumps(out), flush=True)
+
+
+def bench_single(args, workload, steps=None, warmup=None, extra=False):
+    """Full k-way partitions of one graph resident in HBM (one graph per
+    call, ncuts 1, seed 1)."""
     import numpy as np
     import rmetis_amd as R
     from rmetis_amd import _lib, synth
 
     L = _lib.lib()
-    if R.device_count() < 1:
-        raise SystemExit("no HIP device")
-    if args.workload.startswith("batch"):
-        return bench_batch(args)
-    if args.workload == "serve":
-        return bench_serve(args)
+    steps = args.steps * (4 if extra else 1) if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
     t0 = time.time()
-    g, k = synth.config_graph(args.workload, seed=args.seed)
+    g, k = synth.config_graph(workload, seed=args.seed)
     log(f"graph {g} k={k} generated in {time.time() - t0:.1f}s")
     DA = _lib.DeviceArray
     d_xadj, d_adj = DA(g.xadj), DA(g.adjncy)
     d_vw = DA(g.vwgt) if g.vwgt is not None else None
     d_aw = DA(g.adjwgt) if g.adjwgt is not None else None
     d_part = DA(count=g.n, dtype=np.int32)
-    sync = L.rmetis_hip_device_synchronize
-    sync()
-    if args.copy_bytes > 0:
-        src = L.rmetis_hip_device_alloc(args.copy_bytes)
-        dst = L.rmetis_hip_device_alloc(args.copy_bytes)
-        if not src or not dst or L.rmetis_hip_memcpy(dst, src, args.copy_bytes, 3) != 1:
-            raise RuntimeError(f"calibration copy failed: {_lib.last_error()}")
-        sync()
-        L.rmetis_hip_device_free(src)
-        L.rmetis_hip_device_free(dst)
-        log(f"calibration copy of {args.copy_bytes} bytes done")
+    L.rmetis_hip_device_synchronize()
     opts = R.Options(rtype=R.RType.Greedy, seed=args.seed, ncuts=1, niter=10, ufactor=30).to_c_array()
     stats = _lib.HipStats()
     obj = C.c_int32()
@@ -104,60 +278,44 @@ def main():
         if rc != 1:
             raise RuntimeError(f"kway failed rc={rc}: {_lib.last_error()}")
 
-    for i in range(args.warmup):
+    for i in range(warmup):
         step()
         log(f"warmup {i} cut={obj.value}")
-    def prof(tag, size=-1):
-        ln, ms, by = C.c_int64(), C.c_double(), C.c_double()
-        L.rmetis_hip_profile_read(tag, size, C.byref(ln), C.byref(ms), C.byref(by))
-        return ln.value, ms.value, by.value
-
-    # Timed region: only the full gain launches of the finest level are
-    # bracketed by events (the roofline's kernel); every other launch runs
-    # as it does for a caller that profiles nothing.
-    L.rmetis_hip_profile_filter(0, g.n)
-    L.rmetis_hip_profile_enable(1)
-    sync()
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        step()
-    sync()
-    elapsed = time.perf_counter() - t_start
-    log(f"timed {args.steps} steps in {elapsed:.3f}s cut={obj.value}")
-    g_launch, g_ms, g_bytes = prof(0, g.n)
-
-    # One more step, untimed, with every kernel family bracketed and the
-    # per-level statistics (which synchronise once per level) taken.
-    L.rmetis_hip_profile_filter(-1, -1)
-    L.rmetis_hip_profile_enable(1)
-    step(with_stats=True)
-    sync()
-    prof_tags = {"gain": 0, "gain_inc": 6, "compact": 1, "admit": 2, "match": 3, "contract": 4}
-    kernel_ms = {name: round(prof(t)[1], 3) for name, t in prof_tags.items()}
-    L.rmetis_hip_profile_enable(0)
-    avg_ms = g_ms / max(g_launch, 1)
-    bytes_per_launch = g_bytes / max(g_launch, 1)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-
+    total, finest = profiled_step(L, step, g.n)
+    tag = args.roofline_tag or dominant(total)
+    # the roofline kernel at the finest level only: its launches there are
+    # the full-size ones
+    elapsed, ln, ms, by = timed_steps(L, step, steps, tag, g.n)
+    log(f"timed {steps} steps in {elapsed:.3f}s cut={obj.value}")
     part = d_part.numpy()
     pw = np.bincount(part, weights=(g.vwgt if g.vwgt is not None else None), minlength=k)
-    total = float(g.vwgt.sum()) if g.vwgt is not None else float(g.n)
-    maxw = math.floor(total / k * 1.03)
-    violations = int((pw > maxw).sum())
-
+    tot = float(g.vwgt.sum()) if g.vwgt is not None else float(g.n)
+    violations = int((pw > math.floor(tot / k * 1.03)).sum())
     levels = []
     for i in range(stats.nlevels):
         n_i, m_i = stats.level_n[i], stats.level_m[i]
         cm, rm, ps = stats.coarsen_ms[i], stats.refine_ms[i], stats.passes[i]
-        levels.append({"n": n_i, "m": m_i, "coarsen_ms": round(cm, 3), "refine_ms": round(rm, 3),
-                       "passes": ps,
+        levels.append({"n": n_i, "m": m_i, "coarsen_ms": round(cm, 3), "refine_ms": round(rm, 3), "passes": ps,
                        "coarsen_edges_per_s": round(m_i / (cm * 1e-3), 1) if cm > 0 else None,
                        "refine_edges_per_s_per_pass": round(m_i * ps / (rm * 1e-3), 1) if rm > 0 else None})
-
-    # the same partition through the host-buffer drop-in entry point (copies
-    # the CSR in and the partition out over PCIe on every call)
-    host_api = None
+    value = steps / elapsed
+    out = {
+        "metric": "graphs/s (k-way partitions of one graph per call)",
+        "value": round(value, 4), "unit": "graphs/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(1e3 * elapsed / steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"{workload}: {g.name} n={g.n} m={g.m} k={k}", "ctype": "shem", "rtype": "greedy",
+                   "ufactor": 30, "niter": 10, "ncuts": 1, "seed": args.seed, "parallelism": "single GPU"},
+        "edge_cut": int(obj.value), "balance_violations": violations, "edges_per_s": round(g.m * value, 1),
+        "kernel_ms_per_step": {n: round(v, 3) for n, v in total.items()},
+        "level_kernels_finest": level_kernels(finest),
+        "levels": levels,
+        "roofline": roofline(tag, ln, ms, by, workload),
+    }
+    if extra:
+        return out
     if not args.no_host_api:
+        # the same partition through the host-buffer drop-in entry point
         nv, nc, npart = C.c_int32(g.n), C.c_int32(g.ncon), C.c_int32(k)
         i32 = _lib.i32p
         hx, ha = g.xadj.ctypes.data_as(i32), g.adjncy.ctypes.data_as(i32)
@@ -165,7 +323,7 @@ def main():
         haw = g.adjwgt.ctypes.data_as(i32) if g.adjwgt is not None else None
         hpart = np.zeros(g.n, np.int32)
         hobj = C.c_int32()
-        reps = max(1, min(args.steps, 3))
+        reps = max(1, min(steps, 3))
         t1 = time.perf_counter()
         for _ in range(reps):
             rc = L.RMETIS_PartGraphKway(C.byref(nv), C.byref(nc), hx, ha, hvw, None, haw, C.byref(npart), None,
@@ -173,84 +331,19 @@ def main():
             if rc != 1:
                 raise RuntimeError(f"RMETIS_PartGraphKway failed rc={rc}: {_lib.last_error()}")
         t_host = time.perf_counter() - t1
-        host_api = {"value": round(reps / t_host, 4), "unit": "graphs/s", "calls": reps,
-                    "note": "RMETIS_PartGraphKway on host buffers: CSR upload, validation and the "
-                            "partition download included"}
-        log(f"host API: {reps} calls in {t_host:.3f}s cut={hobj.value}")
-
-    cpu = None
+        out["host_api"] = {"value": round(reps / t_host, 4), "unit": "graphs/s", "calls": reps,
+                           "note": "RMETIS_PartGraphKway on host buffers: CSR upload, validation and the "
+                                   "partition download included"}
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as O
-        O.build()
-        runs, t_cpu = 0, 0.0
-        while t_cpu < args.cpu_budget_s and runs < 10:
-            t1 = time.perf_counter()
-            rc, cobj, _ = O.part_graph_kway(g.xadj, g.adjncy, k, vwgt=g.vwgt, adjwgt=g.adjwgt, ncon=g.ncon,
-                                            seed=args.seed + runs, niter=10, ufactor=30)
-            t_cpu += time.perf_counter() - t1
-            runs += 1
-            log(f"cpu oracle run {runs}: cut={cobj} ({t_cpu:.1f}s total)")
-        cpu = {"value": round(runs / t_cpu, 4), "unit": "graphs/s", "cores": 1, "kind": "port",
-               "sample": f"{runs} full k-way partitions of the same graph (seeds {args.seed}..{args.seed + runs - 1}) "
-                         f"by the oracle's C++ restatement of the reference, one host core, {t_cpu:.1f}s",
-               "cut": int(cobj)}
-
-    # HBM bytes per launch of the same kernel from the committed rocprofv3
-    # FETCH_SIZE / WRITE_SIZE passes (scaled by a copy kernel of known size)
-    traffic, traffic_src = None, None
-    tpath = os.path.join(ROOT, "profiles", f"k_gain_tile_{args.workload}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            tj = json.load(fh)
-        if tj.get("level_n") == g.n:
-            traffic, traffic_src = tj["traffic_bytes"], tj["summary"]
-
-    value = args.steps / elapsed
-    out = {
-        "metric": "graphs/s (k-way partitions of one graph per second)",
-        "value": round(value, 4),
-        "unit": "graphs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {g.name} n={g.n} m={g.m} k={k}",
-                   "ctype": "shem", "rtype": "greedy", "ufactor": 30, "niter": 10, "ncuts": 1,
-                   "seed": args.seed, "parallelism": "single GPU"},
-        "edge_cut": int(obj.value),
-        "balance_violations": violations,
-        "edges_per_s": round(g.m * value, 1),
-        "kernel_ms_per_step": kernel_ms,
-        "levels": levels,
-        "roofline": {"bound": "hbm", "kernel": "k_gain (finest level)", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "traffic_source": traffic_src, "bytes_per_launch": int(bytes_per_launch),
-                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": g_launch},
-        "cpu_baseline": cpu,
-        "host_api": host_api,
-    }
-    print(json.dumps(out), flush=True)
-
-
-def _prof(L, tag, size=-1):
-    ln, ms, by = C.c_int64(), C.c_double(), C.c_double()
-    L.rmetis_hip_profile_read(tag, size, C.byref(ln), C.byref(ms), C.byref(by))
-    return ln.value, ms.value, by.value
+        kw = dict(vwgt=g.vwgt, adjwgt=g.adjwgt, ncon=g.ncon, niter=10, ufactor=30)
+        jobs = [lambda O, s=s: O.part_graph_kway(g.xadj, g.adjncy, k, seed=s, **kw) for s in range(1, 9)]
+        out["cpu_baseline"] = cpu_baseline(jobs, args.cpu_budget_s, args.cpu_pool_s,
+                                           f"full k-way partitions of the same graph, seeds 1..8")
+    return out
 
 
 def bench_batch(args):
-    """The uneven batch: `count` graphs (n log-uniform in 2^10..2^18, grids
-    and RGGs, vwgt U{1..4}, adjwgt U{1..10}, k from {2..64}), 8 attempts per
-    graph (seeds s..s+7), all packed into one segmented CSR resident in HBM
-    and partitioned by one rmetis_hip_kway_batch_device call per step.  The
-    plain path (one rmetis_hip_kway_device call per graph, attempts one after
-    another) is timed on every plain_stride-th graph of the same batch."""
+    """The uneven batch (see the module docstring), one batched call per step."""
     import numpy as np
     import rmetis_amd as R
     from rmetis_amd import _lib, synth
@@ -258,11 +351,10 @@ def bench_batch(args):
     L = _lib.lib()
     DA = _lib.DeviceArray
     i32 = _lib.i32p
-    sync = L.rmetis_hip_device_synchronize
     count = int(args.workload[5:] or 1024)
     ncuts = 8
     t0 = time.time()
-    workers = max(1, min(16, os.cpu_count() or 1))
+    workers = host_threads()
     batch = synth.uneven_batch(count, args.seed, workers=workers)
     graphs = [g for g, _ in batch]
     ks = np.array([k for _, k in batch], np.int32)
@@ -271,7 +363,7 @@ def bench_batch(args):
     log(f"batch of {count} graphs: n={n} m={m} generated and packed in {time.time() - t0:.1f}s ({workers} workers)")
     d_x, d_a, d_vw, d_aw = DA(xadj), DA(adjncy), DA(vwgt), DA(adjwgt)
     d_part = DA(count=n, dtype=np.int32)
-    sync()
+    L.rmetis_hip_device_synchronize()
     opts = R.Options(rtype=R.RType.Greedy, seed=args.seed, ncuts=ncuts, niter=10, ufactor=30).to_c_array()
     obj = np.zeros(count, np.int32)
     stats = _lib.HipStats()
@@ -288,30 +380,12 @@ def bench_batch(args):
         t1 = time.perf_counter()
         step()
         log(f"warmup {i}: {time.perf_counter() - t1:.2f}s")
-    # timed region: events only around the finest level's full gain launches
-    L.rmetis_hip_profile_filter(0, n)
-    L.rmetis_hip_profile_enable(1)
-    sync()
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        step()
-        log(f"step {i} done")
-    sync()
-    elapsed = time.perf_counter() - t_start
+    total, finest = profiled_step(L, step, n)
+    tag = args.roofline_tag or dominant(total)
+    log(f"kernel ms per step {dict((k_, round(v, 1)) for k_, v in total.items())}; roofline kernel: {tag}")
+    elapsed, ln, ms, by = timed_steps(L, step, args.steps, tag, n)
     value = count * args.steps / elapsed
     log(f"batched: {args.steps} steps in {elapsed:.2f}s = {value:.2f} graphs/s")
-    g_launch, g_ms, g_bytes = _prof(L, 0, n)
-    # one untimed step with every kernel family timed and the level statistics
-    L.rmetis_hip_profile_filter(-1, -1)
-    L.rmetis_hip_profile_enable(1)
-    step(with_stats=True)
-    sync()
-    prof_tags = {"gain": 0, "gain_inc": 6, "compact": 1, "admit": 2, "match": 3, "contract": 4}
-    kernel_ms = {name: round(_prof(L, t)[1], 2) for name, t in prof_tags.items()}
-    L.rmetis_hip_profile_enable(0)
-    avg_ms = g_ms / max(g_launch, 1)
-    bytes_per_launch = g_bytes / max(g_launch, 1)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     part = d_part.numpy()
     bobj = obj.copy()
     # balance: every part of every graph within floor(total/k * 1.03)
@@ -327,13 +401,12 @@ def bench_batch(args):
                               "limit": lim})
     levels = [{"n": stats.level_n[i], "m": stats.level_m[i], "coarsen_ms": round(stats.coarsen_ms[i], 3),
                "refine_ms": round(stats.refine_ms[i], 3)} for i in range(stats.nlevels)]
-    # the plain path on a sample of the same graphs, same options
+    # the plain path (one graph, one attempt after another per call) on a
+    # sample of the same graphs, same options
     sample = list(range(0, count, max(1, args.plain_stride)))
-    dev = []
-    for i in sample:
-        g = graphs[i]
-        dev.append((DA(g.xadj), DA(g.adjncy), DA(g.vwgt), DA(g.adjwgt), DA(count=g.n, dtype=np.int32)))
-    sync()
+    dev = [(DA(graphs[i].xadj), DA(graphs[i].adjncy), DA(graphs[i].vwgt), DA(graphs[i].adjwgt),
+            DA(count=graphs[i].n, dtype=np.int32)) for i in sample]
+    L.rmetis_hip_device_synchronize()
     pobj = C.c_int32()
 
     def plain(j):
@@ -347,37 +420,21 @@ def bench_batch(args):
         return pobj.value
 
     plain(0)
-    sync()
+    L.rmetis_hip_device_synchronize()
     mism = 0
     t1 = time.perf_counter()
     for j in range(len(sample)):
         if plain(j) != bobj[sample[j]]:
             mism += 1
-    sync()
+    L.rmetis_hip_device_synchronize()
     t_plain = time.perf_counter() - t1
     plain_value = len(sample) / t_plain
     log(f"plain: {len(sample)} graphs in {t_plain:.2f}s = {plain_value:.2f} graphs/s, objval mismatches {mism}")
-    cpu = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as O
-        O.build()
-        runs, t_cpu = 0, 0.0
-        for i in sample:
-            if t_cpu >= args.cpu_budget_s:
-                break
-            g = graphs[i]
-            t2 = time.perf_counter()
-            O.part_graph_kway(g.xadj, g.adjncy, int(ks[i]), vwgt=g.vwgt, adjwgt=g.adjwgt, ncuts=ncuts,
-                              seed=args.seed, niter=10, ufactor=30)
-            t_cpu += time.perf_counter() - t2
-            runs += 1
-        cpu = {"value": round(runs / t_cpu, 4), "unit": "graphs/s", "cores": 1, "kind": "port",
-               "sample": f"the first {runs} graphs of the plain-path sample, {ncuts} attempts each, by the "
-                         f"oracle's C++ restatement of the reference on one host core, {t_cpu:.1f}s"}
-        log(f"cpu oracle: {runs} graphs in {t_cpu:.1f}s")
+    for d in dev:
+        for a in d:
+            a.free()
     out = {
-        "metric": "graphs/s (k-way partitions of the uneven batch, 8 attempts per graph)",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "graphs/s",
         "n_gpus": 1,
@@ -389,26 +446,36 @@ def bench_batch(args):
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {count} graphs, n={n} m={m}, k in {{2..64}}",
+        "config": {"workload": f"uneven batch of {count} graphs (n log-uniform 2^10..2^18, grids and RGGs, "
+                               f"vwgt U{{1..4}}, adjwgt U{{1..10}}, k in {{2..64}}), {ncuts} seeds each, one "
+                               f"segmented CSR: n={n} m={m}",
                    "ctype": "shem", "rtype": "greedy", "ufactor": 30, "niter": 10, "ncuts": ncuts,
                    "seed": args.seed, "parallelism": "single GPU, one segmented CSR"},
         "plain": {"value": round(plain_value, 3), "unit": "graphs/s", "graphs": len(sample),
                   "objval_mismatches": mism,
-                  "note": "rmetis_hip_kway_device per graph (device-resident CSR), same options"},
+                  "note": "rmetis_hip_kway_device per graph (device-resident CSR, the 8 attempts one after "
+                          "another), same options, every plain_stride-th graph"},
         "batched_vs_plain": round(value / plain_value, 2),
         "edge_cut_sum": int(bobj.sum()),
         "balance_violations": violations,
         "balance_violators": violators[:16],
         "edges_per_s": round(m * ncuts * value / count, 1),
-        "kernel_ms_per_step": kernel_ms,
+        "kernel_ms_per_step": {k_: round(v, 2) for k_, v in total.items()},
+        "level_kernels_finest": level_kernels(finest),
         "levels": levels,
-        "roofline": {"bound": "hbm", "kernel": "k_gain (finest union level)", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": int(bytes_per_launch),
-                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": g_launch},
-        "cpu_baseline": cpu,
+        "roofline": roofline(tag, ln, ms, by, args.workload),
     }
-    print(json.dumps(out), flush=True)
+    if not args.no_cpu:
+        def job(i):
+            g = graphs[i]
+            return lambda O: O.part_graph_kway(g.xadj, g.adjncy, int(ks[i]), vwgt=g.vwgt, adjwgt=g.adjwgt,
+                                               ncuts=ncuts, seed=args.seed, niter=10, ufactor=30)
+        out["cpu_baseline"] = cpu_baseline([job(i) for i in sample], args.cpu_budget_s, args.cpu_pool_s,
+                                           f"graphs of the plain-path sample, {ncuts} attempts each")
+        log(f"cpu baseline: {out['cpu_baseline']['value']} graphs/s on {out['cpu_baseline']['cores']} threads")
+    if not args.no_extra:
+        out["rgg20"] = bench_single(args, "rgg20", extra=True)
+    return out
 
 
 def bench_serve(args):
@@ -557,7 +624,7 @@ def bench_serve(args):
                   "note": "rmetis_hip_kway_device one request at a time, same requests"},
         "capacity_vs_plain": round(capacity / plain_value, 2),
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

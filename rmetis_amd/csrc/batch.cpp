@@ -221,7 +221,7 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
             gs.gon = put(cx, L.on);
             gs.cgseg = cx.arena.get<int32_t>(L.g.n);
             {
-                ProfScope ps(kProfMatch, 0.0, L.g.n, s);
+                ProfScope ps(kProfMatch, match_bytes(L.g.n, L.g.m, L.g.adjwgt != nullptr), L.g.n, s);
                 match_graph(L.g, o.ctype_shem, 0, B.cb, s, &gs);
             }
             BLevel C;
@@ -231,6 +231,7 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
             {
                 ProfScope ps(kProfContract, 0.0, L.g.n, s);
                 cr = contract(L.g, B.cb.match, C.cmap, C.g, false, B.cb, alloc, C.heavy, s, &gs);
+                ps.bytes = contract_bytes(L.g.n, L.g.m, ncon, L.g.adjwgt != nullptr, cr.cn, cr.cm);
             }
             C.nheavy = cr.nheavy;
             C.gseg = gs.cgseg;

@@ -373,7 +373,7 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
                 RK_HIP(hipEventRecord(e0, s));
             }
             {
-                ProfScope ps(kProfMatch, 0.0, cur->n, s);
+                ProfScope ps(kProfMatch, match_bytes(cur->n, cur->m, cur->adjwgt != nullptr), cur->n, s);
                 match_graph(*cur, o.ctype_shem, rng.next_u64(), B.cb, s);
             }
             int32_t* cmap = cx.arena.get<int32_t>(cur->n);
@@ -383,6 +383,7 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
             {
                 ProfScope ps(kProfContract, 0.0, cur->n, s);
                 cr = contract(*cur, B.cb.match, cmap, cg, false, B.cb, alloc, heavy, s);   // one sync
+                ps.bytes = contract_bytes(cur->n, cur->m, ncon, cur->adjwgt != nullptr, cr.cn, cr.cm);
             }
             if (st) {
                 LevelStat ls;

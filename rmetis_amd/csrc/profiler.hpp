@@ -117,6 +117,19 @@ private:
     std::map<std::pair<int, long long>, Acc> acc_;
 };
 
+// Algorithmic HBM bytes of one level's matching: xadj, adjncy, adjwgt and
+// the neighbour-state gather at 4 B per entry, the match write.
+inline double match_bytes(long long n, long long m, bool weighted) {
+    return 4.0 * (n + 1) + (weighted ? 12.0 : 8.0) * m + 4.0 * n;
+}
+// ... and of its contraction into n' vertices and m' entries: the coarse
+// numbering (leader flags and cmap, read and written), the fine CSR with the
+// cmap[adjncy] gather, cmap and vertex weights, the coarse CSR and weights.
+inline double contract_bytes(long long n, long long m, int ncon, bool weighted, long long cn, long long cm) {
+    return 8.0 * n + 4.0 * (n + 1) + (weighted ? 12.0 : 8.0) * m + 4.0 * n + 4.0 * n * ncon + 4.0 * (cn + 1) +
+           8.0 * cm + 4.0 * cn * ncon;
+}
+
 // Times the launches issued during its lifetime when profiling is on.
 struct ProfScope {
     int tag;
