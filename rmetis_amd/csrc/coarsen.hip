@@ -40,10 +40,14 @@ __device__ inline uint32_t mix32(uint32_t x) {
 }
 
 // Strict total order on edges, symmetric in (v, u).  Heavy-edge matching:
-// (smaller endpoint degree first, heavier edge, seeded hash) -- an edge is
-// ranked by when the reference's degree-ordered scan (shem.rs:23) would
-// first reach it and by the weight its visiting endpoint maximises
-// (shem.rs:36-44).  Random matching: the seeded hash alone.
+// (smaller floor(log2) of the endpoints' smaller degree first, heavier
+// edge, seeded hash) -- an edge is ranked by roughly when the reference's
+// degree-ordered scan (shem.rs:23) would first reach it and by the weight its
+// visiting endpoint maximises (shem.rs:36-44).  Degrees are compared by
+// their binary magnitude, not exactly: exact degrees chain the greedy
+// matching's decisions along long paths of slowly rising degree (the
+// matching then takes many dependent steps), while magnitudes keep almost
+// all of its size.  Random matching: the seeded hash alone.
 struct EdgeKey {
     uint64_t hi;
     uint32_t lo;
@@ -56,7 +60,7 @@ __device__ inline EdgeKey edge_key_m(int32_t v, int32_t u, int32_t w, int32_t md
     const uint32_t lo = (uint32_t)min(v, u), hi = (uint32_t)max(v, u);
     const uint32_t h = mix32(lo ^ mix32(hi ^ (uint32_t)seed) ^ (uint32_t)(seed >> 32));
     if (!shem) return EdgeKey{0ull, h};
-    const uint32_t dpri = 0xFFFFFFFFu - (uint32_t)mdeg;
+    const uint32_t dpri = 0xFFFFFFFFu - (uint32_t)(31 - __clz(max(mdeg, 1)));
     return EdgeKey{((uint64_t)dpri << 32) | (uint64_t)((uint32_t)w ^ 0x80000000u), h};
 }
 

@@ -440,9 +440,9 @@ def _mix32(x):
 def _greedy_matching(x, a, aw, seed, shem):
     """The matching the GPU computes, by its definition: the sequential greedy
     matching over the edges in descending order of (key, then the smaller and
-    larger endpoint ascending), key = (2^32-1 minus the smaller endpoint
-    degree, edge weight, seeded lowbias32 hash) for heavy-edge matching and the
-    hash alone for random matching."""
+    larger endpoint ascending), key = (2^32-1 minus floor(log2) of the smaller
+    endpoint degree, edge weight, seeded lowbias32 hash) for heavy-edge
+    matching and the hash alone for random matching."""
     n = len(x) - 1
     deg = np.diff(x)
     edges = []
@@ -453,7 +453,7 @@ def _greedy_matching(x, a, aw, seed, shem):
                 w = int(aw[j]) if aw is not None else 1
                 h = _mix32(v ^ _mix32(u ^ (seed & _M32)) ^ ((seed >> 32) & _M32))
                 if shem:
-                    hi = ((_M32 - min(int(deg[v]), int(deg[u]))) << 32) | ((w & _M32) ^ 0x80000000)
+                    hi = ((_M32 - (min(int(deg[v]), int(deg[u])).bit_length() - 1)) << 32) | ((w & _M32) ^ 0x80000000)
                 else:
                     hi = 0
                 edges.append((-hi, -h, v, u))
@@ -480,6 +480,63 @@ def test_gpu_matching_is_the_greedy_matching(shem):
     g2 = _graph(x2, a2, adjwgt=aw2)
     ctype = R.CType.Shem if shem else R.CType.Rm
     assert np.array_equal(R.gpu_matching(g2, 7, ctype), _greedy_matching(x2, a2, aw2, 7, shem))
+
+
+def _greedy_matching_np(x, a, aw, seed, shem):
+    """_greedy_matching with the keys and the edge order computed by numpy
+    (for levels large enough to take the handshake rounds)."""
+    n = len(x) - 1
+    deg = np.diff(x).astype(np.int64)
+    src = np.repeat(np.arange(n, dtype=np.int64), deg)
+    keep = src < a
+    v, u = src[keep], a[keep].astype(np.int64)
+    w = (aw[keep] if aw is not None else np.ones(len(v), np.int32)).astype(np.int64)
+
+    def mix(z):
+        z = z.astype(np.uint64) & np.uint64(_M32)
+        z ^= z >> np.uint64(16)
+        z = (z * np.uint64(0x7FEB352D)) & np.uint64(_M32)
+        z ^= z >> np.uint64(15)
+        z = (z * np.uint64(0x846CA68B)) & np.uint64(_M32)
+        z ^= z >> np.uint64(16)
+        return z
+
+    lo, hi_s = np.uint64(seed & _M32), np.uint64((seed >> 32) & _M32)
+    h = mix(v.astype(np.uint64) ^ mix(u.astype(np.uint64) ^ lo) ^ hi_s)
+    if shem:
+        md = np.minimum(deg[v], deg[u])
+        lg = np.zeros(len(md), np.uint64)                   # floor(log2(md)), md >= 1 on every edge
+        for b in range(1, 32):
+            lg[md >= (1 << b)] = b
+        hi = ((np.uint64(_M32) - lg) << np.uint64(32)) | ((w.astype(np.uint64) & np.uint64(_M32)) ^ np.uint64(0x80000000))
+    else:
+        hi = np.zeros(len(v), np.uint64)
+    order = np.lexsort((u, v, ~h, ~hi))
+    m = np.arange(n, dtype=np.int32)
+    free = np.ones(n, bool)
+    for e in order.tolist():
+        a_, b_ = int(v[e]), int(u[e])
+        if free[a_] and free[b_]:
+            m[a_], m[b_] = b_, a_
+            free[a_] = free[b_] = False
+    return m
+
+
+@pytest.mark.parametrize("shem", [1, 0])
+def test_gpu_matching_large_levels_handshake_then_suitor(shem):
+    # levels of 2^16 vertices and more run handshake rounds before the suitor
+    # chains; the result is still the greedy matching in edge-key order
+    from rmetis_amd import synth
+    ctype = R.CType.Shem if shem else R.CType.Rm
+    cases = [synth.rgg(70000, 12), synth.grid(300, 240)]
+    xr, ar, awr = random_graph(66000, 0.00012, 8, wmax=2)       # many equal keys, isolated vertices
+    for s in cases:
+        g = R.Graph(s.n, 1, s.xadj, s.adjncy, s.vwgt, s.adjwgt)
+        for seed in (5, 0xABCDEF0123456789):
+            assert np.array_equal(R.gpu_matching(g, seed, ctype), _greedy_matching_np(s.xadj, s.adjncy, s.adjwgt,
+                                                                                    seed, shem))
+    g = _graph(xr, ar, adjwgt=awr)
+    assert np.array_equal(R.gpu_matching(g, 9, ctype), _greedy_matching_np(xr, ar, awr, 9, shem))
 
 
 @pytest.mark.parametrize("shem", [1, 0])
