@@ -138,12 +138,14 @@ int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, cons
 
 /* Serving (no counterpart in the reference): a request loop with continuous
  * batching.  Each request is one graph with its own nparts and seed; a
- * server runs up to `slots` (1..3) batched calls at once (one HIP stream each),
+ * server runs up to `slots` (1..8) batched calls at once (one HIP stream each),
  * every idle slot taking the oldest pending request plus the pending
  * requests that can share its batch (same ncon, ubvec and options but the
  * seed) within the device-memory budget (<= 0: 80 % of the free memory at
- * creation) and max_batch_vertices (<= 0: 2^25).  A request's result equals
- * that of RMETIS_PartGraphKway with its seed, whatever it was batched with.
+ * creation) and max_batch_vertices (<= 0: 2^25).  Device memory is taken in
+ * segments of at least 1 GiB as batches need it, up to the budget, and kept
+ * until the server is destroyed.  A request's result equals that of
+ * RMETIS_PartGraphKway with its seed, whatever it was batched with.
  * Returns NULL on error (rmetis_hip_last_error()). */
 typedef struct rmetis_hip_server rmetis_hip_server;
 rmetis_hip_server* rmetis_hip_server_create(int32_t slots, int64_t memory_budget, int64_t max_batch_vertices);
@@ -167,9 +169,12 @@ int64_t rmetis_hip_server_submit_device(rmetis_hip_server* server, idx_t nvtxs, 
 
 /* Result of request id: METIS_OK with part[nvtxs] (host requests; may be
  * NULL, and is not written for device requests) and objval written (and
- * the request forgotten), 0 while it is pending (wait = 0), or its METIS
- * error code.  times_ms[3] (may be NULL) = queued, running, total
- * milliseconds; *batch_graphs = graphs of the batched call that ran it. */
+ * the request forgotten), 0 while it is pending, or its METIS error code.
+ * wait = 0 returns at once, 1 waits until the request is done, and a larger
+ * value waits at most that many milliseconds, then returns METIS_ERROR with
+ * every slot's phase in rmetis_hip_last_error() (the request stays queued).
+ * times_ms[3] (may be NULL) = queued, running, total milliseconds;
+ * *batch_graphs = graphs of the batched call that ran it. */
 int32_t rmetis_hip_server_result(rmetis_hip_server* server, int64_t id, int32_t wait, idx_t* part, idx_t* objval,
                                  double* times_ms, int32_t* batch_graphs);
 

@@ -422,16 +422,18 @@ class Server:
         self._n[rid] = None
         return rid
 
-    def result(self, rid, wait=True):
+    def result(self, rid, wait=True, timeout_ms=None):
         """ServedResult (part None for device requests), or None while
-        pending (wait=False)."""
+        pending (wait=False).  With timeout_ms, waits at most that long and
+        then raises InternalError naming every slot's phase."""
         if rid not in self._n:
             raise InputError(f"unknown or already collected request id {rid}")
         part = np.zeros(self._n[rid], np.int32) if self._n[rid] is not None else None
         obj = C.c_int32()
         t = np.zeros(3, np.float64)
         bg = C.c_int32()
-        rc = _lib.lib().rmetis_hip_server_result(self._h, int(rid), int(bool(wait)), _p(part), C.byref(obj),
+        w = 0 if not wait else (max(2, int(timeout_ms)) if timeout_ms else 1)
+        rc = _lib.lib().rmetis_hip_server_result(self._h, int(rid), w, _p(part), C.byref(obj),
                                                  _p(t, _lib.f64p), C.byref(bg))
         if rc == 0:
             return None

@@ -211,6 +211,7 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
                 RK_HIP(hipEventRecord(e0, s));
             }
             BLevel& L = lv[l];
+            cx.set_phase(kPhaseCoarsen, (int32_t)l);
             std::vector<uint64_t> gseed(G, 0);
             for (int32_t i = 0; i < G; ++i)
                 if (L.on[i]) gseed[i] = rng[i].next_u64();
@@ -273,6 +274,7 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
         }
         const size_t nl = lv.size();
         // ---- coarsest graphs to the host, grown there (kway.rs:66-70)
+        cx.set_phase(kPhaseInitial);
         auto t0 = std::chrono::steady_clock::now();
         std::vector<int32_t> lvl_of(G, -1);
         std::vector<int64_t> csr_off(G, 0), part_off(G, 0);
@@ -346,6 +348,7 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
         std::vector<LevelStat> rst(nl);
         for (int64_t l = (int64_t)nl - 1; l >= 0; --l) {
             BLevel& L = lv[l];
+            cx.set_phase(kPhaseRefine, (int32_t)l);
             int32_t* pf = cx.arena.get<int32_t>(L.g.n);
             if (l + 1 < (int64_t)nl) project(L.g.n, lv[l + 1].cmap, pc, pf, s);
             std::vector<int32_t> goff(G, -1);
@@ -434,6 +437,7 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
             pc = pf;
         }
         // ---- per-graph cut (kway.rs:94) and the strictly better attempts (kway.rs:96-98)
+        cx.set_phase(kPhaseFinish);
         edge_cut_batch(g, pc, gseg0, G, d_cutacc, d_cut, s);
         int32_t* hc = (int32_t*)cx.host_stage2(sizeof(int32_t) * (size_t)G);
         RK_HIP(hipMemcpyAsync(hc, d_cut, sizeof(int32_t) * (size_t)G, hipMemcpyDeviceToHost, s));

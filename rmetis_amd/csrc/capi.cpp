@@ -291,7 +291,7 @@ int32_t rmetis_hip_kway_device_noalloc(idx_t nvtxs, idx_t ncon, const idx_t* d_x
         cx.device = dev;
         cx.stream = nullptr;   // the null stream: no stream is created
         cx.pageable = true;
-        cx.arena.attach(ws, kScratchBytes);
+        cx.arena.attach(ws, kScratchBytes, true);
         int32_t obj = 0;
         try {
             obj = kway_device(cx, g, nparts, tpwgts, ubvec, o, d_part, nullptr);
@@ -378,8 +378,6 @@ int32_t rmetis_hip_kway_batch(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, cons
 rmetis_hip_server* rmetis_hip_server_create(int32_t slots, int64_t memory_budget, int64_t max_batch_vertices) {
     rmetis_hip_server* out = nullptr;
     const int32_t rc = guarded([&]() -> int32_t {
-        // at most kMaxServeSlots: a run with 4 slots stalled (cause not
-        // found), so larger servers are refused rather than run
         if (slots < 1) return METIS_ERROR_INPUT;
         if (slots > kMaxServeSlots) throw Error(METIS_ERROR_INPUT, "a server has at most " +
                                                                std::to_string(kMaxServeSlots) + " slots");
@@ -480,7 +478,8 @@ int32_t rmetis_hip_server_result(rmetis_hip_server* server, int64_t id, int32_t 
                                  double* times_ms, int32_t* batch_graphs) {
     return guarded([&]() -> int32_t {
         if (!server) return METIS_ERROR_INPUT;
-        std::shared_ptr<ServeRequest> r = server->s->result(id, wait != 0);
+        // wait: 0 = poll, 1 = until done, > 1 = at most that many milliseconds
+        std::shared_ptr<ServeRequest> r = server->s->result(id, wait == 0 ? 0 : (wait == 1 ? -1 : (int64_t)wait));
         if (!r) return 0;
         if (r->status != METIS_OK) throw Error(r->status, r->error);
         if (part && !r->on_device) std::copy(r->part.begin(), r->part.end(), part);

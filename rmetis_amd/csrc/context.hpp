@@ -4,6 +4,8 @@
 // reference's entry points are reentrant, c_api.rs:55-81) and repeated calls
 // reuse their memory instead of calling hipMalloc again.
 #pragma once
+#include <algorithm>
+#include <atomic>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -19,12 +21,21 @@ public:
         size_t chunk, off;
     };
     ~Arena() { release(); }
-    // Carves every allocation from [base, base + bytes), memory the arena
-    // does not own; running out is then an error, never a hipMalloc.
-    void attach(void* base, size_t bytes) {
-        release();
-        chunks_.push_back({(char*)base, bytes});
-        external_ = true;
+    // Puts [base, base + bytes), memory the arena does not own, first in its
+    // chain (replacing the previous borrowed range; owned chunks stay after
+    // it).  fixed: running past the borrowed range is an error instead of a
+    // hipMalloc of one more owned chunk.
+    void attach(void* base, size_t bytes, bool fixed) {
+        if (!chunks_.empty() && !chunks_[0].owned) chunks_.erase(chunks_.begin());
+        chunks_.insert(chunks_.begin(), Chunk{(char*)base, bytes, false});
+        fixed_ = fixed;
+        reset();
+    }
+    // Drops the borrowed range (owned chunks stay).
+    void detach() {
+        if (!chunks_.empty() && !chunks_[0].owned) chunks_.erase(chunks_.begin());
+        fixed_ = false;
+        reset();
     }
     void* alloc(size_t bytes) {
         bytes = (bytes + 255) & ~(size_t)255;
@@ -38,13 +49,13 @@ public:
             ++cur_;
             off_ = 0;
         }
-        if (external_)
+        if (fixed_)
             throw Error(-3, "the fixed device workspace (" + std::to_string(chunks_.empty() ? 0 : chunks_[0].size >> 20) +
                                 " MiB) is too small for this graph's hierarchy");
         size_t sz = bytes > kChunk ? bytes : kChunk;
         void* p = nullptr;
         RK_HIP(hipMalloc(&p, sz));
-        chunks_.push_back({(char*)p, sz});
+        chunks_.push_back({(char*)p, sz, true});
         cur_ = chunks_.size() - 1;
         off_ = bytes;
         return p;
@@ -52,20 +63,6 @@ public:
     template <typename T>
     T* get(int64_t count) {
         return (T*)alloc((size_t)(count > 0 ? count : 1) * sizeof(T));
-    }
-    // Makes the first chunk hold at least `bytes`, so that a call whose
-    // needs are known ahead takes one allocation instead of a growing list
-    // of chunks.  Frees the current chunks (the caller has synchronised its
-    // stream) only when they are too small.
-    void reserve(size_t bytes) {
-        bytes = (bytes + 255) & ~(size_t)255;
-        if (!chunks_.empty() && chunks_[0].size >= bytes) return;
-        release();
-        void* p = nullptr;
-        RK_HIP(hipMalloc(&p, bytes));
-        chunks_.push_back({(char*)p, bytes});
-        cur_ = 0;
-        off_ = 0;
     }
     Mark mark() const { return {cur_, off_}; }
     void rewind(Mark m) {
@@ -77,22 +74,17 @@ public:
         off_ = 0;
     }
     void release() {
-        if (!external_)
-            for (auto& c : chunks_) (void)hipFree(c.base);
+        for (auto& c : chunks_)
+            if (c.owned) (void)hipFree(c.base);
         chunks_.clear();
-        external_ = false;
+        fixed_ = false;
         reset();
     }
-    // Bytes handed out so far (the high-water mark of the current chunk
-    // chain up to the cursor).
-    size_t used() const {
+    // Bytes of the chunks the arena allocated itself (not borrowed ranges).
+    size_t owned_bytes() const {
         size_t t = 0;
-        for (size_t i = 0; i < cur_ && i < chunks_.size(); ++i) t += chunks_[i].size;
-        return t + off_;
-    }
-    size_t reserved() const {
-        size_t t = 0;
-        for (auto& c : chunks_) t += c.size;
+        for (auto& c : chunks_)
+            if (c.owned) t += c.size;
         return t;
     }
 
@@ -101,16 +93,23 @@ private:
     struct Chunk {
         char* base;
         size_t size;
+        bool owned;
     };
     std::vector<Chunk> chunks_;
     size_t cur_ = 0, off_ = 0;
-    bool external_ = false;
+    bool fixed_ = false;
 };
 
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     Arena arena;
+    // Progress of the call for observers (a server slot's status): phase *
+    // 1000 + level, see kPhase*.  Null when nobody watches.
+    std::atomic<int32_t>* phase = nullptr;
+    void set_phase(int32_t p, int32_t level = 0) {
+        if (phase) phase->store(p * 1000 + level, std::memory_order_relaxed);
+    }
     // pageable: host staging is ordinary heap memory instead of pinned
     // hipHostMalloc buffers (for callers that allow no HIP allocation)
     bool pageable = false;
@@ -173,13 +172,16 @@ struct Context {
     std::vector<std::pair<char*, size_t>> up_chunks;
     size_t up_cur = 0, up_off = 0;
 
+    // Grows a staging buffer (at least doubling it).  The old buffer is not
+    // freed until the context ends: a hipHostFree waits for every stream of
+    // the device, and other calls may be running on theirs.
     void* grow(void*& p, size_t& cap, size_t bytes) {
         if (bytes > cap) {
-            size_t sz = bytes < (1u << 20) ? (1u << 20) : bytes;
+            size_t sz = std::max<size_t>(bytes, std::max<size_t>(2 * cap, 1u << 20));
             if (pageable) {
                 p = heap_block(sz);     // the old block stays owned by `heap`
             } else {
-                if (p) (void)hipHostFree(p);
+                if (p) retired.push_back(p);
                 p = nullptr;
                 RK_HIP(hipHostMalloc(&p, sz, hipHostMallocDefault));
             }
@@ -187,16 +189,28 @@ struct Context {
         }
         return p;
     }
+    std::vector<void*> retired;   // outgrown pinned buffers
     ~Context() {
         if (!pageable) {
             if (pinned) (void)hipHostFree(pinned);
             if (pinned2) (void)hipHostFree(pinned2);
             if (small) (void)hipHostFree(small);
             for (auto& c : up_chunks) (void)hipHostFree(c.first);
+            for (void* r : retired) (void)hipHostFree(r);
         }
         arena.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
+};
+
+// Phases reported through Context::phase.
+enum : int32_t {
+    kPhaseIdle = 0,
+    kPhasePack = 1,       // building the batch's union CSR
+    kPhaseCoarsen = 2,    // level = the level being matched and contracted
+    kPhaseInitial = 3,    // coarsest graphs to the host, grown there, back
+    kPhaseRefine = 4,     // level = the level being refined
+    kPhaseFinish = 5,     // cuts, best attempts, results to the caller
 };
 
 // Acquire a context for the current device (creates one when the pool is

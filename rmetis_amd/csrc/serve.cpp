@@ -38,12 +38,98 @@ bool same_setup(const ServeRequest& a, const ServeRequest& b) {
            x.rtype_greedy == y.rtype_greedy && x.ncuts == y.ncuts && x.niter == y.niter && x.ufactor == y.ufactor;
 }
 
+constexpr int64_t kSegmentMin = 1ll << 30;   // device memory is taken in segments of 1 GiB or more
+
+const char* phase_name(int32_t p) {
+    switch (p) {
+        case kPhaseIdle: return "idle";
+        case kPhasePack: return "packing the batch";
+        case kPhaseCoarsen: return "coarsening level";
+        case kPhaseInitial: return "growing the coarsest graphs";
+        case kPhaseRefine: return "refining level";
+        case kPhaseFinish: return "finishing";
+        default: return "?";
+    }
+}
+
 }  // namespace
 
+DevicePool::~DevicePool() { release_all(); }
+
+bool DevicePool::fits_free(int64_t bytes) const {
+    for (const Seg& sg : segs_)
+        for (const auto& f : sg.free)
+            if (f.second >= bytes) return true;
+    return false;
+}
+
+bool DevicePool::can_take(int64_t bytes, int64_t extra) const {
+    bytes = round(bytes);
+    return fits_free(bytes) || held_ + extra + bytes <= budget_;
+}
+
+char* DevicePool::take(int64_t bytes, int64_t extra) {
+    bytes = round(bytes);
+    for (Seg& sg : segs_)
+        for (auto it = sg.free.begin(); it != sg.free.end(); ++it)
+            if (it->second >= bytes) {
+                const int64_t off = it->first, len = it->second;
+                sg.free.erase(it);
+                if (len > bytes) sg.free[off + bytes] = len - bytes;
+                taken_ += bytes;
+                return sg.base + off;
+            }
+    if (held_ + extra + bytes > budget_) return nullptr;
+    const int64_t room = budget_ - held_ - extra;
+    const int64_t size = std::min<int64_t>(room, std::max<int64_t>(bytes, kSegmentMin));
+    void* p = nullptr;
+    RK_HIP(hipMalloc(&p, (size_t)size));
+    segs_.push_back({(char*)p, size, {}});
+    held_ += size;
+    if (size > bytes) segs_.back().free[bytes] = size - bytes;
+    taken_ += bytes;
+    return (char*)p;
+}
+
+void DevicePool::give(char* p, int64_t bytes) {
+    bytes = round(bytes);
+    for (Seg& sg : segs_) {
+        if (p < sg.base || p >= sg.base + sg.size) continue;
+        int64_t off = p - sg.base, len = bytes;
+        auto next = sg.free.lower_bound(off);
+        if (next != sg.free.end() && next->first == off + len) {   // merge with the following range
+            len += next->second;
+            next = sg.free.erase(next);
+        }
+        if (next != sg.free.begin()) {                              // and with the preceding one
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                off = prev->first;
+                len += prev->second;
+                sg.free.erase(prev);
+            }
+        }
+        sg.free[off] = len;
+        taken_ -= bytes;
+        return;
+    }
+}
+
+void DevicePool::release_all() {
+    for (Seg& sg : segs_) (void)hipFree(sg.base);
+    segs_.clear();
+    held_ = 0;
+    taken_ = 0;
+}
+
 Server::Server(int device, int slots, int64_t budget, int64_t max_batch_vertices)
-    : device_(device), budget_(budget), max_batch_v_(max_batch_vertices) {
+    : device_(device), budget_(budget), max_batch_v_(max_batch_vertices), pool_(budget) {
     if (slots < 1) throw Error(-2, "a server needs at least one slot");
-    for (int i = 0; i < slots; ++i) threads_.emplace_back([this]() { worker(); });
+    for (int i = 0; i < slots; ++i) {
+        slots_.push_back(std::make_unique<Slot>());
+        slots_.back()->since = Clock::now();
+    }
+    for (int i = 0; i < slots; ++i) threads_.emplace_back([this, i]() { worker(i); });
 }
 
 Server::~Server() {
@@ -78,12 +164,18 @@ int64_t Server::submit(std::shared_ptr<ServeRequest> r) {
     return r->id;
 }
 
-std::shared_ptr<ServeRequest> Server::result(int64_t id, bool wait) {
+std::shared_ptr<ServeRequest> Server::result(int64_t id, int64_t wait_ms) {
     std::unique_lock<std::mutex> lk(mu_);
     auto it = reqs_.find(id);
     if (it == reqs_.end()) throw Error(-2, "unknown request id " + std::to_string(id));
     std::shared_ptr<ServeRequest> r = it->second;
-    if (wait) cv_done_.wait(lk, [&]() { return r->done; });
+    if (wait_ms < 0) {
+        cv_done_.wait(lk, [&]() { return r->done; });
+    } else if (wait_ms > 0) {
+        if (!cv_done_.wait_for(lk, std::chrono::milliseconds(wait_ms), [&]() { return r->done; }))
+            throw Error(-4, "request " + std::to_string(id) + " not finished after " + std::to_string(wait_ms) +
+                                " ms; " + status_locked());
+    }
     if (!r->done) return nullptr;
     reqs_.erase(it);
     return r;
@@ -94,7 +186,37 @@ ServerStats Server::stats() {
     return st_;
 }
 
-void Server::worker() {
+std::string Server::status() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return status_locked();
+}
+
+std::string Server::status_locked() {
+    std::string out = std::to_string(pending_.size()) + " pending, " + std::to_string(st_.inflight_batches) +
+                      " batches in flight, device memory " + std::to_string(pool_.taken() >> 20) + " of " +
+                      std::to_string(pool_.held() >> 20) + " MiB held in use (budget " +
+                      std::to_string(budget_ >> 20) + " MiB)";
+    const auto now = Clock::now();
+    for (size_t i = 0; i < slots_.size(); ++i) {
+        const Slot& sl = *slots_[i];
+        const int32_t ph = sl.phase.load(std::memory_order_relaxed);
+        const double ms = std::chrono::duration<double, std::milli>(now - sl.since).count();
+        out += "; slot " + std::to_string(i) + ": " + phase_name(ph / 1000);
+        if (ph / 1000 == kPhaseCoarsen || ph / 1000 == kPhaseRefine) out += " " + std::to_string(ph % 1000);
+        if (ph / 1000 != kPhaseIdle)
+            out += " (" + std::to_string(sl.graphs) + " graphs, " + std::to_string(sl.vertices) + " vertices)";
+        out += " for " + std::to_string((int64_t)ms) + " ms";
+    }
+    return out;
+}
+
+bool Server::fits_locked(int64_t bytes) const {
+    int64_t extra = 0;
+    for (const auto& sl : slots_) extra += sl->overflow;
+    return pool_.can_take(bytes, extra);
+}
+
+void Server::worker(int slot) {
     (void)hipSetDevice(device_);
     Context* cx = nullptr;
     try {
@@ -102,24 +224,30 @@ void Server::worker() {
     } catch (const Error&) {
         return;   // no usable device: the queue is failed by the destructor
     }
+    Slot& me = *slots_[slot];
+    cx->phase = &me.phase;
     for (;;) {
         std::vector<std::shared_ptr<ServeRequest>> batch;
         int64_t fp = 0;
+        char* range = nullptr;
         {
             std::unique_lock<std::mutex> lk(mu_);
-            // the oldest request runs as soon as it fits next to the batches
-            // in flight (alone it always fits: submit checked the budget)
+            // the oldest request runs as soon as its device memory is free
+            // next to the batches in flight; with none in flight it always
+            // runs (submit checked it against the budget), after the pool
+            // drops segments it cannot use
             cv_work_.wait(lk, [&]() {
-                return stop_ || (!pending_.empty() && (st_.inflight_batches == 0 ||
-                                                       in_use_ + pending_.front()->footprint <= budget_));
+                return stop_ ||
+                       (!pending_.empty() && (st_.inflight_batches == 0 || fits_locked(pending_.front()->footprint)));
             });
             if (stop_) break;
+            if (st_.inflight_batches == 0 && !fits_locked(pending_.front()->footprint)) pool_.release_all();
             const std::shared_ptr<ServeRequest> head = pending_.front();
             int64_t nv = 0;
             for (auto it = pending_.begin(); it != pending_.end();) {
                 const auto& r = *it;
                 const bool first = batch.empty();
-                if (!first && (!same_setup(*head, *r) || in_use_ + fp + r->footprint > budget_ ||
+                if (!first && (!same_setup(*head, *r) || !fits_locked(fp + r->footprint) ||
                                nv + r->n > max_batch_v_)) {
                     ++it;
                     continue;
@@ -129,9 +257,18 @@ void Server::worker() {
                 batch.push_back(r);
                 it = pending_.erase(it);
             }
+            int64_t extra = 0;
+            for (const auto& sl : slots_) extra += sl->overflow;
+            // null only when the slots' own overflow chunks leave no room
+            // under the budget: the batch then runs in its arena's chunks
+            range = pool_.take(fp, extra);
             in_use_ += fp;
             st_.inflight_batches += 1;
             st_.max_inflight_batches = std::max(st_.max_inflight_batches, st_.inflight_batches);
+            me.graphs = (int64_t)batch.size();
+            me.vertices = nv;
+            me.since = Clock::now();
+            me.phase.store(kPhasePack * 1000, std::memory_order_relaxed);
         }
         const auto t_start = Clock::now();
         int32_t status = 1;
@@ -154,8 +291,8 @@ void Server::worker() {
             if (n > INT32_MAX || m > INT32_MAX) throw Error(-4, "batch too large for int32 CSR");
             gvtx[G] = (int32_t)n;
             Arena& ar = cx->arena;
+            if (range) ar.attach(range, (size_t)fp, false);
             ar.reset();
-            ar.reserve((size_t)fp);
             cx->upload_reset();
             // pinned staging only for what comes from the host
             bool any_host = false, fill_w = false;
@@ -253,11 +390,16 @@ void Server::worker() {
             status = -4;
             error = e.what();
         }
+        cx->arena.detach();     // the range goes back to the pool below
         cx->arena.reset();
         cx->upload_reset();
         const auto t_done = Clock::now();
         {
             std::lock_guard<std::mutex> lk(mu_);
+            if (range) pool_.give(range, fp);
+            me.overflow = (int64_t)cx->arena.owned_bytes();
+            me.phase.store(kPhaseIdle * 1000, std::memory_order_relaxed);
+            me.since = t_done;
             for (int32_t i = 0; i < G; ++i) {
                 ServeRequest& r = *batch[i];
                 r.t_start = t_start;

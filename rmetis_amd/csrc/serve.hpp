@@ -2,7 +2,9 @@
 #pragma once
 #include <chrono>
 #include <condition_variable>
+#include <atomic>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -14,7 +16,7 @@
 
 namespace rk {
 
-constexpr int kMaxServeSlots = 3;   // batched calls a server runs at once, at most
+constexpr int kMaxServeSlots = 8;   // batched calls a server runs at once, at most
 
 // One k-way request: a copy of the caller's graph, its part count, shared
 // options and its own seed; the result is filled in by the worker that ran it.
@@ -49,6 +51,38 @@ struct ServerStats {
     int64_t inflight_batches = 0, max_inflight_batches = 0, max_batch_graphs = 0;
 };
 
+// Device memory of a server: segments taken with hipMalloc while there is
+// room under the budget and kept until the server ends, carved into one
+// contiguous range per batch (first fit).  No segment is freed while batches
+// run: a hipFree waits for every stream of the device.  All calls are made
+// with the server's mutex held.
+class DevicePool {
+public:
+    explicit DevicePool(int64_t budget) : budget_(budget) {}
+    ~DevicePool();
+    // A free range of `bytes`, from a new segment when none is free and the
+    // budget allows one (extra: device bytes held elsewhere that count
+    // against the budget); null when neither is possible.
+    char* take(int64_t bytes, int64_t extra);
+    bool can_take(int64_t bytes, int64_t extra) const;
+    void give(char* p, int64_t bytes);
+    // Frees every segment; only while no range is taken (no batch runs).
+    void release_all();
+    int64_t held() const { return held_; }
+    int64_t taken() const { return taken_; }
+
+private:
+    struct Seg {
+        char* base;
+        int64_t size;
+        std::map<int64_t, int64_t> free;   // offset -> length, coalesced
+    };
+    static int64_t round(int64_t b) { return (b + 255) & ~(int64_t)255; }
+    bool fits_free(int64_t bytes) const;
+    int64_t budget_, held_ = 0, taken_ = 0;
+    std::vector<Seg> segs_;
+};
+
 class Server {
 public:
     // slots: batched calls that may run at once (one worker thread and one
@@ -57,17 +91,30 @@ public:
     Server(int device, int slots, int64_t budget, int64_t max_batch_vertices);
     ~Server();
     int64_t submit(std::shared_ptr<ServeRequest> r);
-    // The finished request (removed from the server), or null while pending
-    // (waits for it when wait is true).
-    std::shared_ptr<ServeRequest> result(int64_t id, bool wait);
+    // The finished request (removed from the server), or null while pending.
+    // wait_ms: 0 returns at once, < 0 waits until it is done, > 0 waits at
+    // most that long and then fails with a report of every slot's phase.
+    std::shared_ptr<ServeRequest> result(int64_t id, int64_t wait_ms);
     ServerStats stats();
+    // One line per slot: what it runs and for how long (for diagnostics).
+    std::string status();
 
 private:
-    void worker();
+    void worker(int slot);
+    std::string status_locked();
+    bool fits_locked(int64_t bytes) const;
     int device_;
     int64_t budget_, max_batch_v_;
     int64_t in_use_ = 0;
     int64_t next_id_ = 1;
+    DevicePool pool_;
+    struct Slot {
+        std::atomic<int32_t> phase{0};
+        int64_t graphs = 0, vertices = 0;
+        int64_t overflow = 0;                       // device bytes its arena allocated past its ranges
+        std::chrono::steady_clock::time_point since;
+    };
+    std::vector<std::unique_ptr<Slot>> slots_;
     bool stop_ = false;
     ServerStats st_;
     std::mutex mu_;

@@ -209,7 +209,7 @@ def test_server_without_gpu_fails_loudly():
     with pytest.raises(R.InternalError, match="no HIP device"):
         R.Server(slots=2)
     assert _lib.lib().rmetis_hip_server_create(0, 0, 0) is None      # slots < 1
-    assert _lib.lib().rmetis_hip_server_create(4, 0, 0) is None      # slots > 3
+    assert _lib.lib().rmetis_hip_server_create(9, 0, 0) is None      # slots > 8
 
 
 def test_product_grow_equals_oracle():

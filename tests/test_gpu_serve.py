@@ -113,3 +113,25 @@ def test_served_device_requests_equal_alone():
         assert r.part is None and r.objval == alone.objval
         assert np.array_equal(d[4].numpy(), alone.part)
     _check(hres, R.part_graph_kway(graphs[1][0], 16, ubvec=ub, options=_opts(2)))
+
+
+def test_served_stream_on_four_slots():
+    # four slots (batched calls in flight at once, one stream each) under a
+    # stream of mixed arrivals; results are waited for with a time limit so
+    # that a stall fails the test with every slot's phase instead of hanging
+    rng = np.random.default_rng(5)
+    pool = [(_rgg(int(n), 60 + i, ncon=2, vmax=8), 2 ** int(rng.integers(3, 9)))
+            for i, n in enumerate(rng.integers(2000, 90000, 12))]
+    ub = np.array([1.05, 1.05], np.float32)
+    reqs = [(pool[int(rng.integers(len(pool)))], 100 + j) for j in range(48)]
+    with R.Server(slots=4) as srv:
+        ids = []
+        for j, ((g, k), seed) in enumerate(reqs):
+            ids.append(srv.submit(g, min(k, g.nvtxs), ubvec=ub, options=_opts(1), seed=seed))
+            if j % 4 == 3:
+                time.sleep(0.002)
+        res = [srv.result(rid, timeout_ms=120000) for rid in ids]
+        st = srv.stats()
+    assert st["served"] == len(reqs) and st["max_inflight_batches"] >= 2
+    for ((g, k), seed), r in zip(reqs, res):
+        _check(r, R.part_graph_kway(g, min(k, g.nvtxs), ubvec=ub, options=_opts(seed)))
