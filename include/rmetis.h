@@ -57,8 +57,9 @@ int32_t RMETIS_SetDefaultOptions(idx_t* options);
  * part; optional (NULL = None): vwgt[n*ncon], vsize[n], adjwgt[xadj[n]],
  * tpwgts[nparts*ncon], ubvec[ncon], options[40], objval.  Inputs are copied;
  * part/objval are written only on success.  The multilevel path runs on the
- * current HIP device; RTYPE must be Greedy (1): the FM refinement of the
- * reference is not provided and returns METIS_ERROR. */
+ * current HIP device; RTYPE Greedy (1) refines with the greedy passes of
+ * refine/greedy.rs, anything else (the default, Fm) with the k-way FM of
+ * refine/fm_kway.rs (kway.rs:154-161). */
 int32_t RMETIS_PartGraphKway(const idx_t* nvtxs, const idx_t* ncon, const idx_t* xadj, const idx_t* adjncy,
                              const idx_t* vwgt, const idx_t* vsize, const idx_t* adjwgt, const idx_t* nparts,
                              const real_t* tpwgts, const real_t* ubvec, const idx_t* options, idx_t* objval,
@@ -117,7 +118,7 @@ int32_t rmetis_hip_kway_device_noalloc(idx_t nvtxs, idx_t ncon, const idx_t* d_x
  * pointers, n = gvtx[ngraphs]; d_vwgt and d_adjwgt may be NULL (for every
  * graph: absent vertex weights are not the same as ones, since the coarse
  * vertices of an unweighted graph keep weight 1, graph/mod.rs:300-314).  options are shared
- * (RTYPE must be Greedy; NCUTS attempts per graph); attempt a of graph i uses
+ * (RTYPE must be Greedy, so options[0] must not be -1; NCUTS attempts per graph); attempt a of graph i uses
  * seed seeds[i] + a (options' SEED + a when seeds is NULL; a zero seed draws
  * entropy) and graph i keeps its strictly best cut, so every graph's part
  * and objval equal those of a separate call with that seed.  ubvec[ncon]
@@ -219,6 +220,16 @@ int32_t rmetis_hip_greedy_pass(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const
 int32_t rmetis_hip_greedy_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
                                  const idx_t* vwgt, const idx_t* adjwgt, idx_t* part, idx_t nparts,
                                  const double* tpwgts, const double* ubvec, idx_t niter, idx_t* cut);
+
+/* fm_kway_refine, src/refine/fm_kway.rs:12-30: up to niter FM passes on part
+ * (in/out), stopping after a pass that keeps no move; tpwgts[nparts*ncon]
+ * f64 target weights and ubvec[ncon] f64 imbalance factors as the
+ * reference's function receives them; *cut = the final edge cut.  The queue
+ * pops equal gains smallest vertex first (the reference's order among equal
+ * gains is internal to its priority-queue crate). */
+int32_t rmetis_hip_fm_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy, const idx_t* vwgt,
+                             const idx_t* adjwgt, idx_t* part, idx_t nparts, const double* tpwgts,
+                             const double* ubvec, idx_t niter, idx_t* cut);
 
 /* The GPU matching used in place of shem_matching / random_matching
  * (src/coarsen/shem.rs:13-56, src/coarsen/rm.rs:13-43): seeded, deterministic,

@@ -46,6 +46,7 @@
 #include <cstring>
 #include <deque>
 #include <random>
+#include <set>
 #include <utility>
 #include <vector>
 
@@ -195,6 +196,144 @@ int32_t greedy_refine_impl(const OGraph& g, int32_t* part, int32_t nparts, const
         if (!any) break;                                                     // greedy.rs:183-185
     }
     return edge_cut_impl(g.n, g.xadj.data(), g.adjncy.data(), g.ewp(), part); // greedy.rs:192
+}
+
+// ---- refine/fm_kway.rs:12-158 (k-way FM) --------------------------------
+//
+// The reference keeps its queue in priority-queue 1.4.0's PriorityQueue
+// (max-heap over i32 gains); the order in which it pops vertices of EQUAL
+// gain is an implementation detail of that absent crate.  Here ties pop the
+// smallest vertex id first.  Everything else follows fm_kway.rs line by line.
+
+// fm_kway.rs:47-72: (best part, gain) over parts with a nonzero edge-weight
+// sum, gain > 0 strictly, ties to the lower part; false when none.
+bool fm_best_move(const OGraph& g, const int32_t* part, int32_t nparts, int32_t v, std::vector<int32_t>& edge_to,
+                  int32_t* best_p, int32_t* best_g) {
+    const int32_t pv = part[v];
+    std::fill(edge_to.begin(), edge_to.begin() + nparts, 0);
+    for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {
+        const int32_t u = g.adjncy[j];
+        edge_to[part[u]] = wadd(edge_to[part[u]], g.ew(j));                 // 56-60
+    }
+    int32_t bp = -1, bg = 0;
+    for (int32_t p = 0; p < nparts; ++p) {                                   // 62-70
+        if (p == pv) continue;
+        if (edge_to[p] == 0) continue;
+        const int32_t gain = wsub(edge_to[p], edge_to[pv]);
+        if (gain > bg) { bg = gain; bp = p; }
+    }
+    if (bp < 0) return false;
+    *best_p = bp;
+    *best_g = bg;
+    return true;
+}
+
+// fm_kway.rs:33-158: one pass; returns its boolean (true when improved).
+bool fm_pass_impl(const OGraph& g, int32_t* part, int32_t nparts, std::vector<int32_t>& pw, const double* tpw,
+                  const double* ub, int32_t* nmoves_out) {
+    const int32_t n = g.n, ncon = g.ncon;
+    std::vector<int32_t> edge_to(nparts);
+    std::vector<int32_t> bm_p(n, -1), bm_g(n, 0);       // best_moves (45): target, gain; -1 = None
+    std::set<std::pair<int64_t, int32_t>> pq;            // (-gain, v): max gain first, then smallest v
+    std::vector<int32_t> key(n, 0);                      // queued gain (0 = not queued)
+    auto pq_set = [&](int32_t v, int32_t gain) {
+        if (key[v] != 0) pq.erase({-(int64_t)key[v], v});
+        key[v] = gain;
+        pq.insert({-(int64_t)gain, v});
+    };
+    auto pq_remove = [&](int32_t v) {
+        if (key[v] != 0) pq.erase({-(int64_t)key[v], v});
+        key[v] = 0;
+    };
+    for (int32_t v = 0; v < n; ++v) {                                        // 76-85
+        const int32_t pv = part[v];
+        bool on_boundary = false;
+        for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j)
+            if (part[g.adjncy[j]] != pv) { on_boundary = true; break; }
+        if (!on_boundary) continue;
+        int32_t tp, gain;
+        if (fm_best_move(g, part, nparts, v, edge_to, &tp, &gain)) {
+            bm_p[v] = tp;
+            bm_g[v] = gain;
+            pq_set(v, gain);
+        }
+    }
+    std::vector<char> locked(n, 0);
+    std::vector<int32_t> mv_v, mv_from;
+    int32_t cut = edge_cut_impl(n, g.xadj.data(), g.adjncy.data(), g.ewp(), part);   // 89
+    int32_t best_cut = cut;
+    size_t best_len = 0;
+    std::vector<int32_t> vw(ncon);
+    while (!pq.empty()) {                                                    // 93
+        const int32_t v = pq.begin()->second;
+        const int32_t gain = (int32_t)(-pq.begin()->first);
+        pq.erase(pq.begin());
+        key[v] = 0;
+        if (locked[v] || gain <= 0) break;                                   // 94
+        if (bm_p[v] < 0) continue;                                           // 96-99
+        const int32_t to = bm_p[v], from = part[v];
+        for (int32_t c = 0; c < ncon; ++c) vw[c] = g.vw(v, c);
+        bool feasible = true;                                                // 105-109
+        for (int32_t c = 0; c < ncon; ++c) {
+            const int32_t new_w = wadd(pw[(size_t)to * ncon + c], vw[c]);
+            if (!((double)new_w <= tpw[(size_t)to * ncon + c] * ub[c])) { feasible = false; break; }
+        }
+        if (!feasible) continue;
+        part[v] = to;                                                        // 112-124
+        for (int32_t c = 0; c < ncon; ++c) {
+            pw[(size_t)from * ncon + c] = wsub(pw[(size_t)from * ncon + c], vw[c]);
+            pw[(size_t)to * ncon + c] = wadd(pw[(size_t)to * ncon + c], vw[c]);
+        }
+        locked[v] = 1;
+        mv_v.push_back(v);
+        mv_from.push_back(from);
+        cut = wsub(cut, gain);
+        if (cut < best_cut) { best_cut = cut; best_len = mv_v.size(); }
+        for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {                // 127-143
+            const int32_t u = g.adjncy[j];
+            if (locked[u]) continue;
+            int32_t tp, ug;
+            if (fm_best_move(g, part, nparts, u, edge_to, &tp, &ug)) {
+                bm_p[u] = tp;
+                bm_g[u] = ug;
+                pq_set(u, ug);
+            } else {
+                pq_remove(u);
+                bm_p[u] = -1;
+            }
+        }
+    }
+    for (size_t i = mv_v.size(); i > best_len; --i) {                        // 147-155 rollback
+        const int32_t v = mv_v[i - 1], from = mv_from[i - 1], cur = part[v];
+        for (int32_t c = 0; c < ncon; ++c) {
+            pw[(size_t)cur * ncon + c] = wsub(pw[(size_t)cur * ncon + c], g.vw(v, c));
+            pw[(size_t)from * ncon + c] = wadd(pw[(size_t)from * ncon + c], g.vw(v, c));
+        }
+        part[v] = from;
+    }
+    if (nmoves_out) *nmoves_out = (int32_t)best_len;
+    const int32_t after = edge_cut_impl(n, g.xadj.data(), g.adjncy.data(), g.ewp(), part);
+    return best_len > 0 && best_cut < wadd(after, 1);                        // 157
+}
+
+// fm_kway.rs:12-30
+int32_t fm_refine_impl(const OGraph& g, int32_t* part, int32_t nparts, const double* tpw, const double* ub,
+                       int32_t niter, int64_t* refine_edges) {
+    std::vector<int32_t> pw((size_t)nparts * g.ncon);
+    pweights(g.n, g.ncon, g.vwp(), part, nparts, pw.data());                 // 21
+    for (int32_t it = 0; it < niter; ++it) {                                 // 23-27
+        if (refine_edges) *refine_edges += g.xadj[g.n];
+        if (!fm_pass_impl(g, part, nparts, pw, tpw, ub, nullptr)) break;
+    }
+    return edge_cut_impl(g.n, g.xadj.data(), g.adjncy.data(), g.ewp(), part);   // 29
+}
+
+// partition/kway.rs:145-162 (refine_level): Greedy or, through
+// refine/mod.rs:26-38, k-way FM.
+void refine_level_impl(const OGraph& g, int32_t* part, int32_t nparts, const double* tpw, const double* ub,
+                       int32_t niter, int32_t rtype_greedy, int64_t* refine_edges) {
+    if (rtype_greedy) greedy_refine_impl(g, part, nparts, tpw, ub, niter, refine_edges);
+    else fm_refine_impl(g, part, nparts, tpw, ub, niter, refine_edges);
 }
 
 std::vector<int32_t> shem_impl(const OGraph& g, orc_rng* rng) {
@@ -541,6 +680,35 @@ int32_t orc_greedy_refine(int32_t n, int32_t ncon, const int32_t* xadj, const in
     return greedy_refine_impl(g, part, nparts, tpwgts, ubvec, niter, nullptr);
 }
 
+int32_t orc_fm_kway_refine(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy,
+                           const int32_t* vwgt, const int32_t* adjwgt, int32_t* part, int32_t nparts,
+                           const double* tpwgts, const double* ubvec, int32_t niter) {
+    OGraph g;
+    g.n = n;
+    g.ncon = ncon;
+    g.xadj.assign(xadj, xadj + n + 1);
+    g.adjncy.assign(adjncy, adjncy + xadj[n]);
+    if (vwgt) { g.has_vwgt = true; g.vwgt.assign(vwgt, vwgt + (size_t)n * ncon); }
+    if (adjwgt) { g.has_adjwgt = true; g.adjwgt.assign(adjwgt, adjwgt + xadj[n]); }
+    return fm_refine_impl(g, part, nparts, tpwgts, ubvec, niter, nullptr);
+}
+
+int32_t orc_fm_kway_pass(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy,
+                         const int32_t* vwgt, const int32_t* adjwgt, int32_t* part, int32_t nparts,
+                         const double* tpwgts, const double* ubvec, int32_t* pw, int32_t* nmoves) {
+    OGraph g;
+    g.n = n;
+    g.ncon = ncon;
+    g.xadj.assign(xadj, xadj + n + 1);
+    g.adjncy.assign(adjncy, adjncy + xadj[n]);
+    if (vwgt) { g.has_vwgt = true; g.vwgt.assign(vwgt, vwgt + (size_t)n * ncon); }
+    if (adjwgt) { g.has_adjwgt = true; g.adjwgt.assign(adjwgt, adjwgt + xadj[n]); }
+    std::vector<int32_t> w(pw, pw + (size_t)nparts * ncon);
+    const bool improved = fm_pass_impl(g, part, nparts, w, tpwgts, ubvec, nmoves);
+    std::copy(w.begin(), w.end(), pw);
+    return improved ? 1 : 0;
+}
+
 static OGraph make_graph(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy,
                          const int32_t* vwgt, const int32_t* adjwgt, const int32_t* vsize) {
     OGraph g;
@@ -604,7 +772,6 @@ int32_t orc_part_graph_kway(int32_t n, int32_t ncon, const int32_t* xadj, const 
                             int32_t* objval, int32_t* part, orc_stats* stats) {
     // partition/kway.rs:20-141 (comm = None => SingleComm)
     if (nparts < 2 || nparts > n) return -2;                                 // kway.rs:31-38
-    if (!rtype_greedy) return -4;  // FM refinement is outside the restated path
     OGraph g = make_graph(n, ncon, xadj, adjncy, vwgt, adjwgt, vsize);
     std::vector<double> ub(ncon);                                            // kway.rs:44-49
     for (int32_t c = 0; c < ncon; ++c)
@@ -648,16 +815,16 @@ int32_t orc_part_graph_kway(int32_t n, int32_t ncon, const int32_t* xadj, const 
             return -4;
         }
         double t2 = now_s();
-        greedy_refine_impl(coarsest, cpart.data(), nparts, ct.data(), ub.data(), niter,
-                           stats ? &stats->refine_edges : nullptr);          // kway.rs:73
+        refine_level_impl(coarsest, cpart.data(), nparts, ct.data(), ub.data(), niter, rtype_greedy,
+                          stats ? &stats->refine_edges : nullptr);           // kway.rs:73
         std::vector<int32_t> pcur = std::move(cpart);
         for (int64_t i = (int64_t)lg.size() - 1; i >= 0; --i) {              // kway.rs:76-92
             const OGraph& fine = (i == 0) ? g : lg[i - 1];
             std::vector<int32_t> fp(fine.n);
             for (int32_t v = 0; v < fine.n; ++v) fp[v] = pcur[cmaps[i][v]];
             std::vector<double> lt = targets_impl(fine, nparts, tpwgts);
-            greedy_refine_impl(fine, fp.data(), nparts, lt.data(), ub.data(), niter,
-                               stats ? &stats->refine_edges : nullptr);
+            refine_level_impl(fine, fp.data(), nparts, lt.data(), ub.data(), niter, rtype_greedy,
+                              stats ? &stats->refine_edges : nullptr);   // kway.rs:90
             pcur = std::move(fp);
         }
         double t3 = now_s();

@@ -62,6 +62,18 @@ int32_t orc_greedy_refine(int32_t n, int32_t ncon, const int32_t* xadj, const in
                           int32_t nparts, const double* tpwgts, const double* ubvec,
                           int32_t niter);
 
+/* refine/fm_kway.rs:12-30 (fm_kway_refine) and one pass (33-158).  The
+ * queue pops equal gains smallest vertex first (the reference's order among
+ * equal gains is internal to the priority-queue crate).  The pass returns
+ * the reference's boolean and *nmoves the moves it kept; pw (in/out) is the
+ * running part-weight vector. */
+int32_t orc_fm_kway_refine(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy,
+                           const int32_t* vwgt, const int32_t* adjwgt, int32_t* part, int32_t nparts,
+                           const double* tpwgts, const double* ubvec, int32_t niter);
+int32_t orc_fm_kway_pass(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy,
+                         const int32_t* vwgt, const int32_t* adjwgt, int32_t* part, int32_t nparts,
+                         const double* tpwgts, const double* ubvec, int32_t* pw, int32_t* nmoves);
+
 /* coarsen/shem.rs, coarsen/rm.rs: matching[v] = partner (or v). */
 void orc_shem_matching(int32_t n, const int32_t* xadj, const int32_t* adjncy,
                        const int32_t* adjwgt, orc_rng* rng, int32_t* matching);
@@ -97,7 +109,8 @@ typedef struct {
 } orc_stats;
 
 /* partition/kway.rs:20-141 behind the c_api.rs:140-209 argument conventions
- * (NULL = None).  Options are the already-parsed fields.  Returns METIS codes
+ * (NULL = None).  Options are the already-parsed fields; rtype_greedy 0
+ * refines with k-way FM (kway.rs:158-160).  Returns METIS codes
  * (1 ok, -2 input, -4 other). */
 int32_t orc_part_graph_kway(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy,
                             const int32_t* vwgt, const int32_t* vsize, const int32_t* adjwgt,

@@ -172,6 +172,29 @@ def greedy_refine(ncon, xadj, adjncy, vwgt, adjwgt, part, nparts, tpw, ub, niter
     return part, cut
 
 
+def fm_kway_refine(ncon, xadj, adjncy, vwgt, adjwgt, part, nparts, tpw, ub, niter):
+    xadj, adjncy, vwgt, adjwgt = map(_i32, (xadj, adjncy, vwgt, adjwgt))
+    part = np.array(part, dtype=np.int32)
+    tpw = np.ascontiguousarray(tpw, np.float64)
+    ub = np.ascontiguousarray(ub, np.float64)
+    cut = lib().orc_fm_kway_refine(len(part), ncon, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt),
+                                   _p(part), nparts, _p(tpw, f64p), _p(ub, f64p), niter)
+    return part, cut
+
+
+def fm_kway_pass(ncon, xadj, adjncy, vwgt, adjwgt, part, nparts, tpw, ub, pw):
+    """One FM pass; returns (improved, new_part, new_pw, kept moves)."""
+    xadj, adjncy, vwgt, adjwgt = map(_i32, (xadj, adjncy, vwgt, adjwgt))
+    part = np.array(part, dtype=np.int32)
+    pw = np.array(pw, dtype=np.int32).reshape(-1)
+    tpw = np.ascontiguousarray(tpw, np.float64)
+    ub = np.ascontiguousarray(ub, np.float64)
+    nm = C.c_int32()
+    ok = lib().orc_fm_kway_pass(len(part), ncon, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), _p(part),
+                                nparts, _p(tpw, f64p), _p(ub, f64p), _p(pw), C.byref(nm))
+    return bool(ok), part, pw.reshape(nparts, ncon), nm.value
+
+
 def shem_matching(xadj, adjncy, adjwgt, r):
     xadj, adjncy, adjwgt = map(_i32, (xadj, adjncy, adjwgt))
     m = np.zeros(len(xadj) - 1, np.int32)

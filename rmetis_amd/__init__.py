@@ -277,6 +277,19 @@ def greedy_refine(graph, part, nparts, tpwgts, ubvec, niter):
     return part, cut.value
 
 
+def fm_kway_refine(graph, part, nparts, tpwgts, ubvec, niter):
+    """refine/fm_kway.rs:12-30 -> (new part, cut).  tpwgts: f64 targets
+    [nparts*ncon]; ubvec: f64 [ncon]."""
+    part = np.array(part, dtype=np.int32)
+    tp = np.ascontiguousarray(tpwgts, np.float64)
+    ub = np.ascontiguousarray(ubvec, np.float64)
+    cut = C.c_int32()
+    _raise(_lib.lib().rmetis_hip_fm_refine(
+        graph.nvtxs, graph.ncon, _p(graph.xadj), _p(graph.adjncy), _p(graph.vwgt), _p(graph.adjwgt), _p(part),
+        nparts, _p(tp, _lib.f64p), _p(ub, _lib.f64p), niter, C.byref(cut)))
+    return part, cut.value
+
+
 def greedy_pass(graph, part, nparts, tpwgts, ubvec, mode, pw):
     """One synchronous greedy pass (mode 0 repair, 1 cut) -> (part, pw, moves)."""
     part = np.array(part, dtype=np.int32)

@@ -58,6 +58,8 @@ SIGNATURES = {
                                            f64p, f64p, C.c_int32, i32p, i32p, i32p]),
     "rmetis_hip_greedy_refine": (C.c_int32, [C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, i32p, C.c_int32,
                                              f64p, f64p, C.c_int32, i32p]),
+    "rmetis_hip_fm_refine": (C.c_int32, [C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, i32p, C.c_int32,
+                                         f64p, f64p, C.c_int32, i32p]),
     "rmetis_hip_match": (C.c_int32, [C.c_int32, i32p, i32p, i32p, C.c_int32, C.c_uint64, i32p]),
     "rmetis_hip_build_coarse_graph": (C.c_int32, [C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p,
                                                   i32p, i32p, i32p, i32p, i32p, i32p, i32p, i32p, i32p]),

@@ -85,8 +85,9 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
                        KwayStats* st) {
     if (G <= 0) throw Error(-2, "a batch needs at least one graph");
     if (!o.rtype_greedy)
-        throw Error(-4, "rtype Fm is not provided by the GPU k-way path; set options[METIS_OPTION_RTYPE]=1 "
-                        "(Greedy)");
+        throw Error(-2, "the batched and served forms refine greedily: set options[METIS_OPTION_RTYPE]=1 "
+                        "(Greedy) and options[METIS_OPTION_PTYPE]=0 so that the array is read; k-way FM runs "
+                        "through RMETIS_PartGraphKway and rmetis_hip_kway_device");
     if (gvtx[0] != 0 || gvtx[G] != g.n) throw Error(-2, "graph offsets must start at 0 and end at nvtxs");
     const int32_t ncon = g.ncon;
     std::vector<int32_t> gk(G);

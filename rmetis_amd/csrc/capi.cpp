@@ -414,8 +414,9 @@ int64_t rmetis_hip_server_submit(rmetis_hip_server* server, idx_t nvtxs, idx_t n
         if (nparts > nvtxs) return METIS_ERROR_INPUT;                                 // kway.rs:34-38
         const Options o = parse_options(options);
         if (!o.rtype_greedy)
-            throw Error(-4, "rtype Fm is not provided by the GPU k-way path; set options[METIS_OPTION_RTYPE]=1 "
-                            "(Greedy)");
+            throw Error(-2, "the batched and served forms refine greedily: set options[METIS_OPTION_RTYPE]=1 "
+                            "(Greedy) and options[METIS_OPTION_PTYPE]=0 so that the array is read; k-way FM runs "
+                            "through RMETIS_PartGraphKway and rmetis_hip_kway_device");
         std::vector<int64_t> tot(ncon, 0);
         for (int64_t v = 0; v < nvtxs; ++v)
             for (int c = 0; c < ncon; ++c) tot[c] += vwgt ? vwgt[v * ncon + c] : 1;
@@ -450,8 +451,9 @@ int64_t rmetis_hip_server_submit_device(rmetis_hip_server* server, idx_t nvtxs, 
         if (nvtxs <= 0 || ncon <= 0 || nparts < 2 || nparts > nvtxs) return METIS_ERROR_INPUT;
         const Options o = parse_options(options);
         if (!o.rtype_greedy)
-            throw Error(-4, "rtype Fm is not provided by the GPU k-way path; set options[METIS_OPTION_RTYPE]=1 "
-                            "(Greedy)");
+            throw Error(-2, "the batched and served forms refine greedily: set options[METIS_OPTION_RTYPE]=1 "
+                            "(Greedy) and options[METIS_OPTION_PTYPE]=0 so that the array is read; k-way FM runs "
+                            "through RMETIS_PartGraphKway and rmetis_hip_kway_device");
         int32_t m = 0;
         RK_HIP(hipMemcpy(&m, d_xadj + nvtxs, sizeof(int32_t), hipMemcpyDeviceToHost));
         auto r = std::make_shared<ServeRequest>();
@@ -647,6 +649,41 @@ int32_t rmetis_hip_greedy_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, con
         int32_t c = 0;
         download(*cx.c, &c, B.cut_out, 1);
         RK_HIP(hipStreamSynchronize(s));
+        if (cut) *cut = c;
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_fm_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy, const idx_t* vwgt,
+                             const idx_t* adjwgt, idx_t* part, idx_t nparts, const double* tpwgts,
+                             const double* ubvec, idx_t niter, idx_t* cut) {
+    // fm_kway_refine, src/refine/fm_kway.rs:12-30
+    return guarded([&]() -> int32_t {
+        if (!xadj || !adjncy || !part || !tpwgts || !ubvec || nvtxs <= 0 || ncon <= 0 || nparts < 1 || niter < 0)
+            return METIS_ERROR_INPUT;
+        if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt) || !valid_part(nvtxs, part, nparts) ||
+            !totals_in_range(nvtxs, ncon, vwgt))
+            return METIS_ERROR_INPUT;
+        ContextGuard cx;
+        hipStream_t s = cx->stream;
+        DevGraph g = upload_graph(*cx.c, nvtxs, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
+        int32_t* dp = upload(*cx.c, part, nvtxs);
+        Buffers B = alloc_buffers(*cx.c, g.n, g.m, nparts, ncon);
+        FmBufs F = alloc_fm_buffers(*cx.c, g.n, g.m, nparts, ncon);
+        const size_t kc = (size_t)nparts * ncon;
+        std::vector<long long> fm(kc), om(kc);
+        thresholds_from_targets(nparts, ncon, tpwgts, ubvec, fm.data(), om.data());
+        int32_t* heavy = cx->arena.get<int32_t>(nvtxs);
+        const int nheavy = heavy_list_sync(g, heavy, B.rb, s);
+        RK_HIP(hipMemcpyAsync(B.rb.fmax, fm.data(), kc * sizeof(long long), hipMemcpyHostToDevice, s));
+        fm_refine_level(g, nparts, B.rb.fmax, niter, dp, nheavy, heavy, B.rb, F, s, false, false);
+        edge_cut(g, dp, B.cut_acc, B.cut_out, pick_group(g.n, g.m), s);
+        download(*cx.c, part, dp, nvtxs);
+        int32_t c = 0, err = 0;
+        download(*cx.c, &c, B.cut_out, 1);
+        download(*cx.c, &err, F.err, 1);
+        RK_HIP(hipStreamSynchronize(s));
+        if (err) throw Error(-4, "FM pass exceeded its pop budget");
         if (cut) *cut = c;
         return METIS_OK;
     });

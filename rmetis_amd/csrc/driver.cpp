@@ -180,6 +180,31 @@ Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon
     return B;
 }
 
+FmBufs alloc_fm_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon) {
+    Arena& a = cx.arena;
+    FmBufs f;
+    int64_t tree = 0, len = n;
+    for (;;) {
+        tree += len;
+        if (len <= 1) break;
+        len = (len + 63) / 64;
+    }
+    const int64_t kc = (int64_t)k * ncon;
+    f.tree = a.get<uint64_t>(tree);
+    f.tp = a.get<int32_t>(n);
+    f.locked = (uint8_t*)a.get<int32_t>(((int64_t)n + 3) / 4);
+    f.vrec = a.get<int4>(n);
+    f.g0 = a.get<int32_t>(n);
+    f.gains = a.get<int32_t>(m);
+    f.pw0 = a.get<int32_t>(kc);
+    f.omin = a.get<long long>(kc);
+    f.ovf = a.get<int32_t>(k);
+    f.err = a.get<int32_t>(1);
+    fm_fill_tables(f.pw0, f.omin, kc, cx.stream);
+    RK_HIP(hipMemsetAsync(f.err, 0, sizeof(int32_t), cx.stream));
+    return f;
+}
+
 size_t kway_first_level_bytes(int32_t n, int32_t m, int32_t k, int32_t ncon) {
     ByteCounter c;
     const int64_t ntiles = ((int64_t)n + kTile - 1) / kTile;
@@ -327,12 +352,12 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
                     const Options& o, int32_t* d_part, KwayStats* st) {
     if (k < 2) throw Error(-2, "nparts must be >= 2");                  // kway.rs:31-33
     if (k > g.n) throw Error(-2, "nparts > nvtxs");                     // kway.rs:34-38
-    if (!o.rtype_greedy)
-        throw Error(-4, "rtype Fm is not provided by the GPU k-way path; set options[METIS_OPTION_RTYPE]=1 "
-                        "(Greedy)");
     hipStream_t s = cx.stream;
     const int32_t ncon = g.ncon;
     Buffers B = alloc_buffers(cx, g.n, g.m, k, ncon);
+    const bool fm = !o.rtype_greedy;                                    // kway.rs:154-161: Fm refines by k-way FM
+    FmBufs F;
+    if (fm) F = alloc_fm_buffers(cx, g.n, g.m, k, ncon);
     const std::vector<int32_t> total = total_weight_checked(cx, g, B);
     int32_t* fin_heavy = cx.arena.get<int32_t>(g.n);
     const int32_t fin_nheavy = heavy_list_sync(g, fin_heavy, B.rb, s);
@@ -449,11 +474,31 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
         LevelStat cstat;
         cstat.n = cg.n;
         cstat.m = cg.m;
-        {
-            const size_t li = nl - 1;
-            refine_level(cx, cg, k, d_thr + 2 * kc * li, d_thr + 2 * kc * li + kc, o.niter, pc,
-                         levels.back().nheavy, levels.back().heavy, B, st ? &cstat : nullptr);
-        }
+        // refine_level (kway.rs:145-162): greedy, or k-way FM
+        auto refine = [&](const DevGraph& lg, size_t li, int32_t* part, int nh, const int32_t* hl, LevelStat* ls,
+                          bool cleared, bool pwv) {
+            const long long* thr = d_thr + 2 * kc * li;
+            if (!fm) {
+                refine_level(cx, lg, k, thr, thr + kc, o.niter, part, nh, hl, B, ls, cleared, pwv);
+                return;
+            }
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (ls) {
+                RK_HIP(hipEventCreate(&e0));
+                RK_HIP(hipEventCreate(&e1));
+                RK_HIP(hipEventRecord(e0, s));
+            }
+            fm_refine_level(lg, k, thr, o.niter, part, nh, hl, B.rb, F, s, cleared, pwv);
+            if (ls) {
+                RK_HIP(hipEventRecord(e1, s));
+                RK_HIP(hipEventSynchronize(e1));
+                RK_HIP(hipEventElapsedTime(&ls->refine_ms, e0, e1));
+                ls->passes = o.niter;
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+            }
+        };
+        refine(cg, nl - 1, pc, levels.back().nheavy, levels.back().heavy, st ? &cstat : nullptr, false, false);
         // With explicit vertex weights a coarse vertex weighs the sum of its
         // fine vertices (graph/mod.rs:284-384), so the projected partition has
         // exactly the part weights the coarser level's last pass left in
@@ -467,14 +512,15 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
             const int32_t* flist = i == 0 ? fin_heavy : levels[i - 1].heavy;
             int32_t* pf = cx.arena.get<int32_t>(fine.n);
             project(fine.n, levels[i].cmap, pc, pf, s, keep_pw ? nullptr : B.rb.pw, kc, B.rb.done, 1);
-            refine_level(cx, fine, k, d_thr + 2 * kc * i, d_thr + 2 * kc * i + kc, o.niter, pf, fheavy, flist, B,
-                         st ? &lst[i] : nullptr, true, keep_pw);
+            refine(fine, (size_t)i, pf, fheavy, flist, st ? &lst[i] : nullptr, true, keep_pw);
             pc = pf;
         }
         edge_cut(g, pc, B.cut_acc, B.cut_out, pick_group(g.n, g.m), s);    // kway.rs:94
         RK_HIP(hipMemcpyAsync(B.rb.host_small, B.cut_out, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        if (fm) RK_HIP(hipMemcpyAsync(B.rb.host_small + 1, F.err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         RK_HIP(hipStreamSynchronize(s));
         const int32_t obj = B.rb.host_small[0];
+        if (fm && B.rb.host_small[1]) throw Error(-4, "FM pass exceeded its pop budget");
         if (!have_best || obj < best) {                                 // kway.rs:96-98
             have_best = true;
             best = obj;

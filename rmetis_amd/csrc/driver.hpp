@@ -73,6 +73,10 @@ struct Buffers {
 Buffers alloc_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon, int64_t tiles = -1,
                       int64_t kc_total = -1);
 
+// FM work buffers of a call whose finest level has n vertices and m entries
+// (the constant tables are filled on the context's stream).
+FmBufs alloc_fm_buffers(Context& cx, int32_t n, int32_t m, int32_t k, int32_t ncon);
+
 // Device bytes kway_device takes from its arena before the second
 // coarsening level: the work buffers, the finest heavy-vertex list and the
 // first coarse level (deeper levels take more, which the arena reports).

@@ -168,6 +168,37 @@ void refine_pass_batch(const DevGraph& g, const BatchLevel& bl, int mode, int nh
 void gain_table(const DevGraph& g, int32_t k, int nheavy, const int32_t* heavy_list, RefineBufs& b,
                 const int32_t* part, int4* vrec, int32_t* g0, int32_t* gains, hipStream_t s);
 
+// Candidate lists of every boundary vertex for k-way FM: the gain table of
+// gain_table() (pw0 all zeros and omin all INT64_MIN, so every part counts as
+// overweight), without a host synchronisation; a set *done skips the launch.
+void fm_candidates(const DevGraph& g, int32_t k, int nheavy, const int32_t* heavy_list, RefineBufs& b,
+                   const int32_t* part, const int32_t* pw0, const long long* omin, const int32_t* done, int4* vrec,
+                   int32_t* g0, int32_t* gains, hipStream_t s);
+
+// ---- fm.hip ----
+// Work buffers of the k-way FM passes of one call (sized for its finest level).
+struct FmBufs {
+    uint64_t* tree = nullptr;     // queue keys: n leaves, then the 64-ary max tree above them
+    int32_t* tp = nullptr;        // [n] best target part of each queued vertex
+    uint8_t* locked = nullptr;    // [n]
+    int4* vrec = nullptr;         // [n] candidate records
+    int32_t* g0 = nullptr;        // [n]
+    int32_t* gains = nullptr;     // [m] candidate gains beside RefineBufs::pool
+    int32_t* pw0 = nullptr;       // [k*ncon] zeros
+    long long* omin = nullptr;    // [k*ncon] INT64_MIN
+    int32_t* ovf = nullptr;       // [k] dense part table for recomputes when k exceeds the LDS table
+    int32_t* err = nullptr;       // pop budget exhausted (never expected)
+};
+// One launch: pw0 = 0, omin = INT64_MIN (kc entries each).
+void fm_fill_tables(int32_t* pw0, long long* omin, int64_t kc, hipStream_t s);
+// fm_kway_refine (refine/fm_kway.rs:12-30) on one level: b.pw holds (or, when
+// !pw_valid, receives) partition_weights(part), *b.done is cleared (or
+// cleared already: cleared), niter passes run, each a no-op once a pass kept
+// no move.  No host synchronisation.
+void fm_refine_level(const DevGraph& g, int32_t k, const long long* d_fmax, int niter, int32_t* part, int nheavy,
+                     const int32_t* heavy_list, RefineBufs& b, FmBufs& f, hipStream_t s, bool cleared,
+                     bool pw_valid);
+
 // ---- coarsen.hip ----
 struct CoarsenBufs {
     int32_t* match = nullptr;      // [n]

@@ -584,6 +584,16 @@ void gain_table(const DevGraph& g, int32_t k, int nheavy, const int32_t* heavy_l
     RK_HIP(hipGetLastError());
 }
 
+void fm_candidates(const DevGraph& g, int32_t k, int nheavy, const int32_t* heavy_list, RefineBufs& b,
+                   const int32_t* part, const int32_t* pw0, const long long* omin, const int32_t* done, int4* vrec,
+                   int32_t* g0, int32_t* gains, hipStream_t s) {
+    if (g.n <= 0) return;
+    GainArgs ga{g.n, g.ncon, k, 0, g.xadj, g.adjncy, g.adjwgt, g.vwgt, part, pw0, omin,
+                b.heavy_rec, b.pool, b.tile_rec, b.tile_cnt, b.tile_bits, done, vrec, g0, gains};
+    launch_gain(ga, nheavy, heavy_list, b, s);
+    RK_HIP(hipGetLastError());
+}
+
 void heavy_list_build(int32_t n_max, const int32_t* n_dev, const int32_t* xadj, int32_t* list, int32_t* count,
                       hipStream_t s, bool count_zeroed) {
     if (!count_zeroed) RK_HIP(hipMemsetAsync(count, 0, sizeof(int32_t), s));

@@ -495,3 +495,164 @@ def test_part_graph_kway_is_the_composition_of_its_pinned_pieces(case):
                                               ctype_shem=kw.get("shem", True), ncuts=kw.get("ncuts", 1),
                                               tpwgts=kw.get("tpwgts"))
         assert rc == 1 and o_obj == obj and np.array_equal(o_part, part)
+
+
+# ---------------------------------------------------------------- k-way FM
+def _fm_pass_transcribed(x, a, aw, vw, ncon, part, k, tpw, ub, pw):
+    """refine/fm_kway.rs:33-158 transcribed independently: a binary heap of
+    (-gain, v) with lazy deletion (a popped entry counts only if it is the
+    vertex's current key), equal gains smallest vertex first."""
+    import heapq
+    n = len(x) - 1
+    part = list(map(int, part))
+    pw = [list(map(int, row)) for row in np.asarray(pw).reshape(k, ncon)]
+    w = (lambda j: int(aw[j])) if aw is not None else (lambda j: 1)
+    vwt = (lambda v, c: int(vw[v * ncon + c])) if vw is not None else (lambda v, c: 1)
+
+    def best(v):
+        et = {}
+        for j in range(x[v], x[v + 1]):
+            et[part[a[j]]] = et.get(part[a[j]], 0) + w(j)
+        bp, bg = None, 0
+        for p in range(k):
+            if p != part[v] and et.get(p, 0) != 0:
+                g = et[p] - et.get(part[v], 0)
+                if g > bg:
+                    bp, bg = p, g
+        return (bp, bg) if bp is not None else None
+
+    cur = {}
+    heap = []
+    for v in range(n):
+        if any(part[a[j]] != part[v] for j in range(x[v], x[v + 1])):
+            b = best(v)
+            if b:
+                cur[v] = b
+                heapq.heappush(heap, (-b[1], v))
+    locked = [False] * n
+    moves = 0
+    while heap:
+        g, v = heapq.heappop(heap)
+        if v not in cur or cur[v][1] != -g:
+            continue                                   # stale entry
+        to = cur.pop(v)[0]
+        if not all(pw[to][c] + vwt(v, c) <= tpw[to * ncon + c] * ub[c] for c in range(ncon)):
+            continue
+        frm = part[v]
+        part[v] = to
+        for c in range(ncon):
+            pw[frm][c] -= vwt(v, c)
+            pw[to][c] += vwt(v, c)
+        locked[v] = True
+        moves += 1
+        for j in range(x[v], x[v + 1]):
+            u = int(a[j])
+            if locked[u]:
+                continue
+            b = best(u)
+            if b:
+                if u not in cur or cur[u] != b:
+                    cur[u] = b
+                    heapq.heappush(heap, (-b[1], u))
+            else:
+                cur.pop(u, None)
+    return moves > 0, np.array(part, np.int32), np.array(pw, np.int32), moves
+
+
+def test_fm_reference_case():
+    # fm_kway.rs:209-219: grid3x3, k=3, row stripes, tpwgts 3, ub 1.1
+    x, a = grid_sorted(3, 3)
+    part = np.arange(9) // 3
+    before = O.edge_cut(x, a, None, part)
+    after_part, after = O.fm_kway_refine(1, x, a, None, None, part, 3, [3.0] * 3, [1.1], 5)
+    assert after <= before and after == O.edge_cut(x, a, None, after_part)
+
+
+def test_fm_pass_vs_independent_transcription():
+    rng = np.random.default_rng(8)
+    checked = 0
+    for trial in range(120):
+        n = int(rng.integers(4, 60))
+        x, a, aw = random_graph(n, float(rng.choice([0.05, 0.15, 0.4])), 100 + trial,
+                                wmax=int(rng.choice([1, 3, 9])))
+        ncon = int(rng.integers(1, 3))
+        k = int(rng.integers(2, 6))
+        vw = rng.integers(1, 4, n * ncon).astype(np.int32) if trial % 2 else None
+        part = rng.integers(0, k, n).astype(np.int32)
+        tot = (vw.reshape(n, ncon).sum(0) if vw is not None else np.full(ncon, n)).astype(np.float64)
+        tpw = np.tile(tot / k, k)
+        ub = np.full(ncon, float(rng.choice([1.0, 1.03, 1.2, 1.6])))
+        pw = O.partition_weights(ncon, vw, part, k)
+        ok_o, p_o, pw_o, nm_o = O.fm_kway_pass(ncon, x, a, vw, aw, part, k, tpw, ub, pw)
+        ok_t, p_t, pw_t, nm_t = _fm_pass_transcribed(x, a, aw, vw, ncon, part, k, tpw, ub, pw)
+        assert (ok_o, nm_o) == (ok_t, nm_t) and np.array_equal(p_o, p_t) and np.array_equal(pw_o, pw_t), trial
+        checked += nm_o > 0
+    assert checked > 60
+
+
+def test_fm_invariants():
+    # every kept move has positive gain, so the cut falls by exactly their
+    # sum and the rollback keeps every move; after a pass that moves
+    # nothing, no vertex's best move (fm_kway.rs:47-72) is feasible
+    rng = np.random.default_rng(9)
+    for trial in range(30):
+        n = int(rng.integers(30, 200))
+        x, a, aw = random_graph(n, 0.05, 300 + trial, wmax=5)
+        k = int(rng.integers(2, 7))
+        part = rng.integers(0, k, n).astype(np.int32)
+        tpw = np.full(k, n / k)
+        ub = np.array([1.05])
+        pw = O.partition_weights(1, None, part, k)
+        before = O.edge_cut(x, a, aw, part)
+        ok, p1, pw1, nm = O.fm_kway_pass(1, x, a, None, aw, part, k, tpw, ub, pw)
+        after = O.edge_cut(x, a, aw, p1)
+        assert ok == (nm > 0) and (after < before if nm else after == before)
+        assert np.array_equal(pw1, O.partition_weights(1, None, p1, k))
+        assert (pw1[:, 0] <= np.floor(tpw * ub[0]) + 0).all() or (pw[:, 0] > np.floor(tpw * ub[0])).any()
+        p_f, cut_f = O.fm_kway_refine(1, x, a, None, aw, part, k, tpw, ub, 50)
+        assert cut_f == O.edge_cut(x, a, aw, p_f) and cut_f <= before
+        pwf = O.partition_weights(1, None, p_f, k)[:, 0]
+        for v in range(n):
+            et = np.zeros(k, np.int64)
+            for j in range(x[v], x[v + 1]):
+                et[p_f[a[j]]] += aw[j] if aw is not None else 1
+            gains = [(et[p] - et[p_f[v]], -p) for p in range(k) if p != p_f[v] and et[p] != 0]
+            if gains and max(gains)[0] > 0:
+                bp = -max(gains)[1]
+                assert pwf[bp] + 1 > tpw[bp] * ub[0], (trial, v)
+
+
+def test_part_graph_kway_fm_is_the_composition():
+    # rtype Fm (kway.rs:158-160): the same driver with fm_kway_refine per level
+    import rmetis_amd.synth as S
+    g = S.rgg(1200, 4, vwgt_max=2)
+    for seed in (1, 3):
+        obj, part = _kway_composed_fm(g.xadj, g.adjncy, 8, seed, g.vwgt, g.adjwgt)
+        rc, o_obj, o_part = O.part_graph_kway(g.xadj, g.adjncy, 8, vwgt=g.vwgt, adjwgt=g.adjwgt, seed=seed,
+                                              rtype_greedy=False)
+        assert rc == 1 and o_obj == obj and np.array_equal(o_part, part)
+
+
+def _kway_composed_fm(x, a, k, seed, vwgt, adjwgt, niter=10, ufactor=30):
+    ub = np.full(1, 1.0 + ufactor / 1000.0)
+    r = O.rng(seed)
+    cur = (x, a, vwgt, adjwgt)
+    levels = []
+    while True:
+        cx, ca, cvw, caw = cur
+        m = O.shem_matching(cx, ca, caw, r)
+        cg = O.build_coarse_graph(1, cx, ca, cvw, caw, None, m)
+        levels.append((cur, cg["cmap"]))
+        ratio = np.float32(cg["n"]) / np.float32(len(cx) - 1)
+        cur = (cg["cxadj"], cg["cadjncy"], cg["cvwgt"], cg["cadjwgt"] if cg["adjwgt_present"] else None)
+        if cg["n"] <= max(20 * k, 20) or ratio > np.float32(0.95):
+            break
+    cx, ca, cvw, caw = cur
+    tpw = O.target_weights(len(cx) - 1, 1, cvw, k)
+    part = O.grow_partition(1, cx, ca, cvw, k, tpw, r)
+    part, _ = O.fm_kway_refine(1, cx, ca, cvw, caw, part, k, tpw, ub, niter)
+    for (fx, fa, fvw, faw), cmap in reversed(levels):
+        part = part[cmap]
+        tpw = O.target_weights(len(fx) - 1, 1, fvw, k)
+        part, _ = O.fm_kway_refine(1, fx, fa, fvw, faw, part, k, tpw, ub, niter)
+    return O.edge_cut(x, a, adjwgt, part), part
