@@ -123,5 +123,5 @@ def test_batch_errors():
     bad[4] = 101                      # graph 4 has 100 vertices
     with pytest.raises(R.InputError):
         R.part_graph_kway_batch(graphs, bad, options=opts)
-    with pytest.raises(R.InternalError):
-        R.part_graph_kway_batch(graphs, ks, options=R.Options.for_kway(seed=1))   # Fm refinement
+    with pytest.raises(R.InputError):
+        R.part_graph_kway_batch(graphs, ks, options=R.Options.for_kway(seed=1))   # Fm: single-graph calls only

@@ -596,8 +596,8 @@ def test_kway_errors():
         R.part_graph_kway(g, 10, options=R.Options(rtype=R.RType.Greedy))
     with pytest.raises(R.InputError):
         R.part_graph_kway(g, 1, options=R.Options(rtype=R.RType.Greedy))
-    with pytest.raises(R.InternalError):
-        R.part_graph_kway(g, 2, options=R.Options())   # FM is not provided
+    r = R.part_graph_kway(g, 2, options=R.Options(seed=1))   # the default RTYPE Fm runs k-way FM
+    assert r.objval == g.edge_cut(r.part)
 
 
 def test_kway_weighted():
