@@ -213,6 +213,8 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="issue one device-to-device copy of this many bytes first (a kernel of known "
                          "traffic, to scale FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--rtype", default="greedy", choices=["greedy", "fm"],
+                    help="single-graph workloads: refinement (fm: the reference's default k-way FM)")
     ap.add_argument("--roofline-tag", default="",
                     help="kernel family of the roofline (gain|match|contract; default: the dominant one)")
     args = ap.parse_args()
@@ -265,7 +267,8 @@ def bench_single(args, workload, steps=None, warmup=None, extra=False):
     d_aw = DA(g.adjwgt) if g.adjwgt is not None else None
     d_part = DA(count=g.n, dtype=np.int32)
     L.rmetis_hip_device_synchronize()
-    opts = R.Options(rtype=R.RType.Greedy, seed=args.seed, ncuts=1, niter=10, ufactor=30).to_c_array()
+    rtype = R.RType.Fm if (args.rtype == "fm" and not extra) else R.RType.Greedy
+    opts = R.Options(rtype=rtype, seed=args.seed, ncuts=1, niter=10, ufactor=30).to_c_array()
     stats = _lib.HipStats()
     obj = C.c_int32()
 
@@ -304,7 +307,8 @@ def bench_single(args, workload, steps=None, warmup=None, extra=False):
         "value": round(value, 4), "unit": "graphs/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": round(1e3 * elapsed / steps, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
-        "config": {"workload": f"{workload}: {g.name} n={g.n} m={g.m} k={k}", "ctype": "shem", "rtype": "greedy",
+        "config": {"workload": f"{workload}: {g.name} n={g.n} m={g.m} k={k}", "ctype": "shem",
+                   "rtype": "fm" if rtype == R.RType.Fm else "greedy",
                    "ufactor": 30, "niter": 10, "ncuts": 1, "seed": args.seed, "parallelism": "single GPU"},
         "edge_cut": int(obj.value), "balance_violations": violations, "edges_per_s": round(g.m * value, 1),
         "kernel_ms_per_step": {n: round(v, 3) for n, v in total.items()},
@@ -335,7 +339,8 @@ def bench_single(args, workload, steps=None, warmup=None, extra=False):
                            "note": "RMETIS_PartGraphKway on host buffers: CSR upload, validation and the "
                                    "partition download included"}
     if not args.no_cpu:
-        kw = dict(vwgt=g.vwgt, adjwgt=g.adjwgt, ncon=g.ncon, niter=10, ufactor=30)
+        kw = dict(vwgt=g.vwgt, adjwgt=g.adjwgt, ncon=g.ncon, niter=10, ufactor=30,
+                  rtype_greedy=rtype == R.RType.Greedy)
         jobs = [lambda O, s=s: O.part_graph_kway(g.xadj, g.adjncy, k, seed=s, **kw) for s in range(1, 9)]
         out["cpu_baseline"] = cpu_baseline(jobs, args.cpu_budget_s, args.cpu_pool_s,
                                            f"full k-way partitions of the same graph, seeds 1..8")

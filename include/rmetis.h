@@ -237,6 +237,16 @@ int32_t rmetis_hip_fm_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const i
 int32_t rmetis_hip_match(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* adjwgt, idx_t ctype,
                          uint64_t seed, idx_t* matching);
 
+/* The same matching with a weight cap on the matched pairs, as the k-way
+ * driver runs it on weighted levels: a pair (u, v) is matched only when
+ * vwgt[u][c] + vwgt[v][c] <= cap[c] for every constraint c (cap[ncon]; each
+ * 0 <= cap[c] < INT64_MAX).  The result is the greedy matching of the same
+ * edge order restricted to the pairs that fit.  The reference's matching has
+ * no cap (src/coarsen/shem.rs:13-56): this entry exists to test the GPU's. */
+int32_t rmetis_hip_match_capped(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy, const idx_t* vwgt,
+                                const idx_t* adjwgt, idx_t ctype, uint64_t seed, const int64_t* cap,
+                                idx_t* matching);
+
 /* Graph::build_coarse_graph, src/graph/mod.rs:284-384, for a symmetric
  * matching.  Output capacities: cxadj n+1, cadjncy/cadjwgt xadj[n],
  * cvwgt n*ncon, cvsize n, cmap n.  *cn / *cm receive the coarse sizes;

@@ -304,11 +304,20 @@ def greedy_pass(graph, part, nparts, tpwgts, ubvec, mode, pw):
     return part, pw.reshape(nparts, graph.ncon), moves[:2 * nm.value].reshape(-1, 2)
 
 
-def gpu_matching(graph, seed, ctype=CType.Shem):
-    """The seeded GPU matching that replaces shem/rm matching."""
+def gpu_matching(graph, seed, ctype=CType.Shem, cap=None):
+    """The seeded GPU matching that replaces shem/rm matching; cap[ncon]
+    (int64) limits the matched pairs' summed vertex weights."""
     m = np.zeros(graph.nvtxs, np.int32)
-    _raise(_lib.lib().rmetis_hip_match(graph.nvtxs, _p(graph.xadj), _p(graph.adjncy), _p(graph.adjwgt),
-                                       int(ctype), C.c_uint64(seed), _p(m)))
+    if cap is None:
+        _raise(_lib.lib().rmetis_hip_match(graph.nvtxs, _p(graph.xadj), _p(graph.adjncy), _p(graph.adjwgt),
+                                           int(ctype), C.c_uint64(seed), _p(m)))
+    else:
+        cp = np.ascontiguousarray(cap, np.int64)
+        if cp.shape != (graph.ncon,) or graph.vwgt is None:
+            raise InputError("cap needs one entry per constraint and a weighted graph")
+        _raise(_lib.lib().rmetis_hip_match_capped(graph.nvtxs, graph.ncon, _p(graph.xadj), _p(graph.adjncy),
+                                                  _p(graph.vwgt), _p(graph.adjwgt), int(ctype), C.c_uint64(seed),
+                                                  _p(cp, _lib.i64p), _p(m)))
     return m
 
 

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "librmetis_hip.so")
 i32p = C.POINTER(C.c_int32)
 f32p = C.POINTER(C.c_float)
 f64p = C.POINTER(C.c_double)
+i64p = C.POINTER(C.c_int64)
 
 MAX_LEVELS = 64
 
@@ -61,6 +62,8 @@ SIGNATURES = {
     "rmetis_hip_fm_refine": (C.c_int32, [C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, i32p, C.c_int32,
                                          f64p, f64p, C.c_int32, i32p]),
     "rmetis_hip_match": (C.c_int32, [C.c_int32, i32p, i32p, i32p, C.c_int32, C.c_uint64, i32p]),
+    "rmetis_hip_match_capped": (C.c_int32, [C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, C.c_int32, C.c_uint64,
+                                            i64p, i32p]),
     "rmetis_hip_build_coarse_graph": (C.c_int32, [C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p,
                                                   i32p, i32p, i32p, i32p, i32p, i32p, i32p, i32p, i32p]),
     "rmetis_hip_validate_csr": (C.c_int32, [C.c_int32, C.c_int32, i32p, C.c_int64, i32p, C.c_int64, i32p,

@@ -136,15 +136,17 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
     // the thresholds are rebuilt per level.
     const bool unit = g.vwgt == nullptr;
     std::vector<std::vector<int32_t>> total(G, std::vector<int32_t>(ncon));
+    std::vector<long long> gmaxw;                 // heaviest input vertex per graph and constraint
     if (unit) {
         for (int32_t i = 0; i < G; ++i)
             for (int c = 0; c < ncon; ++c) total[i][c] = gvtx[i + 1] - gvtx[i];
     } else {
-        long long* d_sum = cx.arena.get<long long>((int64_t)G * ncon);
-        graph_column_sums(g, d_gvtx, G, d_sum, s);
-        long long* h = (long long*)cx.host_stage2(sizeof(long long) * (size_t)G * ncon);
-        RK_HIP(hipMemcpyAsync(h, d_sum, sizeof(long long) * (size_t)G * ncon, hipMemcpyDeviceToHost, s));
+        long long* d_sum = cx.arena.get<long long>(2 * (int64_t)G * ncon);
+        graph_column_sums(g, d_gvtx, G, d_sum, s, d_sum + (int64_t)G * ncon);
+        long long* h = (long long*)cx.host_stage2(sizeof(long long) * 2 * (size_t)G * ncon);
+        RK_HIP(hipMemcpyAsync(h, d_sum, sizeof(long long) * 2 * (size_t)G * ncon, hipMemcpyDeviceToHost, s));
         RK_HIP(hipStreamSynchronize(s));
+        gmaxw.assign(h + (size_t)G * ncon, h + 2 * (size_t)G * ncon);
         for (int32_t i = 0; i < G; ++i)
             for (int c = 0; c < ncon; ++c) {
                 const long long v = h[(size_t)i * ncon + c];
@@ -169,6 +171,31 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
         put(cx, om, B.rb.omax);
     };
     if (!unit) upload_thresholds([&](int32_t i, int32_t* t) { std::copy(total[i].begin(), total[i].end(), t); });
+    // The matched pairs' weight cap of every graph (match_weight_cap) and the
+    // first level where it can bind (a level-l vertex weighs at most 2^l
+    // times the graph's heaviest input vertex); a level's table holds the
+    // caps of the graphs where they can bind, INT64_MAX for the others.
+    if (unit) gmaxw.assign((size_t)G * ncon, 1);
+    std::vector<long long> gcap((size_t)G * ncon);
+    std::vector<int> gcap_from(G, 64);
+    for (int32_t i = 0; i < G; ++i)
+        for (int c = 0; c < ncon; ++c) {
+            const long long cp = match_weight_cap(total[i][c], gvtx[i + 1] - gvtx[i], gk[i]);
+            gcap[(size_t)i * ncon + c] = cp;
+            int l = 0;
+            while (l < 62 && (gmaxw[(size_t)i * ncon + c] << (l + 1)) <= cp) ++l;
+            gcap_from[i] = std::min(gcap_from[i], l);
+        }
+    auto level_caps = [&](size_t l) -> long long* {
+        std::vector<long long> t((size_t)G * ncon, LLONG_MAX);
+        bool any = false;
+        for (int32_t i = 0; i < G; ++i)
+            if ((int)l >= gcap_from[i]) {
+                any = true;
+                for (int c = 0; c < ncon; ++c) t[(size_t)i * ncon + c] = gcap[(size_t)i * ncon + c];
+            }
+        return any ? put(cx, t) : nullptr;
+    };
     int32_t* fin_heavy = cx.arena.get<int32_t>(g.n);
     const int32_t fin_nheavy = heavy_list_sync(g, fin_heavy, B.rb, s);
     const std::vector<Extent> ext0 = read_extents(cx, g, gseg0, G, d_ranges);
@@ -224,7 +251,7 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
             gs.cgseg = cx.arena.get<int32_t>(L.g.n);
             {
                 ProfScope ps(kProfMatch, match_bytes(L.g.n, L.g.m, L.g.adjwgt != nullptr), L.g.n, s);
-                match_graph(L.g, o.ctype_shem, 0, B.cb, s, &gs);
+                match_graph(L.g, o.ctype_shem, 0, B.cb, s, &gs, level_caps(l));
             }
             BLevel C;
             C.cmap = cx.arena.get<int32_t>(L.g.n);

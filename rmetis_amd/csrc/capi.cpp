@@ -7,6 +7,7 @@
 // (196-204) and the error-code mapping (205-207).  Every call runs on its own
 // pooled stream and arena, so concurrent calls are safe.
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -698,6 +699,28 @@ int32_t rmetis_hip_match(idx_t nvtxs, const idx_t* xadj, const idx_t* adjncy, co
         DevGraph g = upload_graph(*cx.c, nvtxs, 1, xadj, adjncy, nullptr, adjwgt, nullptr);
         Buffers B = alloc_buffers(*cx.c, g.n, g.m, 2, 1);
         match_graph(g, ctype != 0, seed, B.cb, cx->stream);
+        download(*cx.c, matching, B.cb.match, nvtxs);
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_match_capped(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy, const idx_t* vwgt,
+                                const idx_t* adjwgt, idx_t ctype, uint64_t seed, const int64_t* cap,
+                                idx_t* matching) {
+    return guarded([&]() -> int32_t {
+        if (!xadj || !adjncy || !vwgt || !cap || !matching || nvtxs <= 0 || ncon <= 0) return METIS_ERROR_INPUT;
+        if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt)) return METIS_ERROR_INPUT;
+        for (int32_t c = 0; c < ncon; ++c)
+            if (cap[c] < 0 || cap[c] == LLONG_MAX) return METIS_ERROR_INPUT;
+        ContextGuard cx;
+        DevGraph g = upload_graph(*cx.c, nvtxs, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
+        Buffers B = alloc_buffers(*cx.c, g.n, g.m, 2, ncon);
+        long long* d_cap = cx->arena.get<long long>(ncon);
+        long long* h = (long long*)cx->upload_stage(sizeof(long long) * ncon);
+        for (int32_t c = 0; c < ncon; ++c) h[c] = cap[c];
+        RK_HIP(hipMemcpyAsync(d_cap, h, sizeof(long long) * ncon, hipMemcpyHostToDevice, cx->stream));
+        match_graph(g, ctype != 0, seed, B.cb, cx->stream, nullptr, d_cap);
         download(*cx.c, matching, B.cb.match, nvtxs);
         RK_HIP(hipStreamSynchronize(cx->stream));
         return METIS_OK;

@@ -39,15 +39,23 @@ __device__ inline uint32_t mix32(uint32_t x) {
     return x;
 }
 
+// Pairs whose weight would exceed a cap (1.5 x a part's share of the
+// coarsest level's vertices, see match_weight_cap) are not matched: without
+// it a component can shrink to a few vertices that each outweigh a part
+// while isolated vertices, which never match, keep the level above the
+// coarsening limit; growing a part from such a vertex takes the whole
+// component and no boundary is left to repair the balance.
+//
 // Strict total order on edges, symmetric in (v, u).  Heavy-edge matching:
-// (smaller floor(log2) of the endpoints' smaller degree first, heavier
-// edge, seeded hash) -- an edge is ranked by roughly when the reference's
-// degree-ordered scan (shem.rs:23) would first reach it and by the weight its
-// visiting endpoint maximises (shem.rs:36-44).  Degrees are compared by
-// their binary magnitude, not exactly: exact degrees chain the greedy
-// matching's decisions along long paths of slowly rising degree (the
-// matching then takes many dependent steps), while magnitudes keep almost
-// all of its size.  Random matching: the seeded hash alone.
+// (smaller rank of the endpoints' smaller degree first, heavier edge, seeded
+// hash) -- an edge is ranked by roughly when the reference's degree-ordered
+// scan (shem.rs:23) would first reach it and by the weight its visiting
+// endpoint maximises (shem.rs:36-44).  Degrees up to 4 rank exactly (grid
+// corners, sides and interiors differ), larger ones by binary magnitude:
+// exact large degrees chain the greedy matching's decisions along long paths
+// of slowly rising degree (the matching then takes many dependent steps),
+// while magnitudes keep almost all of its size and let weights decide more
+// often.  Random matching: the seeded hash alone.
 struct EdgeKey {
     uint64_t hi;
     uint32_t lo;
@@ -55,12 +63,19 @@ struct EdgeKey {
     __device__ bool eq(const EdgeKey& o) const { return hi == o.hi && lo == o.lo; }
 };
 
+// Rank of a degree d >= 1 in the edge order: d itself up to 4, then one
+// rank per binary magnitude (5-7 share 4, 8-15 are 5, 16-31 are 6, ...).
+__device__ inline uint32_t degree_rank(int32_t d) {
+    d = max(d, 1);
+    return (uint32_t)(min(d, 4) + max(0, 29 - (int)__clz(d)));
+}
+
 // Key of edge (v, u) of weight w whose endpoints' smaller degree is mdeg.
 __device__ inline EdgeKey edge_key_m(int32_t v, int32_t u, int32_t w, int32_t mdeg, uint64_t seed, int shem) {
     const uint32_t lo = (uint32_t)min(v, u), hi = (uint32_t)max(v, u);
     const uint32_t h = mix32(lo ^ mix32(hi ^ (uint32_t)seed) ^ (uint32_t)(seed >> 32));
     if (!shem) return EdgeKey{0ull, h};
-    const uint32_t dpri = 0xFFFFFFFFu - (uint32_t)(31 - __clz(max(mdeg, 1)));
+    const uint32_t dpri = 0xFFFFFFFFu - degree_rank(mdeg);
     return EdgeKey{((uint64_t)dpri << 32) | (uint64_t)((uint32_t)w ^ 0x80000000u), h};
 }
 
@@ -118,6 +133,12 @@ struct MatchArgs {
     const int32_t* adjwgt;
     int32_t* match;
     uint64_t* suitor;
+    // weight cap (null: none): a pair is matched only when, for every
+    // constraint c, vwgt[u][c] + vwgt[v][c] <= cap[c] (cap[g * ncon + c] for
+    // graph g of a batched level)
+    const int32_t* vwgt;
+    const long long* cap;
+    int32_t ncon;
     // batched levels (null for a single graph): graph of each vertex, and per
     // graph its first vertex, matching seed and whether it is matched here
     const int32_t* gseg;
@@ -130,47 +151,45 @@ __device__ inline uint64_t ld_agent64(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ inline bool pair_fits(const MatchArgs& a, int32_t x, int32_t y, const long long* cap) {
+    for (int c = 0; c < a.ncon; ++c)
+        if ((long long)a.vwgt[(int64_t)x * a.ncon + c] + a.vwgt[(int64_t)y * a.ncon + c] > cap[c]) return false;
+    return true;
+}
+
 __global__ __launch_bounds__(kBlock) void k_suitor_init(int32_t n, uint64_t* __restrict__ suitor) {
     for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock)
         suitor[v] = kNoSuitor;
 }
 
-// G lanes per searching vertex.  A vertex v of the level starts a chain: the
-// current searcher scans its neighbours for the best one (edge order) whose
-// suitor it beats, and claims it with a 64-bit CAS on the suitor word it
-// compared against; on success the displaced suitor (if any) becomes the
-// searcher, on failure the scan is repeated.  Keys of a claimed word only
-// grow, so a stale read can only make a neighbour look easier to claim (the
-// CAS then fails and returns the current word, which is used for that
-// neighbour in the next scan): every retry sees a newer word, and the chain
-// ends.
-template <int G>
-__global__ __launch_bounds__(kBlock) void k_suitor(MatchArgs a) {
-    constexpr int NG = kBlock / G;
-    const int gl = group_lane<G>();
-    for (int64_t v = (int64_t)blockIdx.x * NG + threadIdx.x / G; v < a.n; v += (int64_t)gridDim.x * NG) {
-        if (a.xadj[v + 1] == a.xadj[v]) continue;                 // group-uniform
-        // keys use graph-local ids and the graph's seed, so a graph of a batch
-        // is matched exactly as it is alone
-        uint64_t seed = a.seed;
-        int32_t base = 0;
-        if (a.gseg) {
-            const int32_t gi = a.gseg[v];
-            if (!a.gon[gi]) continue;
-            seed = a.gseed[gi];
-            base = a.gv0[gi];
-        }
-        int32_t cur = (int32_t)v;
-        int32_t fu = -1;          // neighbour of the last failed claim and the word the CAS returned
-        uint64_t fword = 0;
-        while (cur >= 0) {
-            const int32_t beg = a.xadj[cur], end = a.xadj[cur + 1];
-            const int32_t dc = end - beg;
-            EdgeKey best{0ull, 0u};
-            int32_t bu = -1, bw = 0, bd = 0;
-            uint64_t bseen = 0;
+// One chain from vertex v (see k_suitor); CAPPED applies the weight cap
+// (cap non-null).  Inlined twice so that graphs whose cap cannot bind run
+// the uncapped code.
+template <int G, bool CAPPED>
+__device__ __forceinline__ void suitor_chain(const MatchArgs& a, int32_t v, int gl, uint64_t seed, int32_t base,
+                                             const long long* cap) {
+    int32_t cur = (int32_t)v;
+    int32_t fu = -1;          // neighbour of the last failed claim and the word the CAS returned
+    uint64_t fword = 0;
+    while (cur >= 0) {
+        const int32_t beg = a.xadj[cur], end = a.xadj[cur + 1];
+        const int32_t dc = end - beg;
+        EdgeKey best{0ull, 0u};
+        int32_t bu = -1, bw = 0, bd = 0;
+        uint64_t bseen = 0;
+        // The best claimable neighbour, first over every neighbour: when
+        // that pair fits the weight cap it is also the best among the
+        // fitting ones; only otherwise is the scan repeated with the cap
+        // applied to each neighbour (the weights are read only then).
+        for (bool filt = false;;) {
+            best = EdgeKey{0ull, 0u};
+            bu = -1;
+            bw = 0;
+            bd = 0;
+            bseen = 0;
             for (int32_t j = beg + gl; j < end; j += G) {
                 const int32_t u = a.adjncy[j];
+                if (CAPPED && filt && !pair_fits(a, cur, u, cap)) continue;   // the pair would be too heavy
                 const int32_t w = a.adjwgt ? a.adjwgt[j] : 1;
                 uint64_t sv = ld_agent64(&a.suitor[u]);
                 const int32_t du = a.xadj[u + 1] - a.xadj[u];
@@ -215,33 +234,73 @@ __global__ __launch_bounds__(kBlock) void k_suitor(MatchArgs a) {
                     bseen = os;
                 }
             }
-            if (bu < 0) break;                                      // nothing to claim: cur stays unmatched
-            // Claim bu.  When the word changed meanwhile but its new suitor is
-            // still beaten, claim again at once: bu stays the best choice
-            // (the other neighbours can only have become harder to claim).
-            uint64_t old = 0;
-            if (gl == 0) {
-                const unsigned long long mine = suitor_word(cur, bw, min(dc, bd));
-                for (;;) {
-                    old = atomicCAS((unsigned long long*)&a.suitor[bu], (unsigned long long)bseen, mine);
-                    if (old == bseen) break;
-                    const int32_t sid = suitor_id(old);
-                    if (sid == cur) break;
-                    const EdgeKey ks = suitor_key(old, sid, bu, bd, a.xadj, seed, a.shem, base);
-                    if (!edge_before(best, cur - base, bu - base, ks, sid - base, bu - base)) break;
-                    bseen = old;                                        // still beaten: claim against it
-                }
+            if (CAPPED && !filt && cap && bu >= 0 && !pair_fits(a, cur, bu, cap)) {   // group-uniform
+                filt = true;
+                continue;
             }
-            old = __shfl(old, 0, G);
-            bseen = __shfl(bseen, 0, G);
-            if (old == bseen) {
-                cur = suitor_id(bseen);                                // the displaced suitor searches again
-                fu = -1;
-            } else {
-                fu = bu;
-                fword = old;
+            break;
+        }
+        if (bu < 0) break;                                      // nothing to claim: cur stays unmatched
+        // Claim bu.  When the word changed meanwhile but its new suitor is
+        // still beaten, claim again at once: bu stays the best choice
+        // (the other neighbours can only have become harder to claim).
+        uint64_t old = 0;
+        if (gl == 0) {
+            const unsigned long long mine = suitor_word(cur, bw, min(dc, bd));
+            for (;;) {
+                old = atomicCAS((unsigned long long*)&a.suitor[bu], (unsigned long long)bseen, mine);
+                if (old == bseen) break;
+                const int32_t sid = suitor_id(old);
+                if (sid == cur) break;
+                const EdgeKey ks = suitor_key(old, sid, bu, bd, a.xadj, seed, a.shem, base);
+                if (!edge_before(best, cur - base, bu - base, ks, sid - base, bu - base)) break;
+                bseen = old;                                        // still beaten: claim against it
             }
         }
+        old = __shfl(old, 0, G);
+        bseen = __shfl(bseen, 0, G);
+        if (old == bseen) {
+            cur = suitor_id(bseen);                                // the displaced suitor searches again
+            fu = -1;
+        } else {
+            fu = bu;
+            fword = old;
+        }
+    }
+}
+
+// G lanes per searching vertex.  A vertex v of the level starts a chain: the
+// current searcher scans its neighbours for the best one (edge order) whose
+// suitor it beats, and claims it with a 64-bit CAS on the suitor word it
+// compared against; on success the displaced suitor (if any) becomes the
+// searcher, on failure the scan is repeated.  Keys of a claimed word only
+// grow, so a stale read can only make a neighbour look easier to claim (the
+// CAS then fails and returns the current word, which is used for that
+// neighbour in the next scan): every retry sees a newer word, and the chain
+// ends.
+template <int G, bool CAPPED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_suitor(MatchArgs a) {
+    constexpr int NG = kBlock / G;
+    const int gl = group_lane<G>();
+    for (int64_t v = (int64_t)blockIdx.x * NG + threadIdx.x / G; v < a.n; v += (int64_t)gridDim.x * NG) {
+        if (a.xadj[v + 1] == a.xadj[v]) continue;                 // group-uniform
+        // keys use graph-local ids and the graph's seed, so a graph of a batch
+        // is matched exactly as it is alone
+        uint64_t seed = a.seed;
+        int32_t base = 0;
+        const long long* cap = CAPPED ? a.cap : nullptr;
+        if (a.gseg) {
+            const int32_t gi = a.gseg[v];
+            if (!a.gon[gi]) continue;
+            seed = a.gseed[gi];
+            base = a.gv0[gi];
+            if (CAPPED && cap) {
+                cap += (int64_t)gi * a.ncon;
+                if (cap[0] == LLONG_MAX) cap = nullptr;           // cannot bind for this graph here
+            }
+        }
+        if (CAPPED && cap) suitor_chain<G, true>(a, (int32_t)v, gl, seed, base, cap);
+        else suitor_chain<G, false>(a, (int32_t)v, gl, seed, base, nullptr);
     }
 }
 
@@ -634,21 +693,25 @@ inline int grid_cap(int64_t items, int per_block, int cap) {
 }  // namespace
 
 void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, CoarsenBufs& b, hipStream_t s,
-                 const GraphSegs* gs) {
+                 const GraphSegs* gs, const long long* cap) {
     if (g.n <= 0) return;
-    const MatchArgs a{g.n, ctype_shem, seed, g.xadj, g.adjncy, g.adjwgt, b.match, b.suitor,
-                      gs ? gs->gseg : nullptr, gs ? gs->gv0 : nullptr, gs ? gs->gseed : nullptr,
-                      gs ? gs->gon : nullptr};
+    const MatchArgs a{g.n, ctype_shem, seed, g.xadj, g.adjncy, g.adjwgt, b.match, b.suitor, g.vwgt,
+                      g.vwgt ? cap : nullptr, g.ncon, gs ? gs->gseg : nullptr, gs ? gs->gv0 : nullptr,
+                      gs ? gs->gseed : nullptr, gs ? gs->gon : nullptr};
     const int grid = grid_cap(g.n, kBlock, 4096);
     hipLaunchKernelGGL(k_suitor_init, dim3(grid), dim3(kBlock), 0, s, g.n, b.suitor);
     const int G = pick_group(g.n, g.m) > 8 ? 8 : pick_group(g.n, g.m);   // short scans: chains dominate
-    const int sgrid = grid_cap(g.n, kBlock / G, device_cus() * 8);
-    switch (G) {
-        case 4: hipLaunchKernelGGL(k_suitor<4>, dim3(sgrid), dim3(kBlock), 0, s, a); break;
-        case 8: hipLaunchKernelGGL(k_suitor<8>, dim3(sgrid), dim3(kBlock), 0, s, a); break;
-        case 16: hipLaunchKernelGGL(k_suitor<16>, dim3(sgrid), dim3(kBlock), 0, s, a); break;
-        case 32: hipLaunchKernelGGL(k_suitor<32>, dim3(sgrid), dim3(kBlock), 0, s, a); break;
-        default: hipLaunchKernelGGL(k_suitor<64>, dim3(sgrid), dim3(kBlock), 0, s, a); break;
+    // persistent grid of exactly the resident blocks: the vertices are dealt
+    // to the groups in one pass, with no blocks waiting for a second round
+    const bool capped = a.cap != nullptr;   // the capped variant only where a pair can reach the cap
+    if (G == 4) {
+        const int sgrid = grid_cap(g.n, kBlock / 4, device_cus() * 8);
+        if (capped) hipLaunchKernelGGL((k_suitor<4, true>), dim3(sgrid), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_suitor<4, false>), dim3(sgrid), dim3(kBlock), 0, s, a);
+    } else {
+        const int sgrid = grid_cap(g.n, kBlock / 8, device_cus() * 8);
+        if (capped) hipLaunchKernelGGL((k_suitor<8, true>), dim3(sgrid), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_suitor<8, false>), dim3(sgrid), dim3(kBlock), 0, s, a);
     }
     hipLaunchKernelGGL(k_suitor_pairs, dim3(grid), dim3(kBlock), 0, s, g.n, b.suitor, b.match);
     RK_HIP(hipGetLastError());

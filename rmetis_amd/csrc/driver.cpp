@@ -150,7 +150,7 @@ Buffers layout_buffers(A& a, int32_t n, int32_t m, int32_t k, int32_t ncon, int6
     c.scan_tmp = r.scan_tmp;
     B.cut_acc = a.template get<unsigned long long>(1);
     B.cut_out = a.template get<int32_t>(1);
-    B.wsum = a.template get<long long>(std::max(ncon, 1));
+    B.wsum = a.template get<long long>(2 * std::max(ncon, 1));
     return B;
 }
 
@@ -222,17 +222,19 @@ size_t kway_first_level_bytes(int32_t n, int32_t m, int32_t k, int32_t ncon) {
     return c.total;
 }
 
-std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffers& B) {
+std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffers& B, std::vector<int32_t>* maxw) {
     std::vector<int32_t> tot(g.ncon, g.n);
+    if (maxw) maxw->assign(g.ncon, 1);
     if (!g.vwgt) return tot;
-    std::vector<long long> s(g.ncon);
-    column_sums(g, B.wsum, cx.stream);
-    RK_HIP(hipMemcpyAsync(s.data(), B.wsum, sizeof(long long) * g.ncon, hipMemcpyDeviceToHost, cx.stream));
+    std::vector<long long> s(2 * (size_t)g.ncon);
+    column_sums(g, B.wsum, cx.stream, B.wsum + g.ncon);
+    RK_HIP(hipMemcpyAsync(s.data(), B.wsum, sizeof(long long) * 2 * g.ncon, hipMemcpyDeviceToHost, cx.stream));
     RK_HIP(hipStreamSynchronize(cx.stream));
     for (int c = 0; c < g.ncon; ++c) {
         if (s[c] > INT32_MAX)
             throw Error(-4, "total vertex weight of a constraint exceeds the int32 range");
         tot[c] = (int32_t)s[c];
+        if (maxw) (*maxw)[c] = (int32_t)s[g.ncon + c];
     }
     return tot;
 }
@@ -358,7 +360,8 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
     const bool fm = !o.rtype_greedy;                                    // kway.rs:154-161: Fm refines by k-way FM
     FmBufs F;
     if (fm) F = alloc_fm_buffers(cx, g.n, g.m, k, ncon);
-    const std::vector<int32_t> total = total_weight_checked(cx, g, B);
+    std::vector<int32_t> maxw0;
+    const std::vector<int32_t> total = total_weight_checked(cx, g, B, &maxw0);
     int32_t* fin_heavy = cx.arena.get<int32_t>(g.n);
     const int32_t fin_nheavy = heavy_list_sync(g, fin_heavy, B.rb, s);
     std::vector<double> ub(ncon);
@@ -366,6 +369,22 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
         ub[c] = ubvec ? (double)ubvec[c] : 1.0 + (double)o.ufactor / 1000.0;
     const int32_t limit = std::max(20 * k, 20);                         // coarsen/mod.rs:43
     const size_t kc = (size_t)k * ncon;
+    // Matched pairs' weight cap (unit inputs too: their coarse levels are
+    // weighted).  Level l's vertices weigh at most 2^l times the heaviest
+    // input vertex, so a pair there can reach the cap only once 2^(l+1)
+    // times that weight exceeds it: the levels below pass none.
+    long long* d_cap = cx.arena.get<long long>(ncon);
+    int cap_from = 64;                                                  // first level where the cap can bind
+    {
+        long long* hcap = (long long*)cx.upload_stage(sizeof(long long) * ncon);
+        for (int32_t c = 0; c < ncon; ++c) {
+            hcap[c] = match_weight_cap(total[c], g.n, k);
+            int l = 0;
+            while (l < 62 && ((long long)maxw0[c] << (l + 1)) <= hcap[c]) ++l;
+            cap_from = std::min(cap_from, l);
+        }
+        RK_HIP(hipMemcpyAsync(d_cap, hcap, sizeof(long long) * ncon, hipMemcpyHostToDevice, s));
+    }
     std::vector<int32_t> ltot(ncon);
     bool have_best = false;
     int32_t best = 0;
@@ -399,7 +418,8 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
             }
             {
                 ProfScope ps(kProfMatch, match_bytes(cur->n, cur->m, cur->adjwgt != nullptr), cur->n, s);
-                match_graph(*cur, o.ctype_shem, rng.next_u64(), B.cb, s);
+                match_graph(*cur, o.ctype_shem, rng.next_u64(), B.cb, s, nullptr,
+                            (int)levels.size() >= cap_from ? d_cap : nullptr);
             }
             int32_t* cmap = cx.arena.get<int32_t>(cur->n);
             int32_t* heavy = cx.arena.get<int32_t>(cur->n);

@@ -84,7 +84,10 @@ size_t kway_first_level_bytes(int32_t n, int32_t m, int32_t k, int32_t ncon);
 
 // Column sums of vwgt (or n per constraint when vwgt is absent), checked
 // against the int32 range the reference's sums live in.
-std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffers& B);
+// maxw (may be null) receives the heaviest vertex weight per constraint (1
+// without vertex weights).
+std::vector<int32_t> total_weight_checked(Context& cx, const DevGraph& g, Buffers& B,
+                                          std::vector<int32_t>* maxw = nullptr);
 
 // One level's greedy refinement (greedy.rs:36-193) on the device partition.
 // cleared: B.rb.pw and B.rb.done were already zeroed (by the projection).

@@ -30,8 +30,10 @@ void fill_gseg(int32_t ngraphs, const int32_t* gv0, const int32_t* gn, int32_t* 
 // One graph's CSR (n vertices, m entries) into a union at vertex v0, entry e0.
 void pack_csr(int32_t n, int64_t m, const int32_t* xadj, const int32_t* adjncy, int32_t v0, int32_t e0,
               int32_t* uxadj, int32_t* uadjncy, hipStream_t s);
-// out[g*ncon + c] = int64 column sums of vwgt over graph g's vertices (g.vwgt required).
-void graph_column_sums(const DevGraph& g, const int32_t* gvtx, int32_t ngraphs, long long* out, hipStream_t s);
+// out[g*ncon + c] = int64 column sums of vwgt over graph g's vertices (g.vwgt
+// required); outmax (may be null) the column maxima.
+void graph_column_sums(const DevGraph& g, const int32_t* gvtx, int32_t ngraphs, long long* out, hipStream_t s,
+                       long long* outmax = nullptr);
 // out[4g..4g+3] = (first vertex, end vertex, first entry, end entry) of each
 // graph g < ngraphs_total in the level (zeros for graphs not present).
 void graph_ranges(int32_t n, const int32_t* gseg, const int32_t* xadj, int32_t ngraphs_total, int32_t* out,
@@ -50,8 +52,9 @@ void part_weights(const DevGraph& g, const int32_t* part, int32_t k, int32_t* pw
 // part_f[v] = part_c[cmap[v]]; also clears clear_a[0..na) and clear_b[0..nb)
 void project(int32_t n_fine, const int32_t* cmap, const int32_t* part_c, int32_t* part_f, hipStream_t s,
              int32_t* clear_a = nullptr, int64_t na = 0, int32_t* clear_b = nullptr, int32_t nb = 0);
-// out[c] = sum over v of vwgt[v*ncon+c] as int64 (g.vwgt must be present).
-void column_sums(const DevGraph& g, long long* out, hipStream_t s);
+// out[c] = sum over v of vwgt[v*ncon+c] as int64 (g.vwgt must be present);
+// outmax (may be null) the column maxima.
+void column_sums(const DevGraph& g, long long* out, hipStream_t s, long long* outmax = nullptr);
 
 // ---- refine.hip ----
 constexpr int kHeavyGlobalBlocks = 256;
@@ -235,8 +238,20 @@ struct GraphSegs {
     int32_t* cgseg = nullptr;          // [n] out: graph of each coarse vertex
 };
 
+// cap (device, may be null; used only when g.vwgt is present): per
+// constraint the largest weight a matched pair may have (per graph and
+// constraint on batched levels).
 void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, CoarsenBufs& b, hipStream_t s,
-                 const GraphSegs* gs = nullptr);
+                 const GraphSegs* gs = nullptr, const long long* cap = nullptr);
+// The cap of a graph with n vertices and total weight `total` of one
+// constraint partitioned into k parts: 1.5 x total / max(20k, 20) (the
+// coarsening limit), and at least twice the mean vertex weight.
+inline long long match_weight_cap(long long total, long long n, int32_t k) {
+    const long long limit = k * 20LL > 20 ? k * 20LL : 20;
+    const long long share = 3 * total / (2 * limit);
+    const long long mean2 = 2 * ((total + n - 1) / (n > 0 ? n : 1));
+    return share > mean2 ? share : mean2;
+}
 struct CoarseAlloc {
     virtual int32_t* alloc_i32(int64_t count) = 0;
     virtual ~CoarseAlloc() = default;

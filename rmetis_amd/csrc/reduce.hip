@@ -321,19 +321,34 @@ __global__ __launch_bounds__(kBlock) void k_fill_tiles(const int4* __restrict__ 
 // One block per graph: out[g*ncon + c] = sum of vwgt[v*ncon + c] over its
 // vertices [gvtx[g], gvtx[g+1]), as int64.
 __global__ __launch_bounds__(kBlock) void k_graph_colsum(const int32_t* __restrict__ gvtx, int32_t ncon,
-                                                         const int32_t* __restrict__ vwgt, long long* __restrict__ out) {
+                                                         const int32_t* __restrict__ vwgt, long long* __restrict__ out,
+                                                         long long* __restrict__ outmax) {
     __shared__ long long s_part[kBlock / 64];
+    __shared__ int32_t s_max[kBlock / 64];
     const int64_t v0 = gvtx[blockIdx.x], v1 = gvtx[blockIdx.x + 1];
     for (int c = 0; c < ncon; ++c) {
         long long sum = 0;
-        for (int64_t v = v0 + threadIdx.x; v < v1; v += kBlock) sum += vwgt[v * ncon + c];
+        int32_t mx = 0;
+        for (int64_t v = v0 + threadIdx.x; v < v1; v += kBlock) {
+            const int32_t w = vwgt[v * ncon + c];
+            sum += w;
+            mx = max(mx, w);
+        }
         sum = wave_incl_scan(sum);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
         if (__lane_id() == 63) s_part[threadIdx.x >> 6] = sum;
+        if (__lane_id() == 0) s_max[threadIdx.x >> 6] = mx;
         __syncthreads();
         if (threadIdx.x == 0) {
             long long t = 0;
-            for (int w = 0; w < kBlock / 64; ++w) t += s_part[w];
+            int32_t m = 0;
+            for (int w = 0; w < kBlock / 64; ++w) {
+                t += s_part[w];
+                m = max(m, s_max[w]);
+            }
             out[(int64_t)blockIdx.x * ncon + c] = t;
+            if (outmax) outmax[(int64_t)blockIdx.x * ncon + c] = m;
         }
         __syncthreads();
     }
@@ -358,18 +373,32 @@ __global__ __launch_bounds__(kBlock) void k_fill_gseg(int32_t ng, const int32_t*
 
 __global__ __launch_bounds__(kBlock) void k_column_sum(int32_t n, int32_t ncon, int32_t c,
                                                        const int32_t* __restrict__ vwgt,
-                                                       unsigned long long* __restrict__ out) {
+                                                       unsigned long long* __restrict__ out,
+                                                       unsigned long long* __restrict__ outmax) {
     __shared__ long long s_w[kBlock / 64];
+    __shared__ int32_t s_m[kBlock / 64];
     long long sum = 0;
-    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock)
-        sum += vwgt[v * ncon + c];
+    int32_t mx = 0;
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
+        const int32_t w = vwgt[v * ncon + c];
+        sum += w;
+        mx = max(mx, w);
+    }
     long long inc = wave_incl_scan(sum);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
     if (__lane_id() == 63) s_w[threadIdx.x >> 6] = inc;
+    if (__lane_id() == 0) s_m[threadIdx.x >> 6] = mx;
     __syncthreads();
     if (threadIdx.x == 0) {
         long long t = 0;
-        for (int w = 0; w < kBlock / 64; ++w) t += s_w[w];
+        int32_t m = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            t += s_w[w];
+            m = max(m, s_m[w]);
+        }
         atomicAdd(out, (unsigned long long)t);
+        if (outmax) atomicMax(outmax, (unsigned long long)m);   // vertex weights are positive
     }
 }
 
@@ -421,11 +450,13 @@ void project(int32_t n_fine, const int32_t* cmap, const int32_t* part_c, int32_t
     RK_HIP(hipGetLastError());
 }
 
-void column_sums(const DevGraph& g, long long* out, hipStream_t s) {
+void column_sums(const DevGraph& g, long long* out, hipStream_t s, long long* outmax) {
     RK_HIP(hipMemsetAsync(out, 0, sizeof(long long) * g.ncon, s));
+    if (outmax) RK_HIP(hipMemsetAsync(outmax, 0, sizeof(long long) * g.ncon, s));
     for (int c = 0; c < g.ncon; ++c)
         hipLaunchKernelGGL(k_column_sum, dim3(grid_for(g.n, kBlock, 2048)), dim3(kBlock), 0, s, g.n, g.ncon, c,
-                           g.vwgt, (unsigned long long*)(out + c));
+                           g.vwgt, (unsigned long long*)(out + c),
+                           outmax ? (unsigned long long*)(outmax + c) : nullptr);
     RK_HIP(hipGetLastError());
 }
 
@@ -487,9 +518,10 @@ void fill_tiles(const int4* desc, int32_t ndesc, int32_t* tile_v0, int32_t* tile
     RK_HIP(hipGetLastError());
 }
 
-void graph_column_sums(const DevGraph& g, const int32_t* gvtx, int32_t ngraphs, long long* out, hipStream_t s) {
+void graph_column_sums(const DevGraph& g, const int32_t* gvtx, int32_t ngraphs, long long* out, hipStream_t s,
+                       long long* outmax) {
     if (ngraphs > 0)
-        hipLaunchKernelGGL(k_graph_colsum, dim3(ngraphs), dim3(kBlock), 0, s, gvtx, g.ncon, g.vwgt, out);
+        hipLaunchKernelGGL(k_graph_colsum, dim3(ngraphs), dim3(kBlock), 0, s, gvtx, g.ncon, g.vwgt, out, outmax);
     RK_HIP(hipGetLastError());
 }
 

@@ -437,12 +437,18 @@ def _mix32(x):
     return x
 
 
+def _degree_rank(d):
+    d = max(d, 1)
+    return min(d, 4) + max(0, d.bit_length() - 1 - 2)
+
+
 def _greedy_matching(x, a, aw, seed, shem):
     """The matching the GPU computes, by its definition: the sequential greedy
     matching over the edges in descending order of (key, then the smaller and
-    larger endpoint ascending), key = (2^32-1 minus floor(log2) of the smaller
-    endpoint degree, edge weight, seeded lowbias32 hash) for heavy-edge
-    matching and the hash alone for random matching."""
+    larger endpoint ascending), key = (2^32-1 minus the rank of the smaller
+    endpoint degree d -- min(d, 4) + max(0, floor(log2 d) - 2) --, edge weight,
+    seeded lowbias32 hash) for heavy-edge matching and the hash alone for
+    random matching."""
     n = len(x) - 1
     deg = np.diff(x)
     edges = []
@@ -453,7 +459,7 @@ def _greedy_matching(x, a, aw, seed, shem):
                 w = int(aw[j]) if aw is not None else 1
                 h = _mix32(v ^ _mix32(u ^ (seed & _M32)) ^ ((seed >> 32) & _M32))
                 if shem:
-                    hi = ((_M32 - (min(int(deg[v]), int(deg[u])).bit_length() - 1)) << 32) | ((w & _M32) ^ 0x80000000)
+                    hi = ((_M32 - _degree_rank(min(int(deg[v]), int(deg[u])))) << 32) | ((w & _M32) ^ 0x80000000)
                 else:
                     hi = 0
                 edges.append((-hi, -h, v, u))
@@ -482,13 +488,18 @@ def test_gpu_matching_is_the_greedy_matching(shem):
     assert np.array_equal(R.gpu_matching(g2, 7, ctype), _greedy_matching(x2, a2, aw2, 7, shem))
 
 
-def _greedy_matching_np(x, a, aw, seed, shem):
+def _greedy_matching_np(x, a, aw, seed, shem, vwgt=None, cap=None):
     """_greedy_matching with the keys and the edge order computed by numpy
-    (for levels large enough to take the handshake rounds)."""
+    (for large levels); with cap, only the edges whose endpoints' summed
+    weights fit cap[c] for every constraint c take part (the keys still use
+    the full graph's degrees)."""
     n = len(x) - 1
     deg = np.diff(x).astype(np.int64)
     src = np.repeat(np.arange(n, dtype=np.int64), deg)
     keep = src < a
+    if cap is not None:
+        vw = np.asarray(vwgt, np.int64).reshape(n, len(cap))
+        keep &= ((vw[src] + vw[a.astype(np.int64)]) <= np.asarray(cap, np.int64)).all(axis=1)
     v, u = src[keep], a[keep].astype(np.int64)
     w = (aw[keep] if aw is not None else np.ones(len(v), np.int32)).astype(np.int64)
 
@@ -504,11 +515,12 @@ def _greedy_matching_np(x, a, aw, seed, shem):
     lo, hi_s = np.uint64(seed & _M32), np.uint64((seed >> 32) & _M32)
     h = mix(v.astype(np.uint64) ^ mix(u.astype(np.uint64) ^ lo) ^ hi_s)
     if shem:
-        md = np.minimum(deg[v], deg[u])
-        lg = np.zeros(len(md), np.uint64)                   # floor(log2(md)), md >= 1 on every edge
+        md = np.maximum(np.minimum(deg[v], deg[u]), 1)
+        lg = np.zeros(len(md), np.int64)                    # floor(log2(md))
         for b in range(1, 32):
             lg[md >= (1 << b)] = b
-        hi = ((np.uint64(_M32) - lg) << np.uint64(32)) | ((w.astype(np.uint64) & np.uint64(_M32)) ^ np.uint64(0x80000000))
+        rank = (np.minimum(md, 4) + np.maximum(lg - 2, 0)).astype(np.uint64)
+        hi = ((np.uint64(_M32) - rank) << np.uint64(32)) | ((w.astype(np.uint64) & np.uint64(_M32)) ^ np.uint64(0x80000000))
     else:
         hi = np.zeros(len(v), np.uint64)
     order = np.lexsort((u, v, ~h, ~hi))
@@ -523,9 +535,9 @@ def _greedy_matching_np(x, a, aw, seed, shem):
 
 
 @pytest.mark.parametrize("shem", [1, 0])
-def test_gpu_matching_large_levels_handshake_then_suitor(shem):
-    # levels of 2^16 vertices and more run handshake rounds before the suitor
-    # chains; the result is still the greedy matching in edge-key order
+def test_gpu_matching_large_levels(shem):
+    # levels of 2^16 vertices and more (wide grids of suitor chains); the
+    # result is still the greedy matching in edge-key order
     from rmetis_amd import synth
     ctype = R.CType.Shem if shem else R.CType.Rm
     cases = [synth.rgg(70000, 12), synth.grid(300, 240)]
@@ -537,6 +549,29 @@ def test_gpu_matching_large_levels_handshake_then_suitor(shem):
                                                                                     seed, shem))
     g = _graph(xr, ar, adjwgt=awr)
     assert np.array_equal(R.gpu_matching(g, 9, ctype), _greedy_matching_np(xr, ar, awr, 9, shem))
+
+
+@pytest.mark.parametrize("shem", [1, 0])
+def test_gpu_matching_weight_cap(shem):
+    # the capped matching is the greedy matching over the pairs that fit;
+    # caps that every pair fits give the uncapped matching
+    from rmetis_amd import synth
+    ctype = R.CType.Shem if shem else R.CType.Rm
+    cases = [(synth.rgg(3000, 21, vwgt_max=4), 1, [5]), (synth.rgg(70000, 22, vwgt_max=4), 1, [6]),
+             (synth.rgg(5000, 23, vwgt_max=8, ncon=2), 2, [9, 12]), (synth.rgg(4000, 24, vwgt_max=3), 1, [2])]
+    for s, ncon, cap in cases:
+        g = R.Graph(s.n, ncon, s.xadj, s.adjncy, s.vwgt, s.adjwgt)
+        for seed in (4, 0x0F1E2D3C4B5A6978):
+            got = R.gpu_matching(g, seed, ctype, cap=cap)
+            assert np.array_equal(got, _greedy_matching_np(s.xadj, s.adjncy, s.adjwgt, seed, shem, s.vwgt, cap))
+            vw = s.vwgt.reshape(s.n, ncon).astype(np.int64)
+            pair = got != np.arange(s.n)
+            assert ((vw[pair] + vw[got[pair]]) <= np.array(cap)).all()
+        free = [int(2 * s.vwgt.reshape(s.n, ncon)[:, c].max()) for c in range(ncon)]
+        assert np.array_equal(R.gpu_matching(g, 4, ctype, cap=free), R.gpu_matching(g, 4, ctype))
+    with pytest.raises(R.InputError):
+        s = cases[0][0]
+        R.gpu_matching(R.Graph(s.n, 1, s.xadj, s.adjncy, s.vwgt, s.adjwgt), 1, ctype, cap=[-1])
 
 
 @pytest.mark.parametrize("shem", [1, 0])

@@ -84,7 +84,7 @@ def test_server_errors():
             srv.submit(g, 17, options=_opts(1))                   # nparts > nvtxs
         with pytest.raises(R.InputError):
             srv.submit(g, 1, options=_opts(1))                    # nparts < 2
-        with pytest.raises(R.InternalError):
+        with pytest.raises(R.InputError):
             srv.submit(g, 2, options=R.Options.for_kway(seed=1))  # Fm refinement
         rid = srv.submit(g, 2, options=_opts(1))
         r = srv.result(rid)
