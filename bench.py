@@ -74,11 +74,13 @@ def _prof(L, tag, size=-1):
 def profiled_step(L, step, size):
     """One untimed step with every kernel family bracketed: per tag, the
     total ms over the step and (launches, ms, bytes) at the level of `size`
-    vertices."""
+    vertices (a callable: read after the step)."""
     L.rmetis_hip_profile_filter(-1, -1)
     L.rmetis_hip_profile_enable(1)
     step(True)
     L.rmetis_hip_device_synchronize()
+    if callable(size):
+        size = size()
     total = {name: _prof(L, t)[1] for name, t in TAGS.items()}
     finest = {name: _prof(L, t, size) for name, t in TAGS.items()}
     L.rmetis_hip_profile_enable(0)
@@ -385,10 +387,14 @@ def bench_batch(args):
         t1 = time.perf_counter()
         step()
         log(f"warmup {i}: {time.perf_counter() - t1:.2f}s")
-    total, finest = profiled_step(L, step, n)
+    # the finest level of a pass is the union of the attempts it runs together
+    total, finest = profiled_step(L, step, lambda: stats.level_n[0])
+    packed = int(stats.attempts_per_call)
+    fin_n = int(stats.level_n[0])
     tag = args.roofline_tag or dominant(total)
-    log(f"kernel ms per step {dict((k_, round(v, 1)) for k_, v in total.items())}; roofline kernel: {tag}")
-    elapsed, ln, ms, by = timed_steps(L, step, args.steps, tag, n)
+    log(f"kernel ms per step {dict((k_, round(v, 1)) for k_, v in total.items())}; roofline kernel: {tag}; "
+        f"attempts per pass {packed} (finest union level n={fin_n})")
+    elapsed, ln, ms, by = timed_steps(L, step, args.steps, tag, fin_n)
     value = count * args.steps / elapsed
     log(f"batched: {args.steps} steps in {elapsed:.2f}s = {value:.2f} graphs/s")
     part = d_part.numpy()
@@ -455,7 +461,8 @@ def bench_batch(args):
                                f"vwgt U{{1..4}}, adjwgt U{{1..10}}, k in {{2..64}}), {ncuts} seeds each, one "
                                f"segmented CSR: n={n} m={m}",
                    "ctype": "shem", "rtype": "greedy", "ufactor": 30, "niter": 10, "ncuts": ncuts,
-                   "seed": args.seed, "parallelism": "single GPU, one segmented CSR"},
+                   "attempts_per_pass": packed, "seed": args.seed,
+                   "parallelism": "single GPU, one segmented CSR (attempts packed as copies of the batch)"},
         "plain": {"value": round(plain_value, 3), "unit": "graphs/s", "graphs": len(sample),
                   "objval_mismatches": mism,
                   "note": "rmetis_hip_kway_device per graph (device-resident CSR, the 8 attempts one after "

@@ -65,7 +65,9 @@ int32_t RMETIS_PartGraphKway(const idx_t* nvtxs, const idx_t* ncon, const idx_t*
                              const real_t* tpwgts, const real_t* ubvec, const idx_t* options, idx_t* objval,
                              idx_t* part);
 
-/* Per-level figures of the last attempt of a k-way call (finest first). */
+/* Per-level figures of the last attempt of a k-way call (finest first; for
+ * a batched call, of its last pass over the union of the attempts it ran
+ * together). */
 #define RMETIS_HIP_MAX_LEVELS 64
 typedef struct {
     int32_t nlevels;
@@ -76,6 +78,7 @@ typedef struct {
     float coarsen_ms[RMETIS_HIP_MAX_LEVELS];
     float refine_ms[RMETIS_HIP_MAX_LEVELS];
     int32_t passes[RMETIS_HIP_MAX_LEVELS];
+    int32_t attempts_per_call;   /* batched calls: attempts run together in one pass (levels: the last pass's) */
 } rmetis_hip_stats;
 
 /* Same computation as RMETIS_PartGraphKway (src/partition/kway.rs:20-141) on
@@ -123,8 +126,11 @@ int32_t rmetis_hip_kway_device_noalloc(idx_t nvtxs, idx_t ncon, const idx_t* d_x
  * entropy) and graph i keeps its strictly best cut, so every graph's part
  * and objval equal those of a separate call with that seed.  ubvec[ncon]
  * (NULL = 1 + ufactor/1000) applies to every graph; target weights are
- * uniform.  The CSR is not validated.  stats (may be NULL) describes the
- * union levels of the last attempt. */
+ * uniform.  The CSR is not validated.  The attempts run several at a time:
+ * one pass over a union holding one copy of the batch per attempt (as many
+ * as the int32 offsets and the free device memory allow, balanced over
+ * the fewest passes; stats->attempts_per_call).  stats (may be NULL)
+ * describes the union levels of the last pass. */
 int32_t rmetis_hip_kway_batch_device(idx_t ngraphs, const idx_t* gvtx, idx_t ncon, const idx_t* d_xadj,
                                      const idx_t* d_adjncy, const idx_t* d_vwgt, const idx_t* d_adjwgt,
                                      const idx_t* nparts, const real_t* ubvec, const idx_t* options,

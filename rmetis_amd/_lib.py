@@ -23,7 +23,7 @@ class HipStats(C.Structure):
         ("nlevels", C.c_int32), ("coarsest_n", C.c_int32), ("initial_ms", C.c_float),
         ("level_n", C.c_int32 * MAX_LEVELS), ("level_m", C.c_int32 * MAX_LEVELS),
         ("coarsen_ms", C.c_float * MAX_LEVELS), ("refine_ms", C.c_float * MAX_LEVELS),
-        ("passes", C.c_int32 * MAX_LEVELS),
+        ("passes", C.c_int32 * MAX_LEVELS), ("attempts_per_call", C.c_int32),
     ]
 
 

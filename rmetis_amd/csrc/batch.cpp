@@ -6,8 +6,10 @@
 // rule (coarsen/mod.rs:49-69), the same host growing of its coarsest graph
 // (initial/grow.rs:12-103), the same greedy passes (greedy.rs:36-193) and
 // the same best-of-ncuts choice (kway.rs:96-98).  What changes is the
-// schedule: one attempt index of all graphs runs level by level in lockstep,
-// so each kernel launch covers every graph still present at that level.
+// schedule: the attempts of all graphs run level by level in lockstep (as
+// many attempts per pass as the int32 offsets and device memory allow, each
+// on its own copy of the batch), so each kernel launch covers every
+// (graph, attempt) still present at that level.
 //
 // Level l holds the graphs that reached it laid out one after another.  A
 // graph whose coarsening stops at level l is present at l but not matched
@@ -78,17 +80,12 @@ std::vector<Extent> read_extents(Context& cx, const DevGraph& g, const int32_t* 
 
 inline int64_t padded_tiles(int32_t n) { return ((((int64_t)n + kTile - 1) / kTile) + 31) & ~(int64_t)31; }
 
-}  // namespace
-
-void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvtx, const int32_t* nparts,
-                       const float* ubvec, const Options& o, const uint64_t* seeds, int32_t* d_part, int32_t* objval,
-                       KwayStats* st) {
-    if (G <= 0) throw Error(-2, "a batch needs at least one graph");
-    if (!o.rtype_greedy)
-        throw Error(-2, "the batched and served forms refine greedily: set options[METIS_OPTION_RTYPE]=1 "
-                        "(Greedy) and options[METIS_OPTION_PTYPE]=0 so that the array is read; k-way FM runs "
-                        "through RMETIS_PartGraphKway and rmetis_hip_kway_device");
-    if (gvtx[0] != 0 || gvtx[G] != g.n) throw Error(-2, "graph offsets must start at 0 and end at nvtxs");
+// One attempt of every graph of the union g: graph i runs with the seed
+// seeds[i] as given (the caller derives it per attempt).  Writes d_part[g.n]
+// and objval[G] (host).
+void batch_attempt(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvtx, const int32_t* nparts,
+                   const float* ubvec, const Options& o, const uint64_t* seeds, int32_t* d_part, int32_t* objval,
+                   KwayStats* st) {
     const int32_t ncon = g.ncon;
     std::vector<int32_t> gk(G);
     std::vector<int32_t> gkc(G);
@@ -205,21 +202,12 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
         for (int32_t i = 0; i < G; ++i) v0[i] = gvtx[i];
         d_gv00 = put(cx, v0);
     }
-    std::vector<int32_t> best(G, 0);
-    std::vector<uint8_t> have(G, 0);
     if (st) *st = KwayStats{};
     const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::random_device rd;
-    for (int attempt = 0; attempt < o.ncuts; ++attempt) {
-        const Arena::Mark mark = cx.arena.mark();
+    {
         std::vector<Rng> rng;
         rng.reserve(G);
-        for (int32_t i = 0; i < G; ++i) {
-            uint64_t sd = seeds ? seeds[i] : o.seed;
-            if (sd == 0) sd = ((uint64_t)rd() << 32) ^ rd();
-            else sd += (uint64_t)attempt;                                 // kway.rs:55-61
-            rng.emplace_back(sd);
-        }
+        for (int32_t i = 0; i < G; ++i) rng.emplace_back(seeds[i]);
         std::vector<BLevel> lv(1);
         lv[0].g = g;
         lv[0].gseg = gseg0;
@@ -464,21 +452,14 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
             }
             pc = pf;
         }
-        // ---- per-graph cut (kway.rs:94) and the strictly better attempts (kway.rs:96-98)
+        // ---- per-graph cut (kway.rs:94)
         cx.set_phase(kPhaseFinish);
         edge_cut_batch(g, pc, gseg0, G, d_cutacc, d_cut, s);
         int32_t* hc = (int32_t*)cx.host_stage2(sizeof(int32_t) * (size_t)G);
         RK_HIP(hipMemcpyAsync(hc, d_cut, sizeof(int32_t) * (size_t)G, hipMemcpyDeviceToHost, s));
+        RK_HIP(hipMemcpyAsync(d_part, pc, sizeof(int32_t) * (size_t)g.n, hipMemcpyDeviceToDevice, s));
         RK_HIP(hipStreamSynchronize(s));
-        std::vector<uint8_t> improved(G, 0);
-        for (int32_t i = 0; i < G; ++i)
-            if (!have[i] || hc[i] < best[i]) {
-                have[i] = 1;
-                best[i] = hc[i];
-                improved[i] = 1;
-            }
-        copy_improved(g.n, gseg0, put(cx, improved), pc, d_part, s);
-        RK_HIP(hipStreamSynchronize(s));
+        for (int32_t i = 0; i < G; ++i) objval[i] = hc[i];
         if (st) {
             st->levels.clear();
             for (size_t l = 0; l < nl; ++l) {
@@ -490,12 +471,147 @@ void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t*
             }
             st->coarsest_n = (int32_t)part_tot;
             st->initial_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
-            st->attempts = attempt + 1;
+            st->attempts = 1;
+        }
+    }
+}
+
+}  // namespace
+
+int32_t batch_attempts_per_call(int64_t n, int64_t m, int64_t tiles, int64_t kc_total, int32_t ngraphs, int ncuts,
+                                int max_pack, size_t reusable) {
+    int64_t p = ncuts;
+    if (max_pack > 0) p = std::min<int64_t>(p, max_pack);
+    // int32 vertex ids, CSR offsets, tile records and threshold offsets
+    p = std::min<int64_t>(p, (INT32_MAX - 1) / std::max<int64_t>(n, 1));
+    p = std::min<int64_t>(p, INT32_MAX / std::max<int64_t>(m, 1));
+    p = std::min<int64_t>(p, (INT32_MAX / kTile) / std::max<int64_t>(tiles, 1));
+    p = std::min<int64_t>(p, INT32_MAX / std::max<int64_t>(kc_total, 1));
+    p = std::min<int64_t>(p, INT32_MAX / std::max<int64_t>(ngraphs, 1));
+    // device memory per copy: the work buffers (36 B per entry: candidate
+    // pool, contraction staging and hash tables; about 120 B per vertex) and
+    // the hierarchy (the copy itself, the coarse levels, per-level maps and
+    // parts), about 56 B per entry and 200 B per vertex in all, within 60 %
+    // of the device's free memory plus what the arena already holds
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) freeb = 0;
+    const int64_t per_copy = 200 * n + 56 * m + (1 << 20);
+    const int64_t room = (int64_t)((double)(freeb + reusable) * 0.6);
+    p = std::min<int64_t>(p, std::max<int64_t>(room / per_copy, 1));
+    p = std::max<int64_t>(p, 1);
+    // balanced groups: the fewest calls, then the same attempts in each
+    const int64_t groups = (ncuts + p - 1) / p;
+    return (int32_t)((ncuts + groups - 1) / groups);
+}
+
+void kway_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvtx, const int32_t* nparts,
+                       const float* ubvec, const Options& o, const uint64_t* seeds, int32_t* d_part, int32_t* objval,
+                       KwayStats* st, int max_pack) {
+    if (G <= 0) throw Error(-2, "a batch needs at least one graph");
+    if (!o.rtype_greedy)
+        throw Error(-2, "the batched and served forms refine greedily: set options[METIS_OPTION_RTYPE]=1 "
+                        "(Greedy) and options[METIS_OPTION_PTYPE]=0 so that the array is read; k-way FM runs "
+                        "through RMETIS_PartGraphKway and rmetis_hip_kway_device");
+    if (gvtx[0] != 0 || gvtx[G] != g.n) throw Error(-2, "graph offsets must start at 0 and end at nvtxs");
+    int64_t tiles0 = 0, kc_total = 0;
+    for (int32_t i = 0; i < G; ++i) {
+        const int32_t n = gvtx[i + 1] - gvtx[i];
+        if (n <= 0) throw Error(-2, "graph " + std::to_string(i) + " has no vertices");
+        if (nparts[i] < 2) throw Error(-2, "graph " + std::to_string(i) + ": nparts must be >= 2");   // kway.rs:31-33
+        if (nparts[i] > n) throw Error(-2, "graph " + std::to_string(i) + ": nparts > nvtxs");        // kway.rs:34-38
+        tiles0 += padded_tiles(n);
+        kc_total += (int64_t)nparts[i] * g.ncon;
+    }
+    // Attempt a of graph i runs with seed + a (kway.rs:55-61).  The attempts
+    // are independent, so P of them run in one pass over a union holding P
+    // copies of the batch (copy j is attempt a0 + j of every graph): every
+    // level kernel then covers P times the graphs.  Each graph keeps its
+    // strictly best attempt, the earliest among equal cuts (kway.rs:96-98).
+    const int32_t P =
+        batch_attempts_per_call(g.n, g.m, tiles0, kc_total, G, o.ncuts, max_pack, cx.arena.owned_bytes());
+    std::random_device rd;
+    std::vector<int32_t> best(G, 0);
+    std::vector<uint8_t> have(G, 0);
+    hipStream_t s = cx.stream;
+    int32_t* gseg0 = nullptr;
+    if (o.ncuts > 1) {
+        gseg0 = cx.arena.get<int32_t>(g.n);
+        std::vector<int32_t> gn(G), gv(G);
+        for (int32_t i = 0; i < G; ++i) {
+            gn[i] = gvtx[i + 1] - gvtx[i];
+            gv[i] = gvtx[i];
+        }
+        fill_gseg(G, put(cx, gv), put(cx, gn), gseg0, s);
+    }
+    float initial_ms = 0.f;
+    for (int32_t a0 = 0; a0 < o.ncuts; a0 += P) {
+        const int32_t p = std::min<int32_t>(P, o.ncuts - a0);
+        const Arena::Mark mark = cx.arena.mark();
+        const int32_t VG = p * G;
+        std::vector<int32_t> ugvtx((size_t)VG + 1), unparts(VG);
+        std::vector<uint64_t> useeds(VG);
+        for (int32_t j = 0; j < p; ++j)
+            for (int32_t i = 0; i < G; ++i) {
+                const int32_t vg = j * G + i;
+                ugvtx[vg] = j * g.n + gvtx[i];
+                unparts[vg] = nparts[i];
+                uint64_t sd = seeds ? seeds[i] : o.seed;
+                if (sd == 0) sd = ((uint64_t)rd() << 32) ^ rd();
+                else sd += (uint64_t)(a0 + j);                                  // kway.rs:55-61
+                useeds[vg] = sd;
+            }
+        ugvtx[VG] = p * g.n;
+        DevGraph u = g;
+        if (p > 1) {
+            const int64_t un = (int64_t)p * g.n, um = (int64_t)p * g.m;
+            u.n = (int32_t)un;
+            u.m = (int32_t)um;
+            u.xadj = cx.arena.get<int32_t>(un + 1);
+            u.adjncy = cx.arena.get<int32_t>(um);
+            u.adjwgt = g.adjwgt ? cx.arena.get<int32_t>(um) : nullptr;
+            u.vwgt = g.vwgt ? cx.arena.get<int32_t>(un * g.ncon) : nullptr;
+            u.vsize = nullptr;
+            for (int32_t j = 0; j < p; ++j) {
+                pack_csr(g.n, g.m, g.xadj, g.adjncy, j * g.n, j * g.m, u.xadj, u.adjncy, s);
+                if (g.adjwgt)
+                    RK_HIP(hipMemcpyAsync(u.adjwgt + (int64_t)j * g.m, g.adjwgt, sizeof(int32_t) * (size_t)g.m,
+                                          hipMemcpyDeviceToDevice, s));
+                if (g.vwgt)
+                    RK_HIP(hipMemcpyAsync(u.vwgt + (int64_t)j * g.n * g.ncon, g.vwgt,
+                                          sizeof(int32_t) * (size_t)g.n * g.ncon, hipMemcpyDeviceToDevice, s));
+            }
+            const int32_t* hend = put(cx, std::vector<int32_t>{(int32_t)um});
+            RK_HIP(hipMemcpyAsync(u.xadj + un, hend, sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+        }
+        int32_t* upart = (o.ncuts == 1) ? d_part : cx.arena.get<int32_t>(u.n);
+        std::vector<int32_t> uobj(VG);
+        batch_attempt(cx, u, VG, ugvtx.data(), unparts.data(), ubvec, o, useeds.data(), upart, uobj.data(), st);
+        if (st) initial_ms += st->initial_ms;
+        if (o.ncuts > 1) {
+            std::vector<int32_t> choice(G, -1);
+            for (int32_t j = 0; j < p; ++j)
+                for (int32_t i = 0; i < G; ++i) {
+                    const int32_t c = uobj[(size_t)j * G + i];
+                    if (!have[i] || c < best[i]) {
+                        have[i] = 1;
+                        best[i] = c;
+                        choice[i] = j;
+                    }
+                }
+            copy_attempt(g.n, gseg0, put(cx, choice), upart, d_part, s);
+            RK_HIP(hipStreamSynchronize(s));
+        } else {
+            for (int32_t i = 0; i < G; ++i) best[i] = uobj[i];
         }
         cx.arena.rewind(mark);
         cx.upload_reset();
     }
     for (int32_t i = 0; i < G; ++i) objval[i] = best[i];
+    if (st) {
+        st->attempts = o.ncuts;
+        st->attempts_per_call = P;
+        st->initial_ms = initial_ms;
+    }
 }
 
 }  // namespace rk

@@ -152,6 +152,7 @@ void fill_stats(const KwayStats& ks, rmetis_hip_stats* out) {
     out->nlevels = (int32_t)std::min<size_t>(ks.levels.size(), RMETIS_HIP_MAX_LEVELS);
     out->coarsest_n = ks.coarsest_n;
     out->initial_ms = ks.initial_ms;
+    out->attempts_per_call = ks.attempts_per_call;
     for (int i = 0; i < out->nlevels; ++i) {
         out->level_n[i] = ks.levels[i].n;
         out->level_m[i] = ks.levels[i].m;

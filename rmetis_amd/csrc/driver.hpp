@@ -56,6 +56,7 @@ struct KwayStats {
     int32_t coarsest_n = 0;
     float initial_ms = 0.f;
     int32_t attempts = 0;
+    int32_t attempts_per_call = 1;   // batched form: attempts run together in one pass
 };
 
 struct Buffers {
@@ -110,9 +111,19 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
 // seeds[i] + a (o.seed + a when seeds is null; 0 draws entropy), exactly as
 // a separate kway_device call with those options would, and the graph keeps
 // its strictly best cut.  Writes d_part[g.n] and objval[ngraphs] (host).
+// The attempts run batch_attempts_per_call at a time over a union holding
+// that many copies of g; max_pack > 0 caps the copies (a caller with a
+// memory budget of its own passes 1).
 void kway_batch_device(Context& cx, const DevGraph& g, int32_t ngraphs, const int32_t* gvtx, const int32_t* nparts,
                        const float* ubvec, const Options& o, const uint64_t* seeds, int32_t* d_part, int32_t* objval,
-                       KwayStats* st);
+                       KwayStats* st, int max_pack = 0);
+// Attempts per pass of kway_batch_device for a batch of n vertices, m
+// entries, the given padded tile count and k*ncon total: bounded by the
+// int32 ranges of the union, 60 % of the free device memory (plus the
+// reusable bytes the caller's arena holds) and max_pack, then balanced over
+// the fewest passes.
+int32_t batch_attempts_per_call(int64_t n, int64_t m, int64_t tiles, int64_t kc_total, int32_t ngraphs, int ncuts,
+                                int max_pack, size_t reusable);
 
 // Test toggles (no counterpart in the reference): bit 0 recomputes the
 // part-weight table after every projection instead of carrying the coarser

@@ -27,6 +27,10 @@ void gather_csr(const DevGraph& g, const int32_t* desc, int32_t ndesc, int32_t* 
 void copy_improved(int32_t n, const int32_t* gseg, const uint8_t* improved, const int32_t* part, int32_t* best,
                    hipStream_t s);
 void fill_gseg(int32_t ngraphs, const int32_t* gv0, const int32_t* gn, int32_t* gseg, hipStream_t s);
+// best[v] = part[a*n + v] for a = choice[gseg[v]] >= 0 (part: a union of n
+// vertices replicated once per attempt, copy a first at a*n).
+void copy_attempt(int32_t n, const int32_t* gseg, const int32_t* choice, const int32_t* part, int32_t* best,
+                  hipStream_t s);
 // One graph's CSR (n vertices, m entries) into a union at vertex v0, entry e0.
 void pack_csr(int32_t n, int64_t m, const int32_t* xadj, const int32_t* adjncy, int32_t v0, int32_t e0,
               int32_t* uxadj, int32_t* uadjncy, hipStream_t s);

@@ -285,6 +285,19 @@ __global__ __launch_bounds__(kBlock) void k_copy_improved(int32_t n, const int32
     if (v < n && improved[gseg[v]]) best[v] = part[v];
 }
 
+// best[v] = part[a * n + v] for a = choice[gseg[v]] >= 0 (the chosen copy
+// of a union replicated once per attempt).
+__global__ __launch_bounds__(kBlock) void k_copy_attempt(int32_t n, const int32_t* __restrict__ gseg,
+                                                         const int32_t* __restrict__ choice,
+                                                         const int32_t* __restrict__ part,
+                                                         int32_t* __restrict__ best) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+        const int32_t a = choice[gseg[v]];
+        if (a >= 0) best[v] = part[(int64_t)a * n + v];
+    }
+}
+
 // out[4g..4g+3] = (first vertex, end vertex, first entry, end entry) of
 // every graph g with vertices in this level (out cleared beforehand).
 __global__ __launch_bounds__(kBlock) void k_graph_ranges(int32_t n, const int32_t* __restrict__ gseg,
@@ -500,6 +513,14 @@ void copy_improved(int32_t n, const int32_t* gseg, const uint8_t* improved, cons
     if (n > 0)
         hipLaunchKernelGGL(k_copy_improved, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, gseg, improved,
                            part, best);
+    RK_HIP(hipGetLastError());
+}
+
+void copy_attempt(int32_t n, const int32_t* gseg, const int32_t* choice, const int32_t* part, int32_t* best,
+                  hipStream_t s) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_copy_attempt, dim3((unsigned)std::min<int64_t>(((int64_t)n + kBlock - 1) / kBlock, 16384)),
+                           dim3(kBlock), 0, s, n, gseg, choice, part, best);
     RK_HIP(hipGetLastError());
 }
 

@@ -367,7 +367,7 @@ void Server::worker(int slot) {
             int32_t* d_part = ar.get<int32_t>(n);
             const float* ub = batch[0]->ubvec.empty() ? nullptr : batch[0]->ubvec.data();
             kway_batch_device(*cx, g, G, gvtx.data(), nparts.data(), ub, batch[0]->opt, seeds.data(), d_part,
-                              objval.data(), nullptr);
+                              objval.data(), nullptr, 1);   // one attempt at a time: the footprint is per copy
             host_part.resize(n);
             int32_t* hp = (int32_t*)cx->host_stage2(sizeof(int32_t) * (size_t)n);
             for (int32_t i = 0; i < G; ++i) {
