@@ -70,18 +70,14 @@ __device__ inline uint32_t degree_rank(int32_t d) {
     return (uint32_t)(min(d, 4) + max(0, 29 - (int)__clz(d)));
 }
 
-// Key of edge (v, u) of weight w whose endpoints' smaller degree is mdeg.
-__device__ inline EdgeKey edge_key_m(int32_t v, int32_t u, int32_t w, int32_t mdeg, uint64_t seed, int shem) {
+// Key of edge (v, u) of weight w whose endpoints' smaller degree rank is
+// mrank.
+__device__ inline EdgeKey edge_key_r(int32_t v, int32_t u, int32_t w, uint32_t mrank, uint64_t seed, int shem) {
     const uint32_t lo = (uint32_t)min(v, u), hi = (uint32_t)max(v, u);
     const uint32_t h = mix32(lo ^ mix32(hi ^ (uint32_t)seed) ^ (uint32_t)(seed >> 32));
     if (!shem) return EdgeKey{0ull, h};
-    const uint32_t dpri = 0xFFFFFFFFu - degree_rank(mdeg);
+    const uint32_t dpri = 0xFFFFFFFFu - mrank;
     return EdgeKey{((uint64_t)dpri << 32) | (uint64_t)((uint32_t)w ^ 0x80000000u), h};
-}
-
-__device__ inline EdgeKey edge_key(int32_t v, int32_t u, int32_t w, int32_t dv, int32_t du, uint64_t seed,
-                                   int shem) {
-    return edge_key_m(v, u, w, min(dv, du), seed, shem);
 }
 
 // Strict total order on edges: the key, then the endpoint ids (graph-local,
@@ -95,34 +91,47 @@ __device__ inline bool edge_before(const EdgeKey& k1, int32_t a1, int32_t b1, co
     return l1 < l2 || (l1 == l2 && max(a1, b1) < max(a2, b2));
 }
 
-// suitor[u]: kNoSuitor, or the suitor's id in the high word and the weight
-// w of the edge (suitor, u) in the low word.  When 0 <= w < 2^16 and the
-// smaller degree m of the two endpoints is below 2^15, the word is compact:
-// bit 31 of the high word is set (ids are below 2^31) and the low word is
-// m << 16 | w, so a reader rebuilds the suitor's key without loading the
-// suitor's degree.  The word of a pair is a function of the pair alone.
+// suitor[u] is u's word: who courts u, and enough of that edge's key to
+// compare against it without reloading the suitor's degree, plus u's own
+// degree rank so that a vertex scanning u needs no access to u's row.
+// Compact form (edge weight 0 <= w < 2^16): bit 31 of the high word set,
+// the suitor id below it (all ones: nobody), and the low word
+//   u's rank << 22 | rank of the pair's smaller degree << 16 | w.
+// Wide form (w outside that range; bit 31 of the high word clear): the
+// suitor id and w, the degrees then come from xadj.  The word of a pair is a
+// function of the pair alone.
 constexpr uint64_t kNoSuitor = 0xFFFFFFFF00000000ull;
 
-__device__ inline uint64_t suitor_word(int32_t s, int32_t w, int32_t mdeg) {
-    if ((uint32_t)w < 0x10000u && (uint32_t)mdeg < 0x8000u)
-        return ((uint64_t)((uint32_t)s | 0x80000000u) << 32) | ((uint64_t)(uint32_t)mdeg << 16) | (uint32_t)w;
+__device__ inline uint64_t no_suitor_word(uint32_t urank) { return kNoSuitor | ((uint64_t)urank << 22); }
+
+__device__ inline uint64_t suitor_word(int32_t s, int32_t w, uint32_t mrank, uint32_t urank) {
+    if ((uint32_t)w < 0x10000u)
+        return ((uint64_t)((uint32_t)s | 0x80000000u) << 32) | ((uint64_t)urank << 22) | ((uint64_t)mrank << 16) |
+               (uint32_t)w;
     return ((uint64_t)(uint32_t)s << 32) | (uint32_t)w;
 }
+
+__device__ inline bool word_compact(uint64_t word) { return (word >> 63) != 0; }
 
 __device__ inline int32_t suitor_id(uint64_t word) {
     const uint32_t hi = (uint32_t)(word >> 32);
     return hi == 0xFFFFFFFFu ? -1 : (int32_t)(hi & 0x7FFFFFFFu);
 }
 
-// Key of the edge (s, u) recorded by word (s = suitor_id(word) >= 0; du =
-// degree of u); ids in the keys are graph-local (minus base).
-__device__ inline EdgeKey suitor_key(uint64_t word, int32_t s, int32_t u, int32_t du, const int32_t* xadj,
+// Degree rank of u, the owner of word (from the word, or its row when wide).
+__device__ inline uint32_t owner_rank(uint64_t word, int32_t u, const int32_t* xadj) {
+    if (word_compact(word)) return (uint32_t)(word >> 22) & 63u;
+    return degree_rank(xadj[u + 1] - xadj[u]);
+}
+
+// Key of the edge (s, u) recorded by word (s = suitor_id(word) >= 0; urank =
+// u's degree rank); ids in the keys are graph-local (minus base).
+__device__ inline EdgeKey suitor_key(uint64_t word, int32_t s, int32_t u, uint32_t urank, const int32_t* xadj,
                                      uint64_t seed, int shem, int32_t base) {
     const uint32_t lo = (uint32_t)word;
-    if ((uint32_t)(word >> 32) >> 31)
-        return edge_key_m(s - base, u - base, (int32_t)(lo & 0xFFFFu), (int32_t)(lo >> 16), seed, shem);
-    const int32_t ds = shem ? xadj[s + 1] - xadj[s] : 0;
-    return edge_key(s - base, u - base, (int32_t)lo, ds, du, seed, shem);
+    if (word_compact(word)) return edge_key_r(s - base, u - base, (int32_t)(lo & 0xFFFFu), (lo >> 16) & 63u, seed, shem);
+    const uint32_t rs = shem ? degree_rank(xadj[s + 1] - xadj[s]) : 0u;
+    return edge_key_r(s - base, u - base, (int32_t)lo, min(rs, urank), seed, shem);
 }
 
 struct MatchArgs {
@@ -157,115 +166,136 @@ __device__ inline bool pair_fits(const MatchArgs& a, int32_t x, int32_t y, const
     return true;
 }
 
-__global__ __launch_bounds__(kBlock) void k_suitor_init(int32_t n, uint64_t* __restrict__ suitor) {
+__global__ __launch_bounds__(kBlock) void k_suitor_init(int32_t n, const int32_t* __restrict__ xadj,
+                                                        uint64_t* __restrict__ suitor) {
     for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock)
-        suitor[v] = kNoSuitor;
+        suitor[v] = no_suitor_word(degree_rank(xadj[v + 1] - xadj[v]));
 }
 
-// One chain from vertex v (see k_suitor); CAPPED applies the weight cap
-// (cap non-null).  Inlined twice so that graphs whose cap cannot bind run
-// the uncapped code.
-template <int G, bool CAPPED>
-__device__ __forceinline__ void suitor_chain(const MatchArgs& a, int32_t v, int gl, uint64_t seed, int32_t base,
-                                             const long long* cap) {
-    int32_t cur = (int32_t)v;
-    int32_t fu = -1;          // neighbour of the last failed claim and the word the CAS returned
+// State of a group's chain: the current searcher (-1: none), the neighbour
+// of its last failed claim and the word that CAS returned, and whether its
+// scans read the words at agent scope.  A searcher's first scan reads them
+// through the caches (plain loads may return older words, which only make a
+// neighbour look easier to claim: the claim's CAS then fails); after a
+// failed claim its scans read them at agent scope, as coherent as the CAS,
+// until it holds a vertex and the displaced suitor starts a first scan of
+// its own.
+struct Chain {
+    int32_t cur = -1;
+    int32_t fu = -1;
     uint64_t fword = 0;
-    while (cur >= 0) {
-        const int32_t beg = a.xadj[cur], end = a.xadj[cur + 1];
-        const int32_t dc = end - beg;
-        EdgeKey best{0ull, 0u};
-        int32_t bu = -1, bw = 0, bd = 0;
-        uint64_t bseen = 0;
-        // The best claimable neighbour, first over every neighbour: when
-        // that pair fits the weight cap it is also the best among the
-        // fitting ones; only otherwise is the scan repeated with the cap
-        // applied to each neighbour (the weights are read only then).
-        for (bool filt = false;;) {
-            best = EdgeKey{0ull, 0u};
-            bu = -1;
-            bw = 0;
-            bd = 0;
-            bseen = 0;
-            for (int32_t j = beg + gl; j < end; j += G) {
-                const int32_t u = a.adjncy[j];
-                if (CAPPED && filt && !pair_fits(a, cur, u, cap)) continue;   // the pair would be too heavy
-                const int32_t w = a.adjwgt ? a.adjwgt[j] : 1;
-                uint64_t sv = ld_agent64(&a.suitor[u]);
-                const int32_t du = a.xadj[u + 1] - a.xadj[u];
-                if (u == fu && sv != fword) {
-                    // keep the newer of the loaded word and the CAS result
-                    const int32_t s1 = suitor_id(sv);
-                    const int32_t s2 = suitor_id(fword);
-                    if (s1 < 0) sv = fword;
-                    else if (s2 >= 0) {
-                        const EdgeKey k1 = suitor_key(sv, s1, u, du, a.xadj, seed, a.shem, base);
-                        const EdgeKey k2 = suitor_key(fword, s2, u, du, a.xadj, seed, a.shem, base);
-                        if (edge_before(k2, s2 - base, u - base, k1, s1 - base, u - base)) sv = fword;
-                    }
-                }
-                const int32_t sid = suitor_id(sv);
-                if (sid == cur) continue;
-                const EdgeKey k = edge_key(cur - base, u - base, w, dc, du, seed, a.shem);
-                if (sid >= 0) {
-                    const EdgeKey ks = suitor_key(sv, sid, u, du, a.xadj, seed, a.shem, base);
-                    if (!edge_before(k, cur - base, u - base, ks, sid - base, u - base)) continue;
-                }
-                if (bu < 0 || edge_before(k, cur - base, u - base, best, cur - base, bu - base)) {
-                    best = k;
-                    bu = u;
-                    bw = w;
-                    bd = du;
-                    bseen = sv;
+    bool fresh = false;
+};
+
+// One step of a group's chain (see k_suitor): the searcher's scan and claim.
+// CAPPED applies the weight cap (cap non-null); the kernel inlines both so
+// that graphs whose cap cannot bind run the uncapped code.
+template <int G, bool CAPPED>
+__device__ __forceinline__ void suitor_step(const MatchArgs& a, int gl, uint64_t seed, int32_t base,
+                                            const long long* cap, Chain& ch) {
+    const int32_t cur = ch.cur;
+    const int32_t fu = ch.fu;
+    const uint64_t fword = ch.fword;
+    const bool fresh = ch.fresh;
+    const int32_t beg = a.xadj[cur], end = a.xadj[cur + 1];
+    const uint32_t rc = degree_rank(end - beg);
+    EdgeKey best{0ull, 0u};
+    int32_t bu = -1, bw = 0;
+    uint32_t br = 0;          // the best neighbour's degree rank
+    uint64_t bseen = 0;
+    // The best claimable neighbour, first over every neighbour: when
+    // that pair fits the weight cap it is also the best among the
+    // fitting ones; only otherwise is the scan repeated with the cap
+    // applied to each neighbour (the weights are read only then).
+    for (bool filt = false;;) {
+        best = EdgeKey{0ull, 0u};
+        bu = -1;
+        bw = 0;
+        br = 0;
+        bseen = 0;
+        for (int32_t j = beg + gl; j < end; j += G) {
+            const int32_t u = a.adjncy[j];
+            if (CAPPED && filt && !pair_fits(a, cur, u, cap)) continue;   // the pair would be too heavy
+            const int32_t w = a.adjwgt ? a.adjwgt[j] : 1;
+            uint64_t sv = fresh ? ld_agent64(&a.suitor[u]) : a.suitor[u];
+            const uint32_t ru = owner_rank(sv, u, a.xadj);
+            if (u == fu && sv != fword) {
+                // keep the newer of the loaded word and the CAS result
+                const int32_t s1 = suitor_id(sv);
+                const int32_t s2 = suitor_id(fword);
+                if (s1 < 0) sv = fword;
+                else if (s2 >= 0) {
+                    const EdgeKey k1 = suitor_key(sv, s1, u, ru, a.xadj, seed, a.shem, base);
+                    const EdgeKey k2 = suitor_key(fword, s2, u, ru, a.xadj, seed, a.shem, base);
+                    if (edge_before(k2, s2 - base, u - base, k1, s1 - base, u - base)) sv = fword;
                 }
             }
+            const int32_t sid = suitor_id(sv);
+            if (sid == cur) continue;
+            const EdgeKey k = edge_key_r(cur - base, u - base, w, min(rc, ru), seed, a.shem);
+            if (sid >= 0) {
+                const EdgeKey ks = suitor_key(sv, sid, u, ru, a.xadj, seed, a.shem, base);
+                if (!edge_before(k, cur - base, u - base, ks, sid - base, u - base)) continue;
+            }
+            if (bu < 0 || edge_before(k, cur - base, u - base, best, cur - base, bu - base)) {
+                best = k;
+                bu = u;
+                bw = w;
+                br = ru;
+                bseen = sv;
+            }
+        }
 #pragma unroll
-            for (int o = G / 2; o > 0; o >>= 1) {
-                const EdgeKey ob{__shfl_xor(best.hi, o, G), __shfl_xor(best.lo, o, G)};
-                const int32_t ou = __shfl_xor(bu, o, G);
-                const int32_t ow = __shfl_xor(bw, o, G);
-                const int32_t od = __shfl_xor(bd, o, G);
-                const uint64_t os = __shfl_xor(bseen, o, G);
-                if (ou >= 0 && (bu < 0 || edge_before(ob, cur - base, ou - base, best, cur - base, bu - base))) {
-                    best = ob;
-                    bu = ou;
-                    bw = ow;
-                    bd = od;
-                    bseen = os;
-                }
-            }
-            if (CAPPED && !filt && cap && bu >= 0 && !pair_fits(a, cur, bu, cap)) {   // group-uniform
-                filt = true;
-                continue;
-            }
-            break;
-        }
-        if (bu < 0) break;                                      // nothing to claim: cur stays unmatched
-        // Claim bu.  When the word changed meanwhile but its new suitor is
-        // still beaten, claim again at once: bu stays the best choice
-        // (the other neighbours can only have become harder to claim).
-        uint64_t old = 0;
-        if (gl == 0) {
-            const unsigned long long mine = suitor_word(cur, bw, min(dc, bd));
-            for (;;) {
-                old = atomicCAS((unsigned long long*)&a.suitor[bu], (unsigned long long)bseen, mine);
-                if (old == bseen) break;
-                const int32_t sid = suitor_id(old);
-                if (sid == cur) break;
-                const EdgeKey ks = suitor_key(old, sid, bu, bd, a.xadj, seed, a.shem, base);
-                if (!edge_before(best, cur - base, bu - base, ks, sid - base, bu - base)) break;
-                bseen = old;                                        // still beaten: claim against it
+        for (int o = G / 2; o > 0; o >>= 1) {
+            const EdgeKey ob{__shfl_xor(best.hi, o, G), __shfl_xor(best.lo, o, G)};
+            const int32_t ou = __shfl_xor(bu, o, G);
+            const int32_t ow = __shfl_xor(bw, o, G);
+            const uint32_t orr = __shfl_xor(br, o, G);
+            const uint64_t os = __shfl_xor(bseen, o, G);
+            if (ou >= 0 && (bu < 0 || edge_before(ob, cur - base, ou - base, best, cur - base, bu - base))) {
+                best = ob;
+                bu = ou;
+                bw = ow;
+                br = orr;
+                bseen = os;
             }
         }
-        old = __shfl(old, 0, G);
-        bseen = __shfl(bseen, 0, G);
-        if (old == bseen) {
-            cur = suitor_id(bseen);                                // the displaced suitor searches again
-            fu = -1;
-        } else {
-            fu = bu;
-            fword = old;
+        if (CAPPED && !filt && cap && bu >= 0 && !pair_fits(a, cur, bu, cap)) {   // group-uniform
+            filt = true;
+            continue;
         }
+        break;
+    }
+    if (bu < 0) {                                           // nothing to claim: cur stays unmatched
+        ch.cur = -1;
+        return;
+    }
+    // Claim bu.  When the word changed meanwhile but its new suitor is
+    // still beaten, claim again at once: bu stays the best choice
+    // (the other neighbours can only have become harder to claim).
+    uint64_t old = 0;
+    if (gl == 0) {
+        const unsigned long long mine = suitor_word(cur, bw, min(rc, br), br);
+        for (;;) {
+            old = atomicCAS((unsigned long long*)&a.suitor[bu], (unsigned long long)bseen, mine);
+            if (old == bseen) break;
+            const int32_t sid = suitor_id(old);
+            if (sid == cur) break;
+            const EdgeKey ks = suitor_key(old, sid, bu, br, a.xadj, seed, a.shem, base);
+            if (!edge_before(best, cur - base, bu - base, ks, sid - base, bu - base)) break;
+            bseen = old;                                        // still beaten: claim against it
+        }
+    }
+    old = __shfl(old, 0, G);
+    bseen = __shfl(bseen, 0, G);
+    if (old == bseen) {
+        ch.cur = suitor_id(bseen);                             // the displaced suitor searches again
+        ch.fu = -1;
+        ch.fresh = false;
+    } else {
+        ch.fu = bu;
+        ch.fword = old;
+        ch.fresh = true;
     }
 }
 
@@ -278,29 +308,68 @@ __device__ __forceinline__ void suitor_chain(const MatchArgs& a, int32_t v, int 
 // CAS then fails and returns the current word, which is used for that
 // neighbour in the next scan): every retry sees a newer word, and the chain
 // ends.
+//
+// Each loop iteration is one step of every group's chain.  A wave owns the
+// vertex slots w*GPW + i + r*S (i < GPW groups per wave, rounds r, S groups
+// in the grid), dealt in that order: a group whose chain ended takes the
+// wave's next slot at once (a ballot ranks the groups asking), so the groups
+// do not wait for the longest chain of the wave, while the waves still walk
+// the level together round by round (neighbouring vertices' words stay in
+// the caches).
 template <int G, bool CAPPED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_suitor(MatchArgs a) {
-    constexpr int NG = kBlock / G;
+    constexpr int GPW = 64 / G;
     const int gl = group_lane<G>();
-    for (int64_t v = (int64_t)blockIdx.x * NG + threadIdx.x / G; v < a.n; v += (int64_t)gridDim.x * NG) {
-        if (a.xadj[v + 1] == a.xadj[v]) continue;                 // group-uniform
-        // keys use graph-local ids and the graph's seed, so a graph of a batch
-        // is matched exactly as it is alone
-        uint64_t seed = a.seed;
-        int32_t base = 0;
-        const long long* cap = CAPPED ? a.cap : nullptr;
-        if (a.gseg) {
-            const int32_t gi = a.gseg[v];
-            if (!a.gon[gi]) continue;
-            seed = a.gseed[gi];
-            base = a.gv0[gi];
-            if (CAPPED && cap) {
-                cap += (int64_t)gi * a.ncon;
-                if (cap[0] == LLONG_MAX) cap = nullptr;           // cannot bind for this graph here
+    const int lane = __lane_id();
+    const int64_t S = (int64_t)gridDim.x * (kBlock / G);
+    const int64_t w0 = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / 64 * GPW;
+    int64_t slot = 0;             // wave-uniform: the wave's next slot
+    Chain ch;
+    bool more = true;             // group-uniform: slots may remain for the group
+    // keys use graph-local ids and the graph's seed, so a graph of a batch
+    // is matched exactly as it is alone
+    uint64_t seed = a.seed;
+    int32_t base = 0;
+    const long long* cap = nullptr;
+    for (;;) {
+        // deal slots to the groups without a chain until each has one or the
+        // level is done (a slot of a vertex without work is passed over)
+        for (;;) {
+            const bool ask = more && ch.cur < 0;
+            const uint64_t askers = __ballot(ask && gl == 0);
+            if (askers == 0) break;
+            const int rank = __popcll(askers & ((1ull << lane) - 1ull));   // valid on the group's lane 0
+            const int64_t t = slot + __shfl(rank, lane & ~(G - 1), 64);
+            slot += __popcll(askers);
+            // The slot's vertex, taken unless it is past the level (every
+            // later slot is past it too), has no edges, or belongs to a graph
+            // not matched here.  Every value is written on every asking lane
+            // (selects, no divergent assignments).
+            const int64_t v = w0 + (t % GPW) + (t / GPW) * S;
+            const bool inr = ask && v < a.n;
+            const int64_t vv = inr ? v : 0;
+            bool take = inr && a.xadj[vv + 1] != a.xadj[vv];
+            const int32_t gi = a.gseg && take ? a.gseg[vv] : 0;
+            if (a.gseg) take = take && a.gon[gi] != 0;
+            if (ask) more = inr;
+            if (a.gseg) {
+                const uint64_t gsd = take ? a.gseed[gi] : 0;
+                const int32_t gb = take ? a.gv0[gi] : 0;
+                seed = take ? gsd : seed;
+                base = take ? gb : base;
             }
+            if constexpr (CAPPED) {
+                const long long* c = a.cap && a.gseg ? a.cap + (int64_t)gi * a.ncon : a.cap;
+                const bool bind = take && c && c[0] != LLONG_MAX;   // else the cap cannot bind here
+                cap = take ? (bind ? c : nullptr) : cap;
+            }
+            ch.cur = take ? (int32_t)vv : ch.cur;
+            ch.fu = take ? -1 : ch.fu;
+            ch.fresh = take ? false : ch.fresh;
         }
-        if (CAPPED && cap) suitor_chain<G, true>(a, (int32_t)v, gl, seed, base, cap);
-        else suitor_chain<G, false>(a, (int32_t)v, gl, seed, base, nullptr);
+        if (ch.cur < 0) break;                                 // only when no slot is left
+        if (CAPPED && cap) suitor_step<G, true>(a, gl, seed, base, cap, ch);
+        else suitor_step<G, false>(a, gl, seed, base, nullptr, ch);
     }
 }
 
@@ -699,7 +768,7 @@ void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, CoarsenBufs& 
                       g.vwgt ? cap : nullptr, g.ncon, gs ? gs->gseg : nullptr, gs ? gs->gv0 : nullptr,
                       gs ? gs->gseed : nullptr, gs ? gs->gon : nullptr};
     const int grid = grid_cap(g.n, kBlock, 4096);
-    hipLaunchKernelGGL(k_suitor_init, dim3(grid), dim3(kBlock), 0, s, g.n, b.suitor);
+    hipLaunchKernelGGL(k_suitor_init, dim3(grid), dim3(kBlock), 0, s, g.n, g.xadj, b.suitor);
     const int G = pick_group(g.n, g.m) > 8 ? 8 : pick_group(g.n, g.m);   // short scans: chains dominate
     // persistent grid of exactly the resident blocks: the vertices are dealt
     // to the groups in one pass, with no blocks waiting for a second round
