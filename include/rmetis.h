@@ -282,8 +282,10 @@ int32_t rmetis_hip_parse_options(const idx_t* options, int64_t* out);
  * SmallRng::seed_from_u64(seed) (src/partition/kway.rs:62): the BFS growing
  * the k-way path runs on its coarsest level.  tpwgts[nparts*ncon] are the
  * f64 target weights (already scaled to the total, as kway.rs:67-69 passes
- * them).  Host only; returns METIS_OK or METIS_ERROR_INPUT (also for
- * nparts > nvtxs, where the reference's seed loop would not end). */
+ * them).  Host arrays; the seeds are drawn on the host and the growing runs
+ * on the device (the kernel the k-way path uses); returns METIS_OK or
+ * METIS_ERROR_INPUT (also for nparts > nvtxs, where the reference's seed loop
+ * would not end). */
 int32_t rmetis_hip_grow_partition(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
                                   const idx_t* vwgt, idx_t nparts, const double* tpwgts, uint64_t seed,
                                   idx_t* part);

@@ -3,7 +3,7 @@
 // Every graph of the batch is partitioned exactly as kway_device partitions
 // it alone (partition/kway.rs:20-141): the same matching keys and seeds
 // (graph-local ids, the graph's own seed stream), the same coarsening stop
-// rule (coarsen/mod.rs:49-69), the same host growing of its coarsest graph
+// rule (coarsen/mod.rs:49-69), the same growing of its coarsest graph
 // (initial/grow.rs:12-103), the same greedy passes (greedy.rs:36-193) and
 // the same best-of-ncuts choice (kway.rs:96-98).  What changes is the
 // schedule: the attempts of all graphs run level by level in lockstep (as
@@ -203,7 +203,6 @@ void batch_attempt(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvt
         d_gv00 = put(cx, v0);
     }
     if (st) *st = KwayStats{};
-    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     {
         std::vector<Rng> rng;
         rng.reserve(G);
@@ -289,75 +288,60 @@ void batch_attempt(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvt
             if (!any) break;
         }
         const size_t nl = lv.size();
-        // ---- coarsest graphs to the host, grown there (kway.rs:66-70)
+        // ---- initial partitions of the coarsest graphs (kway.rs:66-70): BFS
+        // growing on the device from seeds drawn here per graph, or the
+        // random parts drawn here (uploaded in one copy)
         cx.set_phase(kPhaseInitial);
         auto t0 = std::chrono::steady_clock::now();
         std::vector<int32_t> lvl_of(G, -1);
-        std::vector<int64_t> csr_off(G, 0), part_off(G, 0);
-        int64_t csr_tot = 0, part_tot = 0;
+        std::vector<int64_t> part_off(G, 0);
+        int64_t part_tot = 0;
+        std::vector<GrowJob> jobs;
         for (size_t l = 1; l < nl; ++l)
             for (int32_t i = 0; i < G; ++i) {
                 if (!lv[l].last[i]) continue;
                 const Extent& e = lv[l].ext[i];
                 lvl_of[i] = (int32_t)l;
-                csr_off[i] = csr_tot;
                 part_off[i] = part_tot;
-                csr_tot += (int64_t)e.n + 1 + e.m + (int64_t)e.n * ncon;
                 part_tot += e.n;
+                if (e.n < gk[i])
+                    throw Error(-4, "graph " + std::to_string(i) + ": coarsest graph has fewer vertices than nparts");
+                if (o.iptype == 1) continue;
+                GrowJob j;
+                j.g = lv[l].g;
+                j.v0 = e.v0;
+                j.e0 = e.e0;
+                j.n = e.n;
+                j.m = e.m;
+                j.k = gk[i];
+                j.tpw.resize((size_t)gk[i] * ncon);
+                for (int32_t p = 0; p < gk[i]; ++p)                             // graph/mod.rs:392-410
+                    for (int32_t c = 0; c < ncon; ++c)
+                        j.tpw[(size_t)p * ncon + c] = (1.0 / (double)gk[i]) * (double)(unit ? e.n : total[i][c]);
+                j.seeds = grow_seeds(e.n, gk[i], rng[i]);                       // grow.rs:24-32
+                jobs.push_back(std::move(j));
             }
-        if (csr_tot > INT32_MAX) throw Error(-4, "coarsest graphs of the batch exceed the staging range");
-        int32_t* d_packed = cx.arena.get<int32_t>(csr_tot);
-        for (size_t l = 1; l < nl; ++l) {
-            std::vector<int32_t> desc;
-            for (int32_t i = 0; i < G; ++i)
-                if (lvl_of[i] == (int32_t)l) {
-                    const Extent& e = lv[l].ext[i];
-                    desc.insert(desc.end(), {e.v0, e.n, e.e0, e.m, (int32_t)csr_off[i]});
-                }
-            if (!desc.empty()) gather_csr(lv[l].g, put(cx, desc), (int32_t)(desc.size() / 5), d_packed, s);
-        }
-        int32_t* hcsr = (int32_t*)cx.host_stage2(sizeof(int32_t) * (size_t)csr_tot);
-        RK_HIP(hipMemcpyAsync(hcsr, d_packed, sizeof(int32_t) * (size_t)csr_tot, hipMemcpyDeviceToHost, s));
-        RK_HIP(hipStreamSynchronize(s));
-        int32_t* hpart = (int32_t*)cx.upload_stage(sizeof(int32_t) * (size_t)std::max<int64_t>(part_tot, 1));
-        {
-            std::atomic<int32_t> next{0};
-            std::atomic<int32_t> failed{-1};
-            auto work = [&]() {
-                std::vector<double> tgt;
-                for (;;) {
-                    const int32_t i = next.fetch_add(1);
-                    if (i >= G) break;
-                    const Extent& e = lv[lvl_of[i]].ext[i];
-                    const int32_t* x = hcsr + csr_off[i];
-                    const int32_t* a = x + e.n + 1;
-                    const int32_t* vw = a + e.m;
-                    int32_t* hp = hpart + part_off[i];
-                    const int32_t k = gk[i];
-                    if (o.iptype == 1) {
-                        for (int32_t v = 0; v < e.n; ++v) hp[v] = (int32_t)rng[i].below((uint64_t)k);  // random.rs:8-10
-                        continue;
-                    }
-                    tgt.assign((size_t)k * ncon, 0.0);
-                    for (int32_t p = 0; p < k; ++p)
-                        for (int32_t c = 0; c < ncon; ++c)
-                            tgt[(size_t)p * ncon + c] =
-                                (1.0 / (double)k) * (double)(unit ? e.n : total[i][c]);
-                    if (!grow_host(e.n, ncon, x, a, unit ? nullptr : vw, k, tgt.data(), rng[i], hp))
-                        failed.store(i);
-                }
-            };
-            const unsigned nt = std::min<unsigned>(hw, (unsigned)G);
-            std::vector<std::thread> th;
-            for (unsigned t = 1; t < nt; ++t) th.emplace_back(work);
-            work();
-            for (auto& t : th) t.join();
-            if (failed.load() >= 0)
-                throw Error(-4, "graph " + std::to_string(failed.load()) +
-                                    ": coarsest graph has fewer vertices than nparts");
-        }
+        // the coarsest graphs' parts, graph after graph (grown in one launch
+        // over every level, or drawn here), scattered into each level's part
+        // array before its refinement
         int32_t* d_gparts = cx.arena.get<int32_t>(std::max<int64_t>(part_tot, 1));
-        RK_HIP(hipMemcpyAsync(d_gparts, hpart, sizeof(int32_t) * (size_t)part_tot, hipMemcpyHostToDevice, s));
+        if (o.iptype == 1) {
+            int32_t* hpart = (int32_t*)cx.upload_stage(sizeof(int32_t) * (size_t)std::max<int64_t>(part_tot, 1));
+            for (int32_t i = 0; i < G; ++i) {
+                if (lvl_of[i] < 0) continue;
+                int32_t* hp = hpart + part_off[i];
+                for (int32_t v = 0; v < lv[lvl_of[i]].ext[i].n; ++v)
+                    hp[v] = (int32_t)rng[i].below((uint64_t)gk[i]);           // initial/random.rs:8-10
+            }
+            RK_HIP(hipMemcpyAsync(d_gparts, hpart, sizeof(int32_t) * (size_t)part_tot, hipMemcpyHostToDevice, s));
+        } else {
+            size_t jj = 0;
+            for (size_t l = 1; l < nl; ++l)
+                for (int32_t i = 0; i < G; ++i)
+                    if (lv[l].last[i]) jobs[jj++].out = d_gparts + part_off[i];
+            grow_on_device(cx, jobs);
+        }
+        if (st) RK_HIP(hipStreamSynchronize(s));
         auto t1 = std::chrono::steady_clock::now();
         // ---- refinement from the deepest level up (kway.rs:73-92)
         int32_t* pc = nullptr;

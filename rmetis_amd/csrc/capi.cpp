@@ -810,7 +810,35 @@ int32_t rmetis_hip_grow_partition(idx_t nvtxs, idx_t ncon, const idx_t* xadj, co
         if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt)) return METIS_ERROR_INPUT;
         if (nparts > nvtxs) return METIS_ERROR_INPUT;   // the reference's seed loop would not end
         Rng rng(seed);
-        if (!grow_host(nvtxs, ncon, xadj, adjncy, vwgt, nparts, tpwgts, rng, part)) return METIS_ERROR_INPUT;
+        ContextGuard cx;
+        DevGraph g = upload_graph(*cx.c, nvtxs, ncon, xadj, adjncy, vwgt, nullptr, nullptr);
+        int32_t* d_part = cx->arena.get<int32_t>(nvtxs);
+        GrowJob job;
+        job.g = g;
+        job.n = nvtxs;
+        job.m = xadj[nvtxs];
+        job.k = nparts;
+        job.tpw.assign(tpwgts, tpwgts + (size_t)nparts * ncon);
+        job.seeds = grow_seeds(nvtxs, nparts, rng);
+        job.out = d_part;
+        {
+            // rows of a user graph may repeat an entry (the validation allows
+            // it): one pass with a per-vertex marker of the current row
+            std::vector<int32_t> mark(nvtxs, -1);
+            bool rep = false;
+            for (int32_t v = 0; v < nvtxs && !rep; ++v)
+                for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j) {
+                    if (mark[adjncy[j]] == v) {
+                        rep = true;
+                        break;
+                    }
+                    mark[adjncy[j]] = v;
+                }
+            job.norep = !rep;
+        }
+        grow_on_device(*cx.c, std::vector<GrowJob>{std::move(job)});
+        download(*cx.c, part, d_part, nvtxs);
+        RK_HIP(hipStreamSynchronize(cx->stream));
         return METIS_OK;
     });
 }

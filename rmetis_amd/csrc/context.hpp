@@ -208,7 +208,7 @@ enum : int32_t {
     kPhaseIdle = 0,
     kPhasePack = 1,       // building the batch's union CSR
     kPhaseCoarsen = 2,    // level = the level being matched and contracted
-    kPhaseInitial = 3,    // coarsest graphs to the host, grown there, back
+    kPhaseInitial = 3,    // initial partitions of the coarsest graphs
     kPhaseRefine = 4,     // level = the level being refined
     kPhaseFinish = 5,     // cuts, best attempts, results to the caller
 };

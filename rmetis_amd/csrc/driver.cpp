@@ -276,14 +276,10 @@ void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* d_
     }
 }
 
-bool grow_host(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
-               int32_t k, const double* tpw, Rng& rng, int32_t* part) {
-    // initial/grow.rs:12-103
-    if (n < k) return false;
-    std::fill(part, part + n, -1);
-    std::vector<int64_t> pwt((size_t)k * ncon, 0);
-    auto vw = [&](int32_t v, int32_t c) -> int64_t { return vwgt ? vwgt[(size_t)v * ncon + c] : 1; };
+std::vector<int32_t> grow_seeds(int32_t n, int32_t k, Rng& rng) {
+    if (n < k) throw Error(-4, "coarsest graph has fewer vertices than nparts");
     std::vector<int32_t> seeds;
+    seeds.reserve(k);
     std::vector<char> used(n, 0);
     while ((int32_t)seeds.size() < k) {                                  // grow.rs:24-32
         const int32_t v = (int32_t)rng.below((uint64_t)n);
@@ -292,62 +288,83 @@ bool grow_host(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjn
             seeds.push_back(v);
         }
     }
-    std::vector<std::deque<int32_t>> q(k);
-    for (int32_t p = 0; p < k; ++p) {
-        q[p].push_back(seeds[p]);
-        part[seeds[p]] = p;
-        for (int32_t c = 0; c < ncon; ++c) pwt[(size_t)p * ncon + c] += vw(seeds[p], c);
-    }
-    int32_t active = k;
-    while (active > 0) {                                                 // grow.rs:50-86
-        active = 0;
-        for (int32_t p = 0; p < k; ++p) {
-            bool at_target = true;
-            for (int32_t c = 0; c < ncon; ++c)
-                if (!((double)pwt[(size_t)p * ncon + c] >= tpw[(size_t)p * ncon + c])) {
-                    at_target = false;
-                    break;
-                }
-            if (at_target) continue;
-            bool grew = false;
-            while (!q[p].empty()) {
-                const int32_t v = q[p].front();
-                q[p].pop_front();
-                for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j) {
-                    const int32_t u = adjncy[j];
-                    if (part[u] == -1) {
-                        part[u] = p;
-                        for (int32_t c = 0; c < ncon; ++c) pwt[(size_t)p * ncon + c] += vw(u, c);
-                        q[p].push_back(u);
-                        grew = true;
-                    }
-                }
-                if (grew) break;
-            }
-            if (grew) ++active;
+    return seeds;
+}
+
+void grow_on_device(Context& cx, const std::vector<GrowJob>& jobs) {
+    if (jobs.empty()) return;
+    // per-job tables one after another; the global scratch only for the
+    // graphs too large to grow in LDS
+    int64_t sk = 0, skc = 0, sbig = 0, sn = 0;
+    size_t lds = 0;
+    for (const GrowJob& j : jobs) {
+        if (j.k >= (1 << 24)) throw Error(-2, "growing more than 2^24 - 1 parts is not supported");
+        const int32_t ncon = j.g.ncon;
+        sk += j.k;
+        skc += (int64_t)j.k * ncon;
+        const size_t need = grow_lds_bytes(j.n, j.m, j.k, ncon);
+        if (need <= kGrowLdsBudget) {
+            lds = std::max(lds, need);
+        } else {
+            lds = std::max(lds, grow_global_lds_bytes(j.k));
+            sn += j.n;
+            if (j.k > 4096) sbig += j.k;
         }
     }
-    // grow.rs:89-100: leftovers to the lightest part on constraint 0 (first
-    // minimum).  A running minimum over parts is kept in a min-heap keyed by
-    // (weight, part) so the scan is O(log k) per leftover vertex.
-    std::vector<std::pair<int64_t, int32_t>> heap;
-    heap.reserve(k);
-    for (int32_t p = 0; p < k; ++p) heap.push_back({pwt[(size_t)p * ncon], p});
-    auto cmp = [](const std::pair<int64_t, int32_t>& x, const std::pair<int64_t, int32_t>& y) {
-        return x.first != y.first ? x.first > y.first : x.second > y.second;
-    };
-    std::make_heap(heap.begin(), heap.end(), cmp);
-    for (int32_t v = 0; v < n; ++v) {
-        if (part[v] != -1) continue;
-        std::pop_heap(heap.begin(), heap.end(), cmp);
-        auto top = heap.back();
-        const int32_t best = top.second;
-        part[v] = best;
-        for (int32_t c = 0; c < ncon; ++c) pwt[(size_t)best * ncon + c] += vw(v, c);
-        heap.back().first = pwt[(size_t)best * ncon];
-        std::push_heap(heap.begin(), heap.end(), cmp);
+    int32_t* qnext = sn ? cx.arena.get<int32_t>(sn) : nullptr;
+    unsigned long long* owner = sn ? cx.arena.get<unsigned long long>(sn) : nullptr;
+    int32_t* left = sn ? cx.arena.get<int32_t>(sn) : nullptr;
+    int32_t* st = cx.arena.get<int32_t>(7 * sk);
+    long long* pw = cx.arena.get<long long>(skc);
+    long long* w0 = sbig ? cx.arena.get<long long>(sbig) : nullptr;
+    double* tpw = cx.arena.get<double>(skc);
+    int32_t* seeds = cx.arena.get<int32_t>(sk);
+    double* htpw = (double*)cx.upload_stage(sizeof(double) * (size_t)skc);
+    int32_t* hseeds = (int32_t*)cx.upload_stage(sizeof(int32_t) * (size_t)sk);
+    GrowArgs* hrows = (GrowArgs*)cx.upload_stage(sizeof(GrowArgs) * jobs.size());
+    int64_t ok = 0, okc = 0, obig = 0, on = 0;
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        const GrowJob& j = jobs[i];
+        const int32_t ncon = j.g.ncon;
+        std::copy(j.tpw.begin(), j.tpw.end(), htpw + okc);
+        std::copy(j.seeds.begin(), j.seeds.end(), hseeds + ok);
+        GrowArgs r;
+        r.xadj = j.g.xadj;
+        r.adjncy = j.g.adjncy;
+        r.vwgt = j.g.vwgt;
+        r.out = j.out;
+        r.v0 = j.v0;
+        r.e0 = j.e0;
+        r.n = j.n;
+        r.m = j.m;
+        r.k = j.k;
+        r.ncon = ncon;
+        r.lds = grow_lds_bytes(j.n, j.m, j.k, ncon) <= kGrowLdsBudget ? 1 : 0;
+        r.norep = j.norep ? 1 : 0;
+        r.tpw = tpw + okc;
+        r.seeds = seeds + ok;
+        r.st = st + 7 * ok;
+        r.pw = pw + okc;
+        if (!r.lds) {
+            r.qnext = qnext + on;
+            r.owner = owner + on;
+            r.leftover = left + on;
+            on += j.n;
+            if (j.k > 4096) {
+                r.w0 = w0 + obig;
+                obig += j.k;
+            }
+        }
+        hrows[i] = r;
+        ok += j.k;
+        okc += (int64_t)j.k * ncon;
     }
-    return true;
+    hipStream_t s = cx.stream;
+    GrowArgs* d_rows = cx.arena.get<GrowArgs>((int64_t)jobs.size());
+    RK_HIP(hipMemcpyAsync(tpw, htpw, sizeof(double) * (size_t)skc, hipMemcpyHostToDevice, s));
+    RK_HIP(hipMemcpyAsync(seeds, hseeds, sizeof(int32_t) * (size_t)sk, hipMemcpyHostToDevice, s));
+    RK_HIP(hipMemcpyAsync(d_rows, hrows, sizeof(GrowArgs) * jobs.size(), hipMemcpyHostToDevice, s));
+    grow_partitions(d_rows, (int32_t)jobs.size(), lds, s);
 }
 
 int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwgts, const float* ubvec,
@@ -460,35 +477,31 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
         }
         long long* d_thr = cx.arena.get<long long>((int64_t)(2 * kc * nl));
         RK_HIP(hipMemcpyAsync(d_thr, stage, 2 * kc * nl * sizeof(long long), hipMemcpyHostToDevice, s));
-        // initial partition of the coarsest level on the host (kway.rs:66-70)
+        // initial partition of the coarsest level (kway.rs:66-70): BFS growing
+        // on the device from seeds drawn here, or the random parts drawn here
         auto t0 = std::chrono::steady_clock::now();
-        // coarsest CSR (and weights) through pinned staging in one synchronisation
-        const size_t nvw = cg.vwgt ? (size_t)cg.n * ncon : 0;
-        int32_t* hb = (int32_t*)cx.host_stage2(sizeof(int32_t) * ((size_t)cg.n + 1 + (size_t)cg.m + nvw));
-        RK_HIP(hipMemcpyAsync(hb, cg.xadj, sizeof(int32_t) * (cg.n + 1), hipMemcpyDeviceToHost, s));
-        RK_HIP(hipMemcpyAsync(hb + cg.n + 1, cg.adjncy, sizeof(int32_t) * cg.m, hipMemcpyDeviceToHost, s));
-        if (nvw)
-            RK_HIP(hipMemcpyAsync(hb + cg.n + 1 + cg.m, cg.vwgt, sizeof(int32_t) * nvw, hipMemcpyDeviceToHost, s));
-        RK_HIP(hipStreamSynchronize(s));
-        std::vector<int32_t> hx(hb, hb + cg.n + 1), ha(hb + cg.n + 1, hb + cg.n + 1 + cg.m), hv, hp(cg.n);
-        if (ha.empty()) ha.resize(1);
-        if (nvw) hv.assign(hb + cg.n + 1 + cg.m, hb + cg.n + 1 + cg.m + nvw);
-        for (int c = 0; c < ncon; ++c) ltot[c] = cg.vwgt ? total[c] : cg.n;
-        std::vector<double> tgt(kc);
-        for (int32_t p = 0; p < k; ++p)
-            for (int32_t c = 0; c < ncon; ++c)
-                tgt[(size_t)p * ncon + c] =
-                    (tpwgts ? (double)tpwgts[(size_t)p * ncon + c] : 1.0 / (double)k) * (double)ltot[c];
-        if (o.iptype == 1) {
-            for (int32_t v = 0; v < cg.n; ++v) hp[v] = (int32_t)rng.below((uint64_t)k);   // initial/random.rs:8-10
-        } else if (!grow_host(cg.n, ncon, hx.data(), ha.data(), cg.vwgt ? hv.data() : nullptr, k, tgt.data(), rng,
-                              hp.data())) {
-            throw Error(-4, "coarsest graph has fewer vertices than nparts");
-        }
         int32_t* pc = cx.arena.get<int32_t>(cg.n);
-        int32_t* hstage = (int32_t*)cx.host_stage2(sizeof(int32_t) * cg.n);
-        std::copy(hp.begin(), hp.end(), hstage);
-        RK_HIP(hipMemcpyAsync(pc, hstage, sizeof(int32_t) * cg.n, hipMemcpyHostToDevice, s));
+        if (o.iptype == 1) {
+            int32_t* hp = (int32_t*)cx.host_stage2(sizeof(int32_t) * cg.n);
+            for (int32_t v = 0; v < cg.n; ++v) hp[v] = (int32_t)rng.below((uint64_t)k);   // initial/random.rs:8-10
+            RK_HIP(hipMemcpyAsync(pc, hp, sizeof(int32_t) * cg.n, hipMemcpyHostToDevice, s));
+        } else {
+            for (int c = 0; c < ncon; ++c) ltot[c] = cg.vwgt ? total[c] : cg.n;
+            GrowJob job;
+            job.g = cg;
+            job.n = cg.n;
+            job.m = cg.m;
+            job.k = k;
+            job.out = pc;
+            job.tpw.resize(kc);
+            for (int32_t p = 0; p < k; ++p)                                 // graph/mod.rs:392-410
+                for (int32_t c = 0; c < ncon; ++c)
+                    job.tpw[(size_t)p * ncon + c] =
+                        (tpwgts ? (double)tpwgts[(size_t)p * ncon + c] : 1.0 / (double)k) * (double)ltot[c];
+            job.seeds = grow_seeds(cg.n, k, rng);
+            grow_on_device(cx, std::vector<GrowJob>{std::move(job)});
+        }
+        if (st) RK_HIP(hipStreamSynchronize(s));
         auto t1 = std::chrono::steady_clock::now();
         // refine the coarsest level, then project and refine every finer level
         LevelStat cstat;

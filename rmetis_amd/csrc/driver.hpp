@@ -132,8 +132,24 @@ constexpr int32_t kDebugNoPwCarry = 1;
 void set_debug_flags(int32_t flags);
 int32_t debug_flags();
 
-// initial/grow.rs:12-103 on the host; false when n < k.
-bool grow_host(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
-               int32_t k, const double* tpw, Rng& rng, int32_t* part);
+// grow.rs:24-32: k distinct seed vertices of a graph of n, drawn from rng
+// (throws Error(-4) when n < k: the reference's draw would not end).
+std::vector<int32_t> grow_seeds(int32_t n, int32_t k, Rng& rng);
+
+// One graph to grow (initial/grow.rs:12-103): its level g, first vertex v0
+// and first entry e0 there, n vertices, m entries, k parts, targets
+// tpw[k*ncon], its seeds, and out[n] (device, graph-local) for the parts.
+struct GrowJob {
+    DevGraph g;
+    int64_t v0 = 0, e0 = 0;
+    int32_t n = 0, m = 0, k = 0;
+    std::vector<double> tpw;
+    std::vector<int32_t> seeds;
+    int32_t* out = nullptr;
+    bool norep = true;    // no row repeats an entry (a contracted level)
+};
+// Grows every job's parts on the device, all in one launch (one workgroup
+// per graph), with scratch from the context's arena.
+void grow_on_device(Context& cx, const std::vector<GrowJob>& jobs);
 
 }  // namespace rk

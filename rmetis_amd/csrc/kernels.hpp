@@ -27,6 +27,35 @@ void gather_csr(const DevGraph& g, const int32_t* desc, int32_t ndesc, int32_t* 
 void copy_improved(int32_t n, const int32_t* gseg, const uint8_t* improved, const int32_t* part, int32_t* best,
                    hipStream_t s);
 void fill_gseg(int32_t ngraphs, const int32_t* gv0, const int32_t* gn, int32_t* gseg, hipStream_t s);
+
+// One graph of a level for grow_partitions (initial/grow.rs:12-103).
+struct GrowArgs {
+    const int32_t* xadj = nullptr;    // the graph's level (union CSR, vertex weights or null)
+    const int32_t* adjncy = nullptr;
+    const int32_t* vwgt = nullptr;
+    int32_t* out = nullptr;           // [n] the grown part, graph-local
+    int64_t v0 = 0, e0 = 0;           // the graph's first vertex and first entry in the level
+    int32_t n = 0, m = 0, k = 0, ncon = 0;   // its vertices, entries, parts (< 2^24), constraints
+    int32_t lds = 0;                  // 1: CSR and tables in LDS (grow_lds_bytes fits), the scratch below unused
+    int32_t norep = 0;                // 1: no row repeats an entry (the long-row commits skip first-position checks)
+    const double* tpw = nullptr;      // [k*ncon] target weights
+    const int32_t* seeds = nullptr;   // [k] distinct graph-local seed vertices (grow.rs:24-32)
+    int32_t* qnext = nullptr;         // [n] scratch: BFS queue links
+    unsigned long long* owner = nullptr;   // [n] scratch: claim stamps
+    int32_t* leftover = nullptr;      // [n] scratch: unassigned vertices in order
+    int32_t* st = nullptr;            // [7k] scratch: queue heads, tails, proposals, flags, long-row lists
+    long long* pw = nullptr;          // [k*ncon] scratch: running part weights
+    long long* w0 = nullptr;          // [k] scratch for k > 4096 (constraint-0 weights)
+};
+// LDS bytes of a graph grown in the workgroup's LDS, and of a graph grown
+// in global memory (its leftover weights).
+size_t grow_lds_bytes(int32_t n, int64_t m, int32_t k, int32_t ncon);
+size_t grow_global_lds_bytes(int32_t k);
+constexpr size_t kGrowLdsBudget = 64 * 1024;   // LDS of one growing workgroup at most
+// Grows the part of every listed graph (one workgroup each, lds bytes of
+// dynamic LDS: the largest need among them) into its out array; the graphs
+// may come from different levels.
+void grow_partitions(const GrowArgs* table, int32_t ngraphs, size_t lds, hipStream_t s);
 // best[v] = part[a*n + v] for a = choice[gseg[v]] >= 0 (part: a union of n
 // vertices replicated once per attempt, copy a first at a*n).
 void copy_attempt(int32_t n, const int32_t* gseg, const int32_t* choice, const int32_t* part, int32_t* best,
