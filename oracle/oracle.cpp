@@ -533,6 +533,287 @@ int grow_impl(const OGraph& g, int32_t nparts, const double* tpw, orc_rng* rng, 
     return 0;
 }
 
+// ---- refine/fm.rs:12-145 (2-way FM) -------------------------------------
+//
+// The reference's queue is priority-queue 1.4.0's PriorityQueue (absent
+// here); among equal gains this restatement pops the smallest vertex first,
+// as in the k-way FM above.
+bool fm2_pass_impl(const OGraph& g, int32_t* part, std::vector<int32_t>& pw, const double* tpw, const double* ub) {
+    const int32_t n = g.n, ncon = g.ncon;
+    const int32_t* ew = g.ewp();
+    std::vector<int32_t> gains(n);
+    for (int32_t v = 0; v < n; ++v)                                          // fm.rs:44-49
+        gains[v] = move_gain(g.xadj.data(), g.adjncy.data(), ew, v, part, part[v], 1 - part[v]);
+    auto boundary = [&](int32_t v) {
+        for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j)
+            if (part[g.adjncy[j]] != part[v]) return true;
+        return false;
+    };
+    std::set<std::pair<int32_t, int32_t>> pq;                                // (-gain, v)
+    std::vector<int32_t> key(n, 0);
+    std::vector<char> inq(n, 0);
+    for (int32_t v = 0; v < n; ++v)                                          // fm.rs:52-57
+        if (boundary(v)) {
+            pq.insert({-gains[v], v});
+            key[v] = gains[v];
+            inq[v] = 1;
+        }
+    std::vector<char> locked(n, 0);
+    std::vector<std::pair<int32_t, int32_t>> moves;                          // (v, from)
+    int32_t cut = edge_cut_impl(n, g.xadj.data(), g.adjncy.data(), ew, part);
+    int32_t best_cut = cut;
+    size_t best_len = 0;
+    while (!pq.empty()) {                                                    // fm.rs:66-121
+        const auto top = *pq.begin();
+        pq.erase(pq.begin());
+        const int32_t v = top.second, gain = -top.first;
+        inq[v] = 0;
+        if (locked[v]) continue;
+        const int32_t from = part[v], to = 1 - from;
+        bool feasible = true;                                                // fm.rs:77-83
+        for (int32_t c = 0; c < ncon; ++c)
+            if (!((double)(pw[(size_t)to * ncon + c] + g.vw(v, c)) <= tpw[(size_t)to * ncon + c] * ub[c])) feasible = false;
+        if (!feasible) continue;
+        part[v] = to;                                                        // fm.rs:86-99
+        for (int32_t c = 0; c < ncon; ++c) {
+            pw[(size_t)from * ncon + c] -= g.vw(v, c);
+            pw[(size_t)to * ncon + c] += g.vw(v, c);
+        }
+        locked[v] = 1;
+        moves.push_back({v, from});
+        cut -= gain;
+        if (cut < best_cut) {
+            best_cut = cut;
+            best_len = moves.size();
+        }
+        for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j) {                // fm.rs:101-120
+            const int32_t u = g.adjncy[j];
+            if (locked[u]) continue;
+            gains[u] = move_gain(g.xadj.data(), g.adjncy.data(), ew, u, part, part[u], 1 - part[u]);
+            if (inq[u]) {
+                pq.erase({-key[u], u});
+                pq.insert({-gains[u], u});
+                key[u] = gains[u];
+            } else if (boundary(u)) {
+                pq.insert({-gains[u], u});
+                key[u] = gains[u];
+                inq[u] = 1;
+            }
+        }
+    }
+    const bool improved = best_len > 0 && best_cut < edge_cut_impl(n, g.xadj.data(), g.adjncy.data(), ew, part);
+    for (size_t i = moves.size(); i > best_len; --i) {                       // fm.rs:127-134
+        const int32_t v = moves[i - 1].first, from = moves[i - 1].second, cur = part[v];
+        for (int32_t c = 0; c < ncon; ++c) {
+            pw[(size_t)cur * ncon + c] -= g.vw(v, c);
+            pw[(size_t)from * ncon + c] += g.vw(v, c);
+        }
+        part[v] = from;
+    }
+    return improved;
+}
+
+// fm.rs:12-28
+int32_t fm2_refine_impl(const OGraph& g, int32_t* part, const double* tpw, const double* ub, int32_t niter) {
+    std::vector<int32_t> pw(2 * (size_t)g.ncon);
+    pweights(g.n, g.ncon, g.vwp(), part, 2, pw.data());
+    for (int32_t it = 0; it < niter; ++it)
+        if (!fm2_pass_impl(g, part, pw, tpw, ub)) break;
+    return edge_cut_impl(g.n, g.xadj.data(), g.adjncy.data(), g.ewp(), part);
+}
+
+// graph/mod.rs:209-270: the vertices of part `target` (ascending) with the
+// edges between them, in CSR order; new_to_old maps the new ids back.
+OGraph extract_impl(const OGraph& g, const int32_t* part, int32_t target, std::vector<int32_t>& new_to_old) {
+    new_to_old.clear();
+    for (int32_t v = 0; v < g.n; ++v)
+        if (part[v] == target) new_to_old.push_back(v);
+    std::vector<int32_t> o2n(g.n, -1);
+    for (size_t i = 0; i < new_to_old.size(); ++i) o2n[new_to_old[i]] = (int32_t)i;
+    OGraph s;
+    s.n = (int32_t)new_to_old.size();
+    s.ncon = g.ncon;
+    s.has_vwgt = g.has_vwgt;
+    s.has_adjwgt = g.has_adjwgt;
+    s.has_vsize = g.has_vsize;
+    s.xadj.assign((size_t)s.n + 1, 0);
+    for (int32_t i = 0; i < s.n; ++i) {
+        const int32_t v = new_to_old[i];
+        for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j)
+            if (part[g.adjncy[j]] == target) {
+                s.adjncy.push_back(o2n[g.adjncy[j]]);
+                if (g.has_adjwgt) s.adjwgt.push_back(g.adjwgt[j]);
+            }
+        s.xadj[i + 1] = (int32_t)s.adjncy.size();
+        if (g.has_vwgt)
+            for (int32_t c = 0; c < g.ncon; ++c) s.vwgt.push_back(g.vwgt[(size_t)v * g.ncon + c]);
+        if (g.has_vsize) s.vsize.push_back(g.vsize[v]);
+    }
+    return s;
+}
+
+struct OOptions {
+    int32_t ctype_shem = 1, iptype = 0, rtype_greedy = 0, niter = 10, ufactor = 30;
+    uint64_t seed = 1;
+};
+
+// partition/recursive.rs:140-187 (multilevel_bisect); tpw2 = the two parts'
+// fractions [2 * ncon] (f32), ub as f64.
+std::vector<int32_t> bisect_impl(const OGraph& g, const float* tpw2, const double* ub, const OOptions& o,
+                                 orc_rng* rng) {
+    if (g.n <= 2) {                                                          // recursive.rs:147-150
+        std::vector<int32_t> p(g.n);
+        for (int32_t v = 0; v < g.n; ++v) p[v] = v;
+        return p;
+    }
+    std::vector<OGraph> lg;                                                  // coarsen/mod.rs:36-71, nparts 2
+    std::vector<std::vector<int32_t>> cmaps;
+    const OGraph* cur = &g;
+    for (;;) {
+        std::vector<int32_t> mt = o.ctype_shem ? shem_impl(*cur, rng) : rm_impl(*cur, rng);
+        std::vector<int32_t> cmap;
+        OGraph cg = coarse_impl(*cur, mt, cmap);
+        const float ratio = (float)cg.n / (float)cur->n;
+        const int32_t cgn = cg.n;
+        lg.push_back(std::move(cg));
+        cmaps.push_back(std::move(cmap));
+        if (cgn <= 40 || ratio > 0.95f) break;
+        cur = &lg.back();
+    }
+    const OGraph& coarsest = lg.back();
+    std::vector<double> ct = targets_impl(coarsest, 2, tpw2);                // recursive.rs:157-159
+    std::vector<int32_t> part(coarsest.n);
+    if (o.iptype == 1) {                                                     // initial/mod.rs:17-20
+        for (int32_t v = 0; v < coarsest.n; ++v) part[v] = (int32_t)orc_rng_range_incl(rng, 0, 1);
+    } else {
+        grow_impl(coarsest, 2, ct.data(), rng, part.data());
+    }
+    auto refine2 = [&](const OGraph& lgph, int32_t* pt, const double* t) {   // refine/mod.rs:11-23
+        if (o.rtype_greedy) greedy_refine_impl(lgph, pt, 2, t, ub, o.niter, nullptr);
+        else fm2_refine_impl(lgph, pt, t, ub, o.niter);
+    };
+    refine2(coarsest, part.data(), ct.data());
+    for (int64_t i = (int64_t)lg.size() - 1; i >= 0; --i) {                  // recursive.rs:169-184
+        const OGraph& fine = i == 0 ? g : lg[i - 1];
+        std::vector<int32_t> fp(fine.n);
+        for (int32_t v = 0; v < fine.n; ++v) fp[v] = part[cmaps[i][v]];
+        std::vector<double> lt = targets_impl(fine, 2, tpw2);
+        refine2(fine, fp.data(), lt.data());
+        part = std::move(fp);
+    }
+    return part;
+}
+
+// recursive.rs:60-137 (rb_recurse); tp = the fractions of the nparts parts.
+void rb_recurse_impl(const OGraph& g, int32_t nparts, const std::vector<float>& tp, const double* ub,
+                     const OOptions& o, orc_rng* rng, int32_t* part) {
+    if (nparts == 1) {                                                       // recursive.rs:70-76
+        std::fill(part, part + g.n, 0);
+        return;
+    }
+    if (g.n <= nparts) {                                                     // recursive.rs:79-86
+        for (int32_t v = 0; v < g.n; ++v) part[v] = std::min(v, nparts - 1);
+        return;
+    }
+    const int32_t k1 = nparts / 2, k2 = nparts - k1, ncon = g.ncon;          // recursive.rs:89-104
+    std::vector<float> lw(ncon, 0.f), rw(ncon, 0.f);
+    for (int32_t c = 0; c < ncon; ++c) {
+        for (int32_t p = 0; p < k1; ++p) lw[c] += tp[(size_t)p * ncon + c];
+        for (int32_t p = k1; p < nparts; ++p) rw[c] += tp[(size_t)p * ncon + c];
+    }
+    const float total = lw[0] + rw[0];
+    std::vector<float> bt(2 * (size_t)ncon);
+    for (int32_t c = 0; c < ncon; ++c) {
+        bt[c] = lw[c] / total;
+        bt[ncon + c] = rw[c] / total;
+    }
+    std::vector<int32_t> bp = bisect_impl(g, bt.data(), ub, o, rng);         // recursive.rs:107
+    std::vector<int32_t> n2o0, n2o1;                                          // recursive.rs:110-111
+    OGraph g0 = extract_impl(g, bp.data(), 0, n2o0);
+    OGraph g1 = extract_impl(g, bp.data(), 1, n2o1);
+    const float s0 = lw[0], s1 = rw[0];                                      // recursive.rs:114-117
+    std::vector<float> t0(tp.begin(), tp.begin() + (size_t)k1 * ncon), t1(tp.begin() + (size_t)k1 * ncon, tp.end());
+    for (float& x : t0) x /= s0;
+    for (float& x : t1) x /= s1;
+    std::vector<int32_t> sp0(g0.n), sp1(g1.n);
+    rb_recurse_impl(g0, k1, t0, ub, o, rng, sp0.data());                     // recursive.rs:123-124
+    rb_recurse_impl(g1, k2, t1, ub, o, rng, sp1.data());
+    for (size_t i = 0; i < n2o0.size(); ++i) part[n2o0[i]] = sp0[i];        // recursive.rs:127-132
+    for (size_t i = 0; i < n2o1.size(); ++i) part[n2o1[i]] = sp1[i] + k1;
+}
+
+// recursive.rs:11-56
+int32_t part_recursive_impl(const OGraph& g, int32_t nparts, const float* tpwgts, const float* ubvec,
+                            const OOptions& o, int32_t* part) {
+    if (nparts < 2 || nparts > g.n) return -2;                               // recursive.rs:18-26
+    orc_rng rng;
+    orc_rng_seed_from_u64(&rng, o.seed);                                     // recursive.rs:28-36
+    std::vector<float> tp = tpwgts ? std::vector<float>(tpwgts, tpwgts + (size_t)nparts * g.ncon)
+                                   : std::vector<float>((size_t)nparts * g.ncon, 1.0f / (float)nparts);
+    std::vector<double> ub(g.ncon);
+    for (int32_t c = 0; c < g.ncon; ++c) ub[c] = ubvec ? (double)ubvec[c] : 1.0 + (double)o.ufactor / 1000.0;
+    rb_recurse_impl(g, nparts, tp, ub.data(), o, &rng, part);
+    return 1;
+}
+
+// separator/mod.rs:10-61
+void separator_impl(const OGraph& g, const OOptions& o, std::vector<int32_t>& left, std::vector<int32_t>& right,
+                    std::vector<int32_t>& sep) {
+    const float tpw[2] = {0.5f, 0.5f}, ubv[1] = {1.05f};
+    std::vector<int32_t> part(g.n);
+    // partition_recursive(graph, 2, Some(&tpwgts), Some(&ubvec), ..): the
+    // two-entry arrays are read as [nparts * ncon] and [ncon] (ncon is 1 here)
+    if (part_recursive_impl(g, 2, tpw, ubv, o, part.data()) != 1)
+        for (int32_t v = 0; v < g.n; ++v) part[v] = v < g.n / 2 ? 0 : 1;    // separator/mod.rs:22-27
+    std::vector<int32_t> pw(2 * (size_t)g.ncon);
+    pweights(g.n, g.ncon, g.vwp(), part.data(), 2, pw.data());
+    const int32_t smaller = pw[0] <= pw[(size_t)g.ncon] ? 0 : 1;
+    std::vector<char> in_sep(g.n, 0);
+    for (int32_t v = 0; v < g.n; ++v) {
+        if (part[v] != smaller) continue;
+        for (int32_t j = g.xadj[v]; j < g.xadj[v + 1]; ++j)
+            if (part[g.adjncy[j]] != part[v]) {
+                in_sep[v] = 1;
+                break;
+            }
+    }
+    left.clear();
+    right.clear();
+    sep.clear();
+    for (int32_t v = 0; v < g.n; ++v) {
+        if (in_sep[v]) sep.push_back(v);
+        else if (part[v] == smaller) left.push_back(v);
+        else right.push_back(v);
+    }
+}
+
+// nd.rs:48-120 (nd_recurse)
+void nd_recurse_impl(const OGraph& g, const std::vector<int32_t>& orig, const OOptions& o, int32_t* perm,
+                     int32_t& counter) {
+    if (g.n <= 100) {                                                        // nd.rs:59-66
+        for (int32_t v : orig) perm[v] = counter++;
+        return;
+    }
+    std::vector<int32_t> left, right, sep;
+    separator_impl(g, o, left, right, sep);
+    if ((int64_t)sep.size() >= g.n / 2) {                                    // nd.rs:77-83
+        for (int32_t v : orig) perm[v] = counter++;
+        return;
+    }
+    std::vector<int32_t> side(g.n, 2);                                        // nd.rs:86-90
+    for (int32_t v : left) side[v] = 0;
+    for (int32_t v : right) side[v] = 1;
+    for (int32_t t = 0; t < 2; ++t) {                                        // nd.rs:92-102
+        if ((t == 0 ? left : right).empty()) continue;
+        std::vector<int32_t> n2o;
+        OGraph sg = extract_impl(g, side.data(), t, n2o);
+        std::vector<int32_t> so(n2o.size());
+        for (size_t i = 0; i < n2o.size(); ++i) so[i] = orig[n2o[i]];
+        nd_recurse_impl(sg, so, o, perm, counter);
+    }
+    for (int32_t v : sep) perm[orig[v]] = counter++;                         // nd.rs:105-108
+}
+
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -844,6 +1125,57 @@ int32_t orc_part_graph_kway(int32_t n, int32_t ncon, const int32_t* xadj, const 
     }
     std::copy(best_part.begin(), best_part.end(), part);
     if (objval) *objval = best_obj;
+    return 1;
+}
+
+int32_t orc_fm2_refine(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
+                       const int32_t* adjwgt, int32_t* part, const double* tpw, const double* ub, int32_t niter) {
+    OGraph g = make_graph(n, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
+    return fm2_refine_impl(g, part, tpw, ub, niter);
+}
+
+int32_t orc_fm2_pass(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
+                     const int32_t* adjwgt, int32_t* part, const double* tpw, const double* ub) {
+    OGraph g = make_graph(n, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
+    std::vector<int32_t> pw(2 * (size_t)g.ncon);
+    pweights(g.n, g.ncon, g.vwp(), part, 2, pw.data());
+    return fm2_pass_impl(g, part, pw, tpw, ub) ? 1 : 0;
+}
+
+int32_t orc_part_graph_recursive(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy,
+                                 const int32_t* vwgt, const int32_t* adjwgt, int32_t nparts, const float* tpwgts,
+                                 const float* ubvec, int32_t ctype_shem, int32_t iptype, int32_t rtype_greedy,
+                                 int32_t niter, uint64_t seed, int32_t ufactor, int32_t* objval, int32_t* part) {
+    OGraph g = make_graph(n, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
+    OOptions o;
+    o.ctype_shem = ctype_shem;
+    o.iptype = iptype;
+    o.rtype_greedy = rtype_greedy;
+    o.niter = niter;
+    o.ufactor = ufactor;
+    o.seed = seed;
+    const int32_t rc = part_recursive_impl(g, nparts, tpwgts, ubvec, o, part);
+    if (rc == 1 && objval) *objval = edge_cut_impl(g.n, g.xadj.data(), g.adjncy.data(), g.ewp(), part);
+    return rc;
+}
+
+int32_t orc_node_nd(int32_t n, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt, int32_t ctype_shem,
+                    int32_t iptype, int32_t rtype_greedy, int32_t niter, uint64_t seed, int32_t ufactor,
+                    int32_t* perm, int32_t* iperm) {
+    // nd.rs:17-46: ordering positions assigned bottom-up, then inverted
+    OGraph g = make_graph(n, 1, xadj, adjncy, vwgt, nullptr, nullptr);
+    OOptions o;
+    o.ctype_shem = ctype_shem;
+    o.iptype = iptype;
+    o.rtype_greedy = rtype_greedy;
+    o.niter = niter;
+    o.ufactor = ufactor;
+    o.seed = seed;
+    std::vector<int32_t> orig(n);
+    for (int32_t v = 0; v < n; ++v) orig[v] = v;
+    int32_t counter = 0;
+    nd_recurse_impl(g, orig, o, perm, counter);
+    for (int32_t v = 0; v < n; ++v) iperm[perm[v]] = v;
     return 1;
 }
 

@@ -97,6 +97,23 @@ int32_t orc_grow_partition(int32_t n, int32_t ncon, const int32_t* xadj, const i
                            const int32_t* vwgt, int32_t nparts, const double* tpwgts,
                            orc_rng* rng, int32_t* part);
 
+/* refine/fm.rs:12-145 (2-way FM); tpw[2*ncon] and ub[ncon] as f64; equal
+ * gains pop the smallest vertex first (the crate's order is unpinned). */
+int32_t orc_fm2_refine(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
+                       const int32_t* adjwgt, int32_t* part, const double* tpw, const double* ub, int32_t niter);
+int32_t orc_fm2_pass(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt,
+                     const int32_t* adjwgt, int32_t* part, const double* tpw, const double* ub);
+/* partition/recursive.rs:11-187 (part_graph_recursive with one SmallRng
+ * seeded from seed through the whole recursion). */
+int32_t orc_part_graph_recursive(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy,
+                                 const int32_t* vwgt, const int32_t* adjwgt, int32_t nparts, const float* tpwgts,
+                                 const float* ubvec, int32_t ctype_shem, int32_t iptype, int32_t rtype_greedy,
+                                 int32_t niter, uint64_t seed, int32_t ufactor, int32_t* objval, int32_t* part);
+/* partition/nd.rs:17-120 with separator/mod.rs:10-61. */
+int32_t orc_node_nd(int32_t n, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt, int32_t ctype_shem,
+                    int32_t iptype, int32_t rtype_greedy, int32_t niter, uint64_t seed, int32_t ufactor,
+                    int32_t* perm, int32_t* iperm);
+
 /* Timing/size counters filled by orc_part_graph_kway when non-NULL. */
 typedef struct {
     double  t_coarsen_s;     /* matching + contraction, all levels, all attempts */

@@ -51,6 +51,11 @@ def lib():
                                           C.c_int32, f32p, f32p, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_int32, C.c_int32, C.c_uint64, C.c_int32, i32p, i32p,
                                           C.POINTER(Stats)]
+        L.orc_part_graph_recursive.argtypes = [C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, C.c_int32, f32p,
+                                               f32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                               C.c_int32, i32p, i32p]
+        L.orc_node_nd.argtypes = [C.c_int32, i32p, i32p, i32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                  C.c_uint64, C.c_int32, i32p, i32p]
         _lib = L
     return _lib
 
@@ -193,6 +198,55 @@ def fm_kway_pass(ncon, xadj, adjncy, vwgt, adjwgt, part, nparts, tpw, ub, pw):
     ok = lib().orc_fm_kway_pass(len(part), ncon, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), _p(part),
                                 nparts, _p(tpw, f64p), _p(ub, f64p), _p(pw), C.byref(nm))
     return bool(ok), part, pw.reshape(nparts, ncon), nm.value
+
+
+def fm2_refine(ncon, xadj, adjncy, vwgt, adjwgt, part, tpw, ub, niter):
+    """refine/fm.rs:12-28 (2-way FM); returns (part, cut)."""
+    xadj, adjncy, vwgt, adjwgt = map(_i32, (xadj, adjncy, vwgt, adjwgt))
+    part = np.array(part, dtype=np.int32)
+    tpw = np.ascontiguousarray(tpw, np.float64)
+    ub = np.ascontiguousarray(ub, np.float64)
+    cut = lib().orc_fm2_refine(len(part), ncon, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), _p(part),
+                               _p(tpw, f64p), _p(ub, f64p), niter)
+    return part, cut
+
+
+def fm2_pass(ncon, xadj, adjncy, vwgt, adjwgt, part, tpw, ub):
+    """One 2-way FM pass (fm.rs:31-145); returns (improved, new part)."""
+    xadj, adjncy, vwgt, adjwgt = map(_i32, (xadj, adjncy, vwgt, adjwgt))
+    part = np.array(part, dtype=np.int32)
+    tpw = np.ascontiguousarray(tpw, np.float64)
+    ub = np.ascontiguousarray(ub, np.float64)
+    ok = lib().orc_fm2_pass(len(part), ncon, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), _p(part),
+                            _p(tpw, f64p), _p(ub, f64p))
+    return bool(ok), part
+
+
+def part_graph_recursive(xadj, adjncy, nparts, vwgt=None, adjwgt=None, ncon=1, tpwgts=None, ubvec=None,
+                         ctype_shem=True, iptype=4, rtype_greedy=False, niter=10, seed=1, ufactor=1):
+    """partition/recursive.rs:11-187 (defaults: Options::for_recursive())."""
+    xadj, adjncy, vwgt, adjwgt = map(_i32, (xadj, adjncy, vwgt, adjwgt))
+    n = len(xadj) - 1
+    part = np.zeros(n, np.int32)
+    obj = C.c_int32()
+    tp = None if tpwgts is None else np.ascontiguousarray(tpwgts, np.float32)
+    ub = None if ubvec is None else np.ascontiguousarray(ubvec, np.float32)
+    rc = lib().orc_part_graph_recursive(n, ncon, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), nparts, _p(tp, f32p),
+                                        _p(ub, f32p), int(ctype_shem), iptype, int(rtype_greedy), niter, seed,
+                                        ufactor, C.byref(obj), _p(part))
+    return rc, obj.value, part
+
+
+def node_nd(xadj, adjncy, vwgt=None, ctype_shem=True, iptype=0, rtype_greedy=False, niter=10, seed=1,
+            ufactor=30):
+    """partition/nd.rs:17-120 (defaults: Options::default()); returns (perm, iperm)."""
+    xadj, adjncy, vwgt = map(_i32, (xadj, adjncy, vwgt))
+    n = len(xadj) - 1
+    perm = np.zeros(n, np.int32)
+    iperm = np.zeros(n, np.int32)
+    lib().orc_node_nd(n, _p(xadj), _p(adjncy), _p(vwgt), int(ctype_shem), iptype, int(rtype_greedy), niter, seed,
+                      ufactor, _p(perm), _p(iperm))
+    return perm, iperm
 
 
 def shem_matching(xadj, adjncy, adjwgt, r):
