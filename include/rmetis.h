@@ -65,6 +65,51 @@ int32_t RMETIS_PartGraphKway(const idx_t* nvtxs, const idx_t* ncon, const idx_t*
                              const real_t* tpwgts, const real_t* ubvec, const idx_t* options, idx_t* objval,
                              idx_t* part);
 
+/* Replaces RMETIS_PartGraphRecursive, src/ffi/c_api.rs:25-48 (+
+ * c_partition_impl 140-209 with kway = false, i.e.
+ * partition/recursive.rs:11-57).  Same arguments, copies, checks (nparts
+ * outside 2..nvtxs: METIS_ERROR_INPUT), option parsing and return codes as
+ * RMETIS_PartGraphKway.  Multilevel recursive bisection on the current HIP
+ * device: each depth's subgraphs are bisected together (RTYPE Greedy: greedy
+ * passes; otherwise the 2-way FM of refine/fm.rs); objval is the edge cut. */
+int32_t RMETIS_PartGraphRecursive(const idx_t* nvtxs, const idx_t* ncon, const idx_t* xadj, const idx_t* adjncy,
+                                  const idx_t* vwgt, const idx_t* vsize, const idx_t* adjwgt, const idx_t* nparts,
+                                  const real_t* tpwgts, const real_t* ubvec, const idx_t* options, idx_t* objval,
+                                  idx_t* part);
+
+/* Replaces RMETIS_NodeND, src/ffi/c_api.rs:83-134 (partition/nd.rs:16-44,
+ * separator/mod.rs:10-61).  Required: nvtxs, xadj, adjncy, perm, iperm
+ * (NULL or nvtxs == 0: METIS_ERROR_INPUT); vwgt[n] and options optional.  The
+ * graph is validated as Graph::new(n, 1, ...) does.  perm[v] = the position
+ * of vertex v in the fill-reducing order, iperm its inverse. */
+int32_t RMETIS_NodeND(const idx_t* nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* vwgt,
+                      const idx_t* options, idx_t* perm, idx_t* iperm);
+
+/* RMETIS_PartGraphRecursive / RMETIS_NodeND on a graph already in device
+ * memory (read in place; d_part / d_perm / d_iperm are device buffers of
+ * nvtxs entries).  No reference counterpart: the same computations without
+ * the copies, for callers whose CSR lives in HBM. */
+int32_t rmetis_hip_recursive_device(idx_t nvtxs, idx_t ncon, const idx_t* d_xadj, const idx_t* d_adjncy,
+                                    const idx_t* d_vwgt, const idx_t* d_adjwgt, idx_t nparts, const real_t* tpwgts,
+                                    const real_t* ubvec, const idx_t* options, idx_t* objval, idx_t* d_part);
+int32_t rmetis_hip_nodend_device(idx_t nvtxs, const idx_t* d_xadj, const idx_t* d_adjncy, const idx_t* d_vwgt,
+                                 const idx_t* options, idx_t* d_perm, idx_t* d_iperm);
+
+/* METIS .graph files (README.md:208-218, DEVELOPMENT_GUIDE.md:846-874):
+ * header "nvtxs nedges [fmt] [ncon]", one line per vertex with its size
+ * (fmt 100), ncon weights (fmt 010) and 1-based neighbours each followed by
+ * its edge weight (fmt 001); '%' starts a comment line.  The reference names
+ * the format but ships no reader.  rmetis_read_graph fills 0-based CSR
+ * arrays allocated with malloc (NULL for absent weights / sizes; free them
+ * with rmetis_free) and returns METIS_OK or METIS_ERROR_INPUT for a missing
+ * or malformed file (message in rmetis_hip_last_error()).
+ * rmetis_write_graph writes the same format (NULL weights are left out). */
+int32_t rmetis_read_graph(const char* path, idx_t* nvtxs, idx_t* ncon, idx_t** xadj, idx_t** adjncy, idx_t** vwgt,
+                          idx_t** vsize, idx_t** adjwgt);
+int32_t rmetis_write_graph(const char* path, idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                           const idx_t* vwgt, const idx_t* vsize, const idx_t* adjwgt);
+void rmetis_free(void* p);
+
 /* Per-level figures of the last attempt of a k-way call (finest first; for
  * a batched call, of its last pass over the union of the attempts it ran
  * together). */
@@ -236,6 +281,16 @@ int32_t rmetis_hip_greedy_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, con
 int32_t rmetis_hip_fm_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy, const idx_t* vwgt,
                              const idx_t* adjwgt, idx_t* part, idx_t nparts, const double* tpwgts,
                              const double* ubvec, idx_t niter, idx_t* cut);
+
+/* fm_refine, src/refine/fm.rs:12-29: up to niter 2-way FM passes on part
+ * (0/1 per vertex, in/out), stopping after a pass that does not improve;
+ * tpwgts[2*ncon] f64 target weights and ubvec[ncon] f64 factors as the
+ * reference's function receives them; *cut = the final edge cut.  The
+ * queue pops equal gains smallest vertex first (the reference's order among
+ * equal gains is internal to its priority-queue crate). */
+int32_t rmetis_hip_fm2_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy, const idx_t* vwgt,
+                              const idx_t* adjwgt, idx_t* part, const double* tpwgts, const double* ubvec, idx_t niter,
+                              idx_t* cut);
 
 /* The GPU matching used in place of shem_matching / random_matching
  * (src/coarsen/shem.rs:13-56, src/coarsen/rm.rs:13-43): seeded, deterministic,

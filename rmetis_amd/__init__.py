@@ -1,7 +1,9 @@
 """rmetis_amd -- MI355X (gfx950) multilevel k-way partitioning, a drop-in for
 javagg/rmetis's ``part_graph_kway`` / ``Graph`` / ``Options`` API (src/lib.rs:48-56,
 src/graph/mod.rs:13-50, src/types.rs:12-245) and ``RMETIS_PartGraphKway`` C ABI
-(src/ffi/c_api.rs:55-81).
+(src/ffi/c_api.rs:55-81), with its siblings ``part_graph_recursive`` /
+``node_nd`` (``RMETIS_PartGraphRecursive`` / ``RMETIS_NodeND``, c_api.rs:25-134)
+and METIS ``.graph`` files (``read_graph`` / ``write_graph``).
 
 The host-side Python mirror calls the C ABI of ``librmetis_hip.so``; every
 computation runs in the HIP kernels on the current GPU.  Names, argument
@@ -11,6 +13,7 @@ meanings and errors follow the reference: invalid input raises
 """
 import ctypes as C
 import enum
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -290,6 +293,19 @@ def fm_kway_refine(graph, part, nparts, tpwgts, ubvec, niter):
     return part, cut.value
 
 
+def fm2_refine(graph, part, tpwgts, ubvec, niter):
+    """refine/fm.rs:12-29 (2-way FM) -> (new part, cut).  part: 0/1 per
+    vertex; tpwgts: f64 targets [2*ncon]; ubvec: f64 [ncon]."""
+    part = np.array(part, dtype=np.int32)
+    tp = np.ascontiguousarray(tpwgts, np.float64)
+    ub = np.ascontiguousarray(ubvec, np.float64)
+    cut = C.c_int32()
+    _raise(_lib.lib().rmetis_hip_fm2_refine(
+        graph.nvtxs, graph.ncon, _p(graph.xadj), _p(graph.adjncy), _p(graph.vwgt), _p(graph.adjwgt), _p(part),
+        _p(tp, _lib.f64p), _p(ub, _lib.f64p), niter, C.byref(cut)))
+    return part, cut.value
+
+
 def greedy_pass(graph, part, nparts, tpwgts, ubvec, mode, pw):
     """One synchronous greedy pass (mode 0 repair, 1 cut) -> (part, pw, moves)."""
     part = np.array(part, dtype=np.int32)
@@ -337,6 +353,81 @@ def part_graph_kway(graph, nparts, tpwgts=None, ubvec=None, options=None):
                                          _p(tp, _lib.f32p), _p(ub, _lib.f32p), _p(opts), C.byref(obj), _p(part))
     _raise(rc)
     return PartitionResult(part=part, objval=obj.value)
+
+
+def part_graph_recursive(graph, nparts, tpwgts=None, ubvec=None, options=None):
+    """src/lib.rs:44 (partition/recursive.rs:11-57) through
+    RMETIS_PartGraphRecursive (c_api.rs:25-48)."""
+    options = options if options is not None else Options()
+    n = C.c_int32(graph.nvtxs)
+    nc = C.c_int32(graph.ncon)
+    np_ = C.c_int32(int(nparts))
+    tp = None if tpwgts is None else np.ascontiguousarray(tpwgts, np.float32)
+    ub = None if ubvec is None else np.ascontiguousarray(ubvec, np.float32)
+    opts = options.to_c_array()
+    part = np.zeros(graph.nvtxs, np.int32)
+    obj = C.c_int32()
+    rc = _lib.lib().RMETIS_PartGraphRecursive(C.byref(n), C.byref(nc), _p(graph.xadj), _p(graph.adjncy),
+                                              _p(graph.vwgt), _p(graph.vsize), _p(graph.adjwgt), C.byref(np_),
+                                              _p(tp, _lib.f32p), _p(ub, _lib.f32p), _p(opts), C.byref(obj),
+                                              _p(part))
+    _raise(rc)
+    return PartitionResult(part=part, objval=obj.value)
+
+
+@dataclass
+class NDResult:
+    """src/types.rs:132-139: perm[v] = new position of vertex v, iperm its
+    inverse."""
+    perm: np.ndarray
+    iperm: np.ndarray
+
+
+def node_nd(graph, options=None):
+    """src/lib.rs:45 (partition/nd.rs:16-44) through RMETIS_NodeND
+    (c_api.rs:83-134).  Uses the graph's CSR and vertex weights (ncon 1)."""
+    options = options if options is not None else Options()
+    n = C.c_int32(graph.nvtxs)
+    opts = options.to_c_array()
+    perm = np.zeros(graph.nvtxs, np.int32)
+    iperm = np.zeros(graph.nvtxs, np.int32)
+    vw = graph.vwgt if graph.ncon == 1 else None
+    rc = _lib.lib().RMETIS_NodeND(C.byref(n), _p(graph.xadj), _p(graph.adjncy), _p(vw), _p(opts), _p(perm),
+                                  _p(iperm))
+    _raise(rc)
+    return NDResult(perm=perm, iperm=iperm)
+
+
+def read_graph(path):
+    """A METIS .graph file (README.md:208-218) -> Graph (0-based CSR, the
+    file's vertex weights, sizes and edge weights when present)."""
+    n, nc = C.c_int32(), C.c_int32()
+    px, pa, pw, pz, pe = (_lib.i32p() for _ in range(5))
+    L = _lib.lib()
+    _raise(L.rmetis_read_graph(os.fsencode(path), C.byref(n), C.byref(nc), C.byref(px), C.byref(pa), C.byref(pw),
+                               C.byref(pz), C.byref(pe)))
+    try:
+        def arr(ptr, count):
+            if not ptr:
+                return None
+            return np.ctypeslib.as_array(ptr, shape=(count,)).copy() if count > 0 else np.zeros(0, np.int32)
+        xadj = arr(px, n.value + 1)
+        m = int(xadj[-1])
+        adjncy = arr(pa, m)
+        vwgt = arr(pw, n.value * nc.value)
+        vsize = arr(pz, n.value)
+        adjwgt = arr(pe, m)
+    finally:
+        for ptr in (px, pa, pw, pz, pe):
+            if ptr:
+                L.rmetis_free(ptr)
+    return Graph(n.value, nc.value, xadj, adjncy, vwgt=vwgt, adjwgt=adjwgt, vsize=vsize)
+
+
+def write_graph(path, graph):
+    """Writes graph as a METIS .graph file (1-based, weights when present)."""
+    _raise(_lib.lib().rmetis_write_graph(os.fsencode(path), graph.nvtxs, graph.ncon, _p(graph.xadj),
+                                         _p(graph.adjncy), _p(graph.vwgt), _p(graph.vsize), _p(graph.adjwgt)))
 
 
 def pack_batch(graphs):

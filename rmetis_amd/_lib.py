@@ -32,6 +32,19 @@ SIGNATURES = {
     "RMETIS_SetDefaultOptions": (C.c_int32, [i32p]),
     "RMETIS_PartGraphKway": (C.c_int32, [i32p, i32p, i32p, i32p, i32p, i32p, i32p, i32p, f32p, f32p, i32p,
                                          i32p, i32p]),
+    "RMETIS_PartGraphRecursive": (C.c_int32, [i32p, i32p, i32p, i32p, i32p, i32p, i32p, i32p, f32p, f32p, i32p,
+                                              i32p, i32p]),
+    "RMETIS_NodeND": (C.c_int32, [i32p, i32p, i32p, i32p, i32p, i32p, i32p]),
+    "rmetis_hip_recursive_device": (C.c_int32, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, C.c_int32, f32p, f32p, i32p, i32p, C.c_void_p]),
+    "rmetis_hip_nodend_device": (C.c_int32, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, i32p, C.c_void_p,
+                                             C.c_void_p]),
+    "rmetis_read_graph": (C.c_int32, [C.c_char_p, i32p, i32p, C.POINTER(i32p), C.POINTER(i32p), C.POINTER(i32p),
+                                      C.POINTER(i32p), C.POINTER(i32p)]),
+    "rmetis_write_graph": (C.c_int32, [C.c_char_p, C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, i32p]),
+    "rmetis_free": (None, [C.c_void_p]),
+    "rmetis_hip_fm2_refine": (C.c_int32, [C.c_int32, C.c_int32, i32p, i32p, i32p, i32p, i32p, f64p, f64p,
+                                          C.c_int32, i32p]),
     "rmetis_hip_kway_device": (C.c_int32, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_void_p, C.c_int32, f32p, f32p, i32p, i32p, C.c_void_p,
                                            C.POINTER(HipStats)]),

@@ -78,14 +78,58 @@ std::vector<Extent> read_extents(Context& cx, const DevGraph& g, const int32_t* 
     return e;
 }
 
+// The 2-way FM tables of a level: the graphs present, each with its tree
+// region, and per tree level the prefix of their node counts.
+Fm2Level fm2_level(Context& cx, const int32_t* gseg, const std::vector<Extent>& ext, int32_t G) {
+    std::vector<Fm2Graph> gd;
+    std::vector<int32_t> of(G, -1);
+    int64_t tb = 0;
+    int nlvl = 1;
+    for (int32_t i = 0; i < G; ++i) {
+        if (ext[i].n <= 0) continue;
+        Fm2Graph d;
+        d.g = i;
+        d.v0 = ext[i].v0;
+        d.n = ext[i].n;
+        d.tbase = tb;
+        of[i] = (int32_t)gd.size();
+        gd.push_back(d);
+        tb += fm2_tree_nodes(ext[i].n);
+        nlvl = std::max(nlvl, (int)fm2_tree_levels(ext[i].n));
+    }
+    Fm2Level lv;
+    lv.ngraphs = (int32_t)gd.size();
+    lv.gtotal = G;
+    lv.nlvl = nlvl;
+    lv.gseg = gseg;
+    const size_t ng = gd.size();
+    std::vector<int64_t> nodes((size_t)kFm2MaxLvl * (ng + 1), 0);
+    for (int l = 1; l < nlvl; ++l) {
+        int64_t acc = 0;
+        for (size_t i = 0; i < ng; ++i) {
+            nodes[(size_t)l * (ng + 1) + i] = acc;
+            acc += fm2_level_nodes(gd[i].n, l);   // 0 when the graph's tree is shorter
+        }
+        nodes[(size_t)l * (ng + 1) + ng] = acc;
+        lv.level_nodes[l] = acc;
+    }
+    lv.graphs = put(cx, gd);
+    lv.gdesc_of = put(cx, of);
+    lv.nodes = put(cx, nodes);
+    return lv;
+}
+
 inline int64_t padded_tiles(int32_t n) { return ((((int64_t)n + kTile - 1) / kTile) + 31) & ~(int64_t)31; }
 
 // One attempt of every graph of the union g: graph i runs with the seed
-// seeds[i] as given (the caller derives it per attempt).  Writes d_part[g.n]
-// and objval[G] (host).
+// seeds[i] as given (the caller derives it per attempt) and, when tpw is
+// given, the target fractions tpw[gkc[i] + p*ncon + c] of its parts (the
+// k*ncon tables of the graphs one after another; else 1/k each).  Greedy
+// refinement, or 2-way FM (refine/fm.rs) when every graph has two parts.
+// Writes d_part[g.n] and objval[G] (host).
 void batch_attempt(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvtx, const int32_t* nparts,
                    const float* ubvec, const Options& o, const uint64_t* seeds, int32_t* d_part, int32_t* objval,
-                   KwayStats* st) {
+                   KwayStats* st, const float* tpw = nullptr) {
     const int32_t ncon = g.ncon;
     std::vector<int32_t> gk(G);
     std::vector<int32_t> gkc(G);
@@ -105,6 +149,7 @@ void batch_attempt(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvt
         tiles0 += padded_tiles(n);
     }
     if (tiles0 > INT32_MAX / kTile) throw Error(-2, "batch too large for the tile tables");
+    if (!o.rtype_greedy && kmax != 2) throw Error(-2, "batched FM refinement is the 2-way FM of bisections");
     hipStream_t s = cx.stream;
     ArenaAlloc alloc(cx.arena);
     Buffers B = alloc_buffers(cx, g.n, g.m, kmax, ncon, tiles0, kc_total);
@@ -162,7 +207,8 @@ void batch_attempt(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvt
         std::vector<int32_t> t(ncon);
         for (int32_t i = 0; i < G; ++i) {
             ltot(i, t.data());
-            level_thresholds(gk[i], ncon, nullptr, t.data(), ub.data(), fm.data() + gkc[i], om.data() + gkc[i]);
+            level_thresholds(gk[i], ncon, tpw ? tpw + gkc[i] : nullptr, t.data(), ub.data(), fm.data() + gkc[i],
+                             om.data() + gkc[i]);
         }
         put(cx, fm, B.rb.fmax);
         put(cx, om, B.rb.omax);
@@ -193,6 +239,29 @@ void batch_attempt(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvt
             }
         return any ? put(cx, t) : nullptr;
     };
+    // 2-way FM work buffers (trees sized by the finest level's graphs)
+    Fm2Bufs F2{};
+    if (!o.rtype_greedy) {
+        int64_t tree = 0;
+        for (int32_t i = 0; i < G; ++i) tree += fm2_tree_nodes(gvtx[i + 1] - gvtx[i]);
+        F2.leaf = cx.arena.get<uint64_t>(g.n);
+        F2.tree = cx.arena.get<uint64_t>(tree);
+        F2.state = (uint8_t*)cx.arena.get<int32_t>(((int64_t)g.n + 3) / 4);
+        F2.gain = cx.arena.get<int32_t>(g.n);
+        F2.ext = cx.arena.get<int32_t>(g.n);
+        F2.moves = cx.arena.get<int32_t>(g.n);
+        int32_t* symf = cx.arena.get<int32_t>(1);
+        fm2_symmetry(g, symf, s);             // holds for every level: contraction keeps it
+        F2.sym = symf;
+        int32_t* pg = cx.arena.get<int32_t>(5 * (int64_t)G + 1);
+        F2.cut0 = pg;
+        F2.cut1 = pg + G;
+        F2.best_len = pg + 2 * (int64_t)G;
+        F2.best_cut = pg + 3 * (int64_t)G;
+        F2.nmoves = pg + 4 * (int64_t)G;
+        F2.err = pg + 5 * (int64_t)G;
+        RK_HIP(hipMemsetAsync(F2.err, 0, sizeof(int32_t), s));
+    }
     int32_t* fin_heavy = cx.arena.get<int32_t>(g.n);
     const int32_t fin_nheavy = heavy_list_sync(g, fin_heavy, B.rb, s);
     const std::vector<Extent> ext0 = read_extents(cx, g, gseg0, G, d_ranges);
@@ -316,8 +385,10 @@ void batch_attempt(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvt
                 j.k = gk[i];
                 j.tpw.resize((size_t)gk[i] * ncon);
                 for (int32_t p = 0; p < gk[i]; ++p)                             // graph/mod.rs:392-410
-                    for (int32_t c = 0; c < ncon; ++c)
-                        j.tpw[(size_t)p * ncon + c] = (1.0 / (double)gk[i]) * (double)(unit ? e.n : total[i][c]);
+                    for (int32_t c = 0; c < ncon; ++c) {
+                        const double f = tpw ? (double)tpw[gkc[i] + p * ncon + c] : 1.0 / (double)gk[i];
+                        j.tpw[(size_t)p * ncon + c] = f * (double)(unit ? e.n : total[i][c]);
+                    }
                 j.seeds = grow_seeds(e.n, gk[i], rng[i]);                       // grow.rs:24-32
                 jobs.push_back(std::move(j));
             }
@@ -422,9 +493,14 @@ void batch_attempt(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvt
                 });
             part_weights_batch(L.g, pf, L.gseg, d_gkc, d_gk, kc_total, B.rb.pw, s);   // greedy.rs:50
             RK_HIP(hipMemsetAsync(d_gdone, 0, sizeof(int32_t) * (size_t)G, s));
-            refine_pass_batch(L.g, bl, 0, L.nheavy, L.heavy, B.rb, pf, s);                  // greedy.rs:60-123
-            for (int it = 0; it < o.niter; ++it)                                           // greedy.rs:126-190
-                refine_pass_batch(L.g, bl, 1, L.nheavy, L.heavy, B.rb, pf, s, it > 0);
+            if (o.rtype_greedy) {
+                refine_pass_batch(L.g, bl, 0, L.nheavy, L.heavy, B.rb, pf, s);              // greedy.rs:60-123
+                for (int it = 0; it < o.niter; ++it)                                       // greedy.rs:126-190
+                    refine_pass_batch(L.g, bl, 1, L.nheavy, L.heavy, B.rb, pf, s, it > 0);
+            } else {                                                                       // refine/mod.rs:179-182
+                const Fm2Level fl = fm2_level(cx, L.gseg, L.ext, G);
+                fm2_refine_batch(L.g, fl, F2, d_gkc, B.rb.fmax, B.rb.pw, pf, d_gdone, o.niter, d_cutacc, s);
+            }
             if (st) {
                 RK_HIP(hipEventRecord(e1, s));
                 RK_HIP(hipEventSynchronize(e1));
@@ -461,6 +537,54 @@ void batch_attempt(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvt
 }
 
 }  // namespace
+
+void fm2_refine_device(Context& cx, const DevGraph& g, const long long* d_fmax, int niter, int32_t* part) {
+    hipStream_t s = cx.stream;
+    int32_t* gseg = cx.arena.get<int32_t>(g.n);
+    RK_HIP(hipMemsetAsync(gseg, 0, sizeof(int32_t) * (size_t)g.n, s));
+    std::vector<Extent> ext(1);
+    ext[0].n = g.n;
+    const Fm2Level fl = fm2_level(cx, gseg, ext, 1);
+    Fm2Bufs F2{};
+    F2.leaf = cx.arena.get<uint64_t>(g.n);
+    F2.tree = cx.arena.get<uint64_t>(fm2_tree_nodes(g.n));
+    F2.state = (uint8_t*)cx.arena.get<int32_t>(((int64_t)g.n + 3) / 4);
+    F2.gain = cx.arena.get<int32_t>(g.n);
+    F2.ext = cx.arena.get<int32_t>(g.n);
+    F2.moves = cx.arena.get<int32_t>(g.n);
+    int32_t* symf = cx.arena.get<int32_t>(1);
+    fm2_symmetry(g, symf, s);
+    F2.sym = symf;
+    int32_t* pg = cx.arena.get<int32_t>(16);
+    F2.cut0 = pg;
+    F2.cut1 = pg + 1;
+    F2.best_len = pg + 2;
+    F2.best_cut = pg + 3;
+    F2.nmoves = pg + 4;
+    F2.err = pg + 5;
+    int32_t* gdone = pg + 6;
+    int32_t* gkc = pg + 7;
+    int32_t* gk = pg + 8;
+    RK_HIP(hipMemsetAsync(pg, 0, sizeof(int32_t) * 16, s));
+    put(cx, std::vector<int32_t>{2}, gk);
+    int32_t* pw = cx.arena.get<int32_t>(2 * (int64_t)g.ncon);
+    unsigned long long* acc = cx.arena.get<unsigned long long>(1);
+    part_weights_batch(g, part, gseg, gkc, gk, 2 * (int64_t)g.ncon, pw, s);   // fm.rs:20
+    fm2_refine_batch(g, fl, F2, gkc, d_fmax, pw, part, gdone, niter, acc, s);
+    int32_t err = 0;
+    RK_HIP(hipMemcpyAsync(&err, F2.err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    RK_HIP(hipStreamSynchronize(s));
+    if (err) throw Error(-4, "2-way FM pass exceeded its pop budget");
+}
+
+void bisect_batch_device(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvtx, const float* tpw2,
+                         const float* ubvec, const Options& o, const uint64_t* seeds, int32_t* d_part) {
+    if (G <= 0) return;
+    if (gvtx[0] != 0 || gvtx[G] != g.n) throw Error(-4, "bisection batch offsets do not cover the union");
+    const std::vector<int32_t> two(G, 2);
+    std::vector<int32_t> cut(G);
+    batch_attempt(cx, g, G, gvtx, two.data(), ubvec, o, seeds, d_part, cut.data(), nullptr, tpw2);
+}
 
 int32_t batch_attempts_per_call(int64_t n, int64_t m, int64_t tiles, int64_t kc_total, int32_t ngraphs, int ncuts,
                                 int max_pack, size_t reusable) {

@@ -16,6 +16,7 @@
 
 #include "../../include/rmetis.h"
 #include "driver.hpp"
+#include "graphio.hpp"
 #include "profiler.hpp"
 #include "serve.hpp"
 
@@ -53,6 +54,7 @@ Context* acquire_context() {
 }
 
 void release_context(Context* c) {
+    c->phase = nullptr;   // an observer's counter never outlives the call that set it
     std::lock_guard<std::mutex> lk(g_pool_mu);
     g_pool.push_back(c);
 }
@@ -226,6 +228,173 @@ int32_t RMETIS_PartGraphKway(const idx_t* nvtxs, const idx_t* ncon, const idx_t*
         RK_HIP(hipStreamSynchronize(cx->stream));
         std::copy(hp.begin(), hp.end(), part);
         if (objval) *objval = obj;
+        return METIS_OK;
+    });
+}
+
+int32_t RMETIS_PartGraphRecursive(const idx_t* nvtxs, const idx_t* ncon, const idx_t* xadj, const idx_t* adjncy,
+                                  const idx_t* vwgt, const idx_t* vsize, const idx_t* adjwgt, const idx_t* nparts,
+                                  const real_t* tpwgts, const real_t* ubvec, const idx_t* options, idx_t* objval,
+                                  idx_t* part) {
+    // c_api.rs:25-48 -> c_partition_impl(kway = false) -> partition_recursive
+    return guarded([&]() -> int32_t {
+        if (!nvtxs || !ncon || !xadj || !adjncy || !nparts || !part) return METIS_ERROR_INPUT;   // c_api.rs:157-161
+        const int32_t n = *nvtxs, nc = *ncon, np = *nparts;
+        if (n <= 0 || nc <= 0 || np < 2) return METIS_ERROR_INPUT;                            // c_api.rs:167
+        if (!validate_csr(n, nc, xadj, adjncy, vwgt)) return METIS_ERROR_INPUT;              // c_api.rs:186-187
+        if (np > n) return METIS_ERROR_INPUT;                                                 // recursive.rs:22-26
+        if (!totals_in_range(n, nc, vwgt)) return METIS_ERROR_INPUT;
+        const Options o = parse_options(options);
+        ContextGuard cx;
+        DevGraph g = upload_graph(*cx.c, n, nc, xadj, adjncy, vwgt, adjwgt, nullptr);
+        (void)vsize;  // the edge-cut objective never reads vsize
+        int32_t* d_part = cx->arena.get<int32_t>(n);
+        const int32_t obj = recursive_device(*cx.c, g, np, tpwgts, ubvec, o, d_part);
+        std::vector<int32_t> hp(n);
+        download(*cx.c, hp.data(), d_part, n);
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        std::copy(hp.begin(), hp.end(), part);
+        if (objval) *objval = obj;
+        return METIS_OK;
+    });
+}
+
+int32_t RMETIS_NodeND(const idx_t* nvtxs, const idx_t* xadj, const idx_t* adjncy, const idx_t* vwgt,
+                      const idx_t* options, idx_t* perm, idx_t* iperm) {
+    // c_api.rs:83-134 -> nd::node_nd (partition/nd.rs:16-44)
+    return guarded([&]() -> int32_t {
+        if (!nvtxs || !xadj || !adjncy || !perm || !iperm) return METIS_ERROR_INPUT;
+        const int32_t n = *nvtxs;
+        if (n <= 0) return METIS_ERROR_INPUT;
+        if (!validate_csr(n, 1, xadj, adjncy, vwgt)) return METIS_ERROR_INPUT;   // Graph::new(n, 1, ...)
+        if (!totals_in_range(n, 1, vwgt)) return METIS_ERROR_INPUT;
+        const Options o = parse_options(options);
+        ContextGuard cx;
+        DevGraph g = upload_graph(*cx.c, n, 1, xadj, adjncy, vwgt, nullptr, nullptr);
+        int32_t* d_perm = cx->arena.get<int32_t>(n);
+        int32_t* d_iperm = cx->arena.get<int32_t>(n);
+        node_nd_device(*cx.c, g, o, d_perm, d_iperm);
+        std::vector<int32_t> hp(n), hi(n);
+        download(*cx.c, hp.data(), d_perm, n);
+        download(*cx.c, hi.data(), d_iperm, n);
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        std::copy(hp.begin(), hp.end(), perm);
+        std::copy(hi.begin(), hi.end(), iperm);
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_recursive_device(idx_t nvtxs, idx_t ncon, const idx_t* d_xadj, const idx_t* d_adjncy,
+                                    const idx_t* d_vwgt, const idx_t* d_adjwgt, idx_t nparts, const real_t* tpwgts,
+                                    const real_t* ubvec, const idx_t* options, idx_t* objval, idx_t* d_part) {
+    return guarded([&]() -> int32_t {
+        if (!d_xadj || !d_adjncy || !d_part) return METIS_ERROR_INPUT;
+        if (nvtxs <= 0 || ncon <= 0 || nparts < 2) return METIS_ERROR_INPUT;
+        const Options o = parse_options(options);
+        ContextGuard cx;
+        DevGraph g;
+        g.n = nvtxs;
+        g.ncon = ncon;
+        RK_HIP(hipMemcpyAsync(&g.m, d_xadj + nvtxs, sizeof(int32_t), hipMemcpyDeviceToHost, cx->stream));
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        g.xadj = (int32_t*)d_xadj;
+        g.adjncy = (int32_t*)d_adjncy;
+        g.vwgt = (int32_t*)d_vwgt;
+        g.adjwgt = (int32_t*)d_adjwgt;
+        const int32_t obj = recursive_device(*cx.c, g, nparts, tpwgts, ubvec, o, d_part);
+        if (objval) *objval = obj;
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_hip_nodend_device(idx_t nvtxs, const idx_t* d_xadj, const idx_t* d_adjncy, const idx_t* d_vwgt,
+                                 const idx_t* options, idx_t* d_perm, idx_t* d_iperm) {
+    return guarded([&]() -> int32_t {
+        if (!d_xadj || !d_adjncy || !d_perm || !d_iperm || nvtxs <= 0) return METIS_ERROR_INPUT;
+        const Options o = parse_options(options);
+        ContextGuard cx;
+        DevGraph g;
+        g.n = nvtxs;
+        g.ncon = 1;
+        RK_HIP(hipMemcpyAsync(&g.m, d_xadj + nvtxs, sizeof(int32_t), hipMemcpyDeviceToHost, cx->stream));
+        RK_HIP(hipStreamSynchronize(cx->stream));
+        g.xadj = (int32_t*)d_xadj;
+        g.adjncy = (int32_t*)d_adjncy;
+        g.vwgt = (int32_t*)d_vwgt;
+        node_nd_device(*cx.c, g, o, d_perm, d_iperm);
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_read_graph(const char* path, idx_t* nvtxs, idx_t* ncon, idx_t** xadj, idx_t** adjncy, idx_t** vwgt,
+                          idx_t** vsize, idx_t** adjwgt) {
+    return guarded([&]() -> int32_t {
+        if (!path || !nvtxs || !ncon || !xadj || !adjncy) return METIS_ERROR_INPUT;
+        const GraphFile f = read_graph_file(path);
+        auto out = [](const std::vector<int32_t>& v, bool has) -> idx_t* {
+            if (!has) return nullptr;
+            idx_t* p = (idx_t*)std::malloc(sizeof(idx_t) * std::max<size_t>(v.size(), 1));
+            if (!p) throw std::bad_alloc();
+            std::copy(v.begin(), v.end(), p);
+            return p;
+        };
+        idx_t* x = out(f.xadj, true);
+        idx_t* a = out(f.adjncy, true);
+        idx_t* w = out(f.vwgt, f.has_vwgt);
+        idx_t* z = out(f.vsize, f.has_vsize);
+        idx_t* e = out(f.adjwgt, f.has_adjwgt);
+        *nvtxs = f.n;
+        *ncon = f.ncon;
+        *xadj = x;
+        *adjncy = a;
+        if (vwgt) *vwgt = w;
+        else std::free(w);
+        if (vsize) *vsize = z;
+        else std::free(z);
+        if (adjwgt) *adjwgt = e;
+        else std::free(e);
+        return METIS_OK;
+    });
+}
+
+int32_t rmetis_write_graph(const char* path, idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy,
+                           const idx_t* vwgt, const idx_t* vsize, const idx_t* adjwgt) {
+    return guarded([&]() -> int32_t {
+        if (!path) return METIS_ERROR_INPUT;
+        write_graph_file(path, nvtxs, ncon, xadj, adjncy, vwgt, vsize, adjwgt);
+        return METIS_OK;
+    });
+}
+
+void rmetis_free(void* p) { std::free(p); }
+
+int32_t rmetis_hip_fm2_refine(idx_t nvtxs, idx_t ncon, const idx_t* xadj, const idx_t* adjncy, const idx_t* vwgt,
+                              const idx_t* adjwgt, idx_t* part, const double* tpwgts, const double* ubvec, idx_t niter,
+                              idx_t* cut) {
+    // fm_refine, src/refine/fm.rs:12-29
+    return guarded([&]() -> int32_t {
+        if (!xadj || !adjncy || !part || !tpwgts || !ubvec || nvtxs <= 0 || ncon <= 0 || niter < 0)
+            return METIS_ERROR_INPUT;
+        if (!validate_csr(nvtxs, ncon, xadj, adjncy, vwgt) || !valid_part(nvtxs, part, 2) ||
+            !totals_in_range(nvtxs, ncon, vwgt))
+            return METIS_ERROR_INPUT;
+        ContextGuard cx;
+        hipStream_t s = cx->stream;
+        DevGraph g = upload_graph(*cx.c, nvtxs, ncon, xadj, adjncy, vwgt, adjwgt, nullptr);
+        int32_t* dp = upload(*cx.c, part, nvtxs);
+        const size_t kc = 2 * (size_t)ncon;
+        std::vector<long long> fm(kc), om(kc);
+        thresholds_from_targets(2, ncon, tpwgts, ubvec, fm.data(), om.data());
+        long long* dfm = upload(*cx.c, fm.data(), (int64_t)kc);
+        fm2_refine_device(*cx.c, g, dfm, niter, dp);
+        unsigned long long* acc = cx->arena.get<unsigned long long>(1);
+        int32_t* out = cx->arena.get<int32_t>(1);
+        edge_cut(g, dp, acc, out, pick_group(g.n, g.m), s);
+        download(*cx.c, part, dp, nvtxs);
+        int32_t c = 0;
+        download(*cx.c, &c, out, 1);
+        RK_HIP(hipStreamSynchronize(s));
+        if (cut) *cut = c;
         return METIS_OK;
     });
 }

@@ -117,6 +117,23 @@ int32_t kway_device(Context& cx, const DevGraph& g, int32_t k, const float* tpwg
 void kway_batch_device(Context& cx, const DevGraph& g, int32_t ngraphs, const int32_t* gvtx, const int32_t* nparts,
                        const float* ubvec, const Options& o, const uint64_t* seeds, int32_t* d_part, int32_t* objval,
                        KwayStats* st, int max_pack = 0);
+// One multilevel bisection (partition/recursive.rs:138-187) of every graph
+// of the union g: graph i (vertices gvtx[i]..gvtx[i+1], at least 3) into two
+// parts with the target fractions tpw2[2*ncon*i ...] and the seed seeds[i];
+// greedy or 2-way FM refinement per o.  d_part receives 0/1 per vertex.
+void bisect_batch_device(Context& cx, const DevGraph& g, int32_t ngraphs, const int32_t* gvtx, const float* tpw2,
+                         const float* ubvec, const Options& o, const uint64_t* seeds, int32_t* d_part);
+// fm_refine (refine/fm.rs:12-29) of one graph with two parts: d_fmax holds
+// floor(target * ub) per part and constraint; part is refined in place.
+void fm2_refine_device(Context& cx, const DevGraph& g, const long long* d_fmax, int niter, int32_t* part);
+// RMETIS_PartGraphRecursive (partition/recursive.rs:11-57) of g: part on
+// the device, the edge cut returned.
+int32_t recursive_device(Context& cx, const DevGraph& g, int32_t nparts, const float* tpwgts, const float* ubvec,
+                         const Options& o, int32_t* d_part);
+// RMETIS_NodeND (partition/nd.rs:16-44) of g (ncon 1): perm and iperm on
+// the device.
+void node_nd_device(Context& cx, const DevGraph& g, const Options& o, int32_t* d_perm, int32_t* d_iperm);
+
 // Attempts per pass of kway_batch_device for a batch of n vertices, m
 // entries, the given padded tile count and k*ncon total: bounded by the
 // int32 ranges of the union, 60 % of the free device memory (plus the

@@ -235,6 +235,92 @@ void fm_refine_level(const DevGraph& g, int32_t k, const long long* d_fmax, int 
                      const int32_t* heavy_list, RefineBufs& b, FmBufs& f, hipStream_t s, bool cleared,
                      bool pw_valid);
 
+// ---- fm2.hip ----
+// One graph of a batched level refined by 2-way FM: its id (index of its
+// per-graph tables), first vertex, size and the start of its tree region.
+struct Fm2Graph {
+    int32_t g = 0, v0 = 0, n = 0, pad = 0;
+    int64_t tbase = 0;
+};
+constexpr int kFm2MaxLvl = 12;  // tree levels, leaves included (8^11 > 2^31)
+// A batched level for fm2_refine_batch.
+struct Fm2Level {
+    const int32_t* gseg = nullptr;      // [n] graph of each vertex (device)
+    const int32_t* gdesc_of = nullptr;  // [gtotal] index into graphs of each graph id, -1 when absent (device)
+    const Fm2Graph* graphs = nullptr;   // [ngraphs] the graphs present (device)
+    int32_t ngraphs = 0;
+    int32_t gtotal = 0;                 // graph ids of the batch
+    int nlvl = 0;                       // tree levels of the largest graph present
+    const int64_t* nodes = nullptr;     // [kFm2MaxLvl][ngraphs + 1] prefix node counts per level (device)
+    int64_t level_nodes[kFm2MaxLvl] = {};   // total nodes per level (host)
+};
+// Work buffers of the 2-way FM passes (sized for the finest level; per-graph
+// arrays for the batch's graph ids).
+struct Fm2Bufs {
+    uint64_t* leaf = nullptr;     // [n] queue keys
+    uint64_t* tree = nullptr;     // [sum of fm2_tree_nodes] tree nodes above the leaves
+    uint8_t* state = nullptr;     // [n] side, locked and queued bits
+    int32_t* gain = nullptr;      // [n] gain of a move to the other side
+    int32_t* ext = nullptr;       // [n] row entries on the other side
+    const int32_t* sym = nullptr; // device flag from fm2_symmetry (of the call's input graph)
+    int32_t* moves = nullptr;     // [n]
+    int32_t* cut0 = nullptr;      // [gtotal] cut at a pass start
+    int32_t* cut1 = nullptr;      // [gtotal] cut after the pops
+    int32_t* best_len = nullptr;  // [gtotal]
+    int32_t* best_cut = nullptr;  // [gtotal]
+    int32_t* nmoves = nullptr;    // [gtotal]
+    int32_t* err = nullptr;       // pop budget exhausted (never expected)
+};
+// *flag (device) = 1 when the graph's rows are pair-symmetric (every entry
+// (v, u, w) matched by as many (u, v, w) in u's row), else 0.  Subgraphs and
+// coarse graphs of a pair-symmetric graph are pair-symmetric too.
+void fm2_symmetry(const DevGraph& g, int32_t* flag, hipStream_t s);
+// The queue tree of a graph of n vertices (8-ary, leaves at level 0): its
+// nodes above the leaves, its levels (leaves included) and its node count at
+// level l (0 beyond its top).
+int64_t fm2_tree_nodes(int32_t n);
+int32_t fm2_tree_levels(int32_t n);
+int64_t fm2_level_nodes(int32_t n, int l);
+// fm_refine (refine/fm.rs:12-29) on every graph of a batched level with two
+// parts: pw holds partition_weights(part) per graph at gkc[g], fmax the
+// floor(target * ub) thresholds; gdone[g] is set when a pass does not
+// improve (and must be clear on entry for the graphs to refine).  niter
+// passes, each a no-op for the graphs done.  cutacc: gtotal u64 scratch.
+void fm2_refine_batch(const DevGraph& g, const Fm2Level& lv, Fm2Bufs& f, const int32_t* gkc, const long long* fmax,
+                      int32_t* pw, int32_t* part, int32_t* gdone, int niter, unsigned long long* cutacc,
+                      hipStream_t s);
+
+// ---- split.hip ----
+// pre[0..n] = exclusive prefix counts of label == l (flag: n scratch ints).
+void label_prefix(int32_t n, const int32_t* label, int32_t l, int32_t* flag, int32_t* pre, void* scan_tmp,
+                  hipStream_t s);
+// counts[g*L + l] = vertices of graph g with label l (pre: device array of L
+// device pointers to the label prefixes; gv0: ngraphs + 1 first vertices).
+void split_counts(int32_t ngraphs, int32_t L, const int32_t* gv0, const int32_t* const* pre, int32_t* counts,
+                  hipStream_t s);
+// rank[v] = v's position among the vertices of its child (g, label[v]);
+// nid[v] = cbase[g*L + l] + rank[v] when cbase >= 0, else -1.
+void split_rank(int32_t n, int32_t L, const int32_t* label, const int32_t* gseg, const int32_t* gv0,
+                const int32_t* const* pre, const int32_t* cbase, int32_t* nid, int32_t* rank, hipStream_t s);
+// The child union (nn vertices): out.xadj/adjncy/adjwgt/vwgt (allocated by
+// the caller, adjwgt/vwgt null when g has none) and norig[nid[v]] = orig[v]
+// (orig null: the identity); ndeg: nn + 1 scratch ints.
+void split_graph(const DevGraph& g, const int32_t* orig, const int32_t* label, const int32_t* nid, int32_t nn,
+                 DevGraph& out, int32_t* norig, int32_t* ndeg, void* scan_tmp, hipStream_t s);
+// out[orig[v]] = pa[c] + min(rank[v], pcap[c]) for v of a child c = g*L +
+// label[v] with pa[c] >= 0 (orig null: the identity).
+void place_children(int32_t n, int32_t L, const int32_t* label, const int32_t* gseg, const int32_t* rank,
+                    const int32_t* orig, const int32_t* pa, const int32_t* pcap, int32_t* out, hipStream_t s);
+// label[v] = l for the vertices of the graphs g with flag[g] != 0.
+void relabel_graphs(int32_t n, const int32_t* gseg, const uint8_t* flag, int32_t l, int32_t* label, hipStream_t s);
+// iperm[perm[v]] = v.
+void inverse_permutation(int32_t n, const int32_t* perm, int32_t* iperm, hipStream_t s);
+// Vertex separator labels of bisected graphs (separator/mod.rs:29-58): 0 the
+// lighter side, 1 the other, 2 the lighter side's vertices with a neighbour
+// across; pw holds each graph's two side weights at 2*ncon*g.
+void separator_labels(const DevGraph& g, const int32_t* side, const int32_t* gseg, const int32_t* pw,
+                      int32_t* label, hipStream_t s);
+
 // ---- coarsen.hip ----
 struct CoarsenBufs {
     int32_t* match = nullptr;      // [n]

@@ -422,6 +422,7 @@ void Server::worker(int slot) {
         cv_done_.notify_all();
         cv_work_.notify_all();
     }
+    cx->phase = nullptr;   // the slot's counter dies with the server; the context lives on in the pool
     release_context(cx);
 }
 
