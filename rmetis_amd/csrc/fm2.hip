@@ -16,8 +16,8 @@
 // the level run side by side:
 //  * k_fm2_init (grid-wide): every vertex's gain and boundary flag, its
 //    queue key and its state byte (side, locked, queued);
-//  * k_fm2_tree (grid-wide, one launch per tree level): an 8-ary max tree of
-//    keys above each graph's leaves;
+//  * k_fm2_tree (grid-wide, one launch per tree level): a 64-ary max tree
+//    of keys above each graph's leaves;
 //  * k_fm2_pass: the workgroup copies the graph's queue and vertex state
 //    into LDS when they fit (else they stay in global memory), then its
 //    first wave runs the pops without block
@@ -42,8 +42,8 @@ namespace {
 constexpr int kF2Threads = 256;
 constexpr int kF2MaxCon = 16;      // constraints held in LDS by the pops
 constexpr int kF2G = 8;            // lanes per vertex in k_fm2_init
-constexpr int kF2Fan = 8;          // children per tree node
-constexpr int kF2FanLog = 3;
+constexpr int kF2Fan = 64;         // children per tree node (one wave)
+constexpr int kF2FanLog = 6;
 constexpr size_t kF2Lds = 156 * 1024;   // dynamic LDS of a pass: leaves, tree and state bytes
 
 constexpr uint8_t kSide = 1, kLocked = 2, kQueued = 4;
@@ -170,12 +170,37 @@ __global__ __launch_bounds__(kBlock) void k_fm2_tree(Fm2Args a, int l, const int
     }
 }
 
+// One max step of a 64-bit value with the lane that the DPP pattern CTRL
+// names, in the rows ROWS (lanes whose source is outside the row keep x).
+template <int CTRL, int ROWS>
+__device__ inline uint64_t dpp_max_step(uint64_t x) {
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    const uint32_t ylo = (uint32_t)__builtin_amdgcn_update_dpp((int)lo, (int)lo, CTRL, ROWS, 0xF, false);
+    const uint32_t yhi = (uint32_t)__builtin_amdgcn_update_dpp((int)hi, (int)hi, CTRL, ROWS, 0xF, false);
+    const uint64_t y = ((uint64_t)yhi << 32) | ylo;
+    return y > x ? y : x;
+}
+
+// 64-bit maximum over the wave (every lane active) by DPP row shifts and
+// row broadcasts; lane 63's result, as a wave-uniform value.
+__device__ inline uint64_t wave_max_dpp(uint64_t x) {
+    x = dpp_max_step<0x111, 0xF>(x);   // row_shr:1
+    x = dpp_max_step<0x112, 0xF>(x);   // row_shr:2
+    x = dpp_max_step<0x114, 0xF>(x);   // row_shr:4
+    x = dpp_max_step<0x118, 0xF>(x);   // row_shr:8: lane 15 of each row holds the row's maximum
+    x = dpp_max_step<0x142, 0xA>(x);   // row_bcast:15 into rows 1 and 3
+    x = dpp_max_step<0x143, 0xC>(x);   // row_bcast:31 into rows 2 and 3: lane 63 holds the maximum
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // The tree above the changed leaves x0 and x1 of each lane (-1: none), one
-// level per wave barrier.  Level l >= 1 sits at tree + s_off[l] with
-// s_len[l] nodes (LDS tables).  Per level the distinct parent nodes are
-// listed first (changed leaves are mostly neighbours, which share parents),
-// then eight lanes recompute each listed node from its eight children (one
-// contiguous read per node, a max over the eight lanes).
+// level per wave barrier; every lane of the wave takes part.  Level l >= 1
+// sits at tree + s_off[l] with s_len[l] nodes (LDS tables).  Per level each
+// distinct parent node of the changed entries (changed leaves are mostly
+// neighbours, which share parents) is recomputed once: lane c reads child
+// c, a DPP maximum over the wave, lane 0 writes.
 __device__ inline void f2_tree_update(int nlvl, const int64_t* s_len, const int64_t* s_off, uint64_t* leaves,
                                       uint64_t* tree, int32_t x0, int32_t x1) {
     const int lane = __lane_id();
@@ -183,35 +208,20 @@ __device__ inline void f2_tree_update(int nlvl, const int64_t* s_len, const int6
         const int sh = kF2FanLog * l;
         const int32_t na = x0 >= 0 ? (int32_t)(x0 >> sh) : -1;
         const int32_t nb = x1 >= 0 ? (int32_t)(x1 >> sh) : -1;
-        // the distinct nodes, lane d holding the d-th
+        const int64_t clen = s_len[l - 1];
+        const uint64_t* src = l == 1 ? leaves : tree + s_off[l - 1];
+        uint64_t* dst = tree + s_off[l];
         bool ra = na >= 0, rb = nb >= 0;
-        int32_t mine = -1;
-        int cnt = 0;
         for (;;) {
             const uint64_t left = __ballot(ra || rb);
             if (left == 0) break;
             const int f = __builtin_ctzll(left);
-            const int32_t nd = __shfl(ra ? na : nb, f, 64);
-            if (lane == cnt) mine = nd;
+            const int32_t nd = __builtin_amdgcn_readlane(ra ? na : nb, f);
             ra = ra && na != nd;
             rb = rb && nb != nd;
-            ++cnt;
-        }
-        const int64_t clen = s_len[l - 1];
-        const uint64_t* src = l == 1 ? leaves : tree + s_off[l - 1];
-        uint64_t* dst = tree + s_off[l];
-        const int c = lane & (kF2Fan - 1);
-        for (int b = 0; b < cnt; b += 64 / kF2Fan) {
-            const int d = b + lane / kF2Fan;
-            const int32_t nd = __shfl(mine, d < 64 ? d : 0, 64);
-            const int64_t child = (int64_t)nd * kF2Fan + c;
-            uint64_t m = (d < cnt && child < clen) ? src[child] : 0ull;
-#pragma unroll
-            for (int o = kF2Fan / 2; o > 0; o >>= 1) {
-                const uint64_t y = __shfl_xor(m, o, kF2Fan);
-                m = y > m ? y : m;
-            }
-            if (d < cnt && c == 0) dst[nd] = m;
+            const int64_t child = (int64_t)nd * kF2Fan + lane;
+            const uint64_t m = wave_max_dpp(child < clen ? src[child] : 0ull);
+            if (lane == 0) dst[nd] = m;
         }
         wave_fence_sync();
     }
@@ -227,6 +237,7 @@ template <bool LDS>
 __global__ __launch_bounds__(kF2Threads) void k_fm2_pass(Fm2Args a) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ int32_t s_pw[2 * kF2MaxCon];
+    __shared__ long long s_fm[2 * kF2MaxCon];
     __shared__ int64_t s_len[kFm2MaxLvl], s_off[kFm2MaxLvl];
     const Fm2Graph gd = a.graphs[blockIdx.x];
     if (a.done[gd.g]) return;
@@ -278,7 +289,10 @@ __global__ __launch_bounds__(kF2Threads) void k_fm2_pass(Fm2Args a) {
         exts = a.ext + v0;
         leaves = a.leaf + v0;
     }
-    for (int i = tid; i < 2 * ncon; i += kF2Threads) s_pw[i] = pwg[i];
+    for (int i = tid; i < 2 * ncon; i += kF2Threads) {
+        s_pw[i] = pwg[i];
+        s_fm[i] = fm[i];
+    }
     {
         // Read the graph's rows once with the whole workgroup, so that the
         // pops' row reads (one wave, one vertex after another) find them in
@@ -311,13 +325,16 @@ __global__ __launch_bounds__(kF2Threads) void k_fm2_pass(Fm2Args a) {
         const int32_t i = 0x7FFFFFFF - (int32_t)(uint32_t)key;
         const int32_t gain = (int32_t)((uint32_t)(key >> 32) ^ 0x80000000u);
         const int32_t v = v0 + i;
+        // v's row bounds are read together with its weights, before the
+        // move is decided (they are needed only when it moves)
+        const int32_t beg = a.xadj[v], deg = a.xadj[v + 1] - beg;
         const uint8_t sv = state[i];
         const int32_t from = sv & kSide, to = from ^ 1;
         int32_t vw = 0;
         bool fits = true;
         if (lane < ncon) {                                                    // fm.rs:74-81
             vw = a.vwgt ? a.vwgt[(int64_t)v * ncon + lane] : 1;
-            fits = (long long)wadd(s_pw[to * ncon + lane], vw) <= fm[to * ncon + lane];
+            fits = (long long)wadd(s_pw[to * ncon + lane], vw) <= s_fm[to * ncon + lane];
         }
         const bool feas = __ballot(!fits) == 0;
         wave_fence_sync();
@@ -349,7 +366,6 @@ __global__ __launch_bounds__(kF2Threads) void k_fm2_pass(Fm2Args a) {
             // the unlocked neighbours' gains and boundary counts (fm.rs:99-118)
             // from v's row: an entry to v's old side turns external (+2w),
             // one to its new side internal (-2w)
-            const int32_t beg = a.xadj[v], deg = a.xadj[v + 1] - beg;
             for (int32_t t0 = 0; t0 < deg; t0 += 64) {
                 int32_t c1 = -1;
                 const int32_t t = t0 + lane;
@@ -381,7 +397,6 @@ __global__ __launch_bounds__(kF2Threads) void k_fm2_pass(Fm2Args a) {
             }
         } else if (feas) {
             // the unlocked neighbours' gains from their own rows (fm.rs:99-118)
-            const int32_t beg = a.xadj[v], deg = a.xadj[v + 1] - beg;
             for (int32_t t0 = 0; t0 < deg; t0 += 64) {
                 int32_t c1 = -1;
                 const int32_t t = t0 + lane;

@@ -275,7 +275,7 @@ struct Fm2Bufs {
 // (v, u, w) matched by as many (u, v, w) in u's row), else 0.  Subgraphs and
 // coarse graphs of a pair-symmetric graph are pair-symmetric too.
 void fm2_symmetry(const DevGraph& g, int32_t* flag, hipStream_t s);
-// The queue tree of a graph of n vertices (8-ary, leaves at level 0): its
+// The queue tree of a graph of n vertices (64-ary, leaves at level 0): its
 // nodes above the leaves, its levels (leaves included) and its node count at
 // level l (0 beyond its top).
 int64_t fm2_tree_nodes(int32_t n);
