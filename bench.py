@@ -19,7 +19,8 @@ pool over the host cores) and, as an extra key, the single-graph workload
 (`rgg20`: 2^20-point random geometric graph, k = 64).
 
 Other workloads: --workload rgg20 | grid256 | rmat22 | rggNN | rmatNN | batchN
-| serve.
+| serve | siblings (recursive bisection and nested dissection, also an extra
+key of the default run).
 """
 import argparse
 import ctypes as C
@@ -195,7 +196,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batch1024",
                     help="batch1024 (default: uneven batch of that many graphs) | rgg20 | grid256 | rmat22 | "
-                         "rggNN | rmatNN | serve")
+                         "rggNN | rmatNN | serve | siblings")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="serve: length of the arrival stream")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-pool", type=int, default=192, help="serve: distinct graphs the requests draw from")
@@ -245,6 +246,8 @@ def main():
         out = bench_batch(args)
     elif args.workload == "serve":
         out = bench_serve(args)
+    elif args.workload == "siblings":
+        out = bench_siblings(args)
     else:
         out = bench_single(args, args.workload)
     print(json.dumps(out), flush=True)
@@ -487,7 +490,96 @@ def bench_batch(args):
         log(f"cpu baseline: {out['cpu_baseline']['value']} graphs/s on {out['cpu_baseline']['cores']} threads")
     if not args.no_extra:
         out["rgg20"] = bench_single(args, "rgg20", extra=True)
+        out["siblings"] = bench_siblings(args, extra=True)
     return out
+
+
+def bench_siblings(args, extra=False):
+    """The C-ABI siblings on graphs resident in HBM: recursive bisection
+    (rmetis_hip_recursive_device, rgg 2^17, k = 64) and nested dissection
+    (rmetis_hip_nodend_device, 256 x 256 grid), options as RMETIS_* with the
+    default array except seed (SHEM, grow, FM, ufactor 30), against the
+    oracle's restatement of the reference on the same graphs and options."""
+    import numpy as np
+    import rmetis_amd as R
+    from rmetis_amd import _lib, synth
+
+    L = _lib.lib()
+    DA = _lib.DeviceArray
+    steps = max(2, args.steps if not extra else 3)
+    opts = R.Options(seed=args.seed).to_c_array()
+    out = {}
+    rb_g, rb_k = synth.rgg(1 << 17, args.seed, vwgt_max=4), 64
+    nd_g = synth.grid(256, 256)
+    for name, g in (("recursive", rb_g), ("nodend", nd_g)):
+        d_x, d_a = DA(g.xadj), DA(g.adjncy)
+        d_vw = DA(g.vwgt) if g.vwgt is not None else None
+        d_aw = DA(g.adjwgt) if g.adjwgt is not None else None
+        d_o1, d_o2 = DA(count=g.n, dtype=np.int32), DA(count=g.n, dtype=np.int32)
+        obj = C.c_int32()
+
+        def step():
+            if name == "recursive":
+                rc = L.rmetis_hip_recursive_device(g.n, g.ncon, d_x.ptr, d_a.ptr,
+                                                   d_vw.ptr if d_vw is not None else None,
+                                                   d_aw.ptr if d_aw is not None else None, rb_k, None, None,
+                                                   opts.ctypes.data_as(_lib.i32p), C.byref(obj), d_o1.ptr)
+            else:
+                rc = L.rmetis_hip_nodend_device(g.n, d_x.ptr, d_a.ptr, d_vw.ptr if d_vw is not None else None,
+                                                opts.ctypes.data_as(_lib.i32p), d_o1.ptr, d_o2.ptr)
+            if rc != 1:
+                raise RuntimeError(f"{name} failed rc={rc}: {_lib.last_error()}")
+
+        step()
+        L.rmetis_hip_device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        L.rmetis_hip_device_synchronize()
+        el = time.perf_counter() - t0
+        e = {"value": round(steps / el, 3), "unit": "graphs/s", "ms_per_call": round(1e3 * el / steps, 2),
+             "steps": steps, "graph": f"{g.name} n={g.n} m={g.m}"}
+        if name == "recursive":
+            e["k"] = rb_k
+            e["edge_cut"] = int(obj.value)
+        else:
+            perm = d_o1.numpy()
+            e["perm_is_permutation"] = bool(np.array_equal(np.sort(perm), np.arange(g.n)))
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle as O
+            O.build()
+            t1, calls, cuts = 0.0, 0, []
+            while calls < 4 and (calls == 0 or t1 < (4.0 if extra else args.cpu_budget_s)):
+                t = time.perf_counter()
+                if name == "recursive":
+                    _, cut, _ = O.part_graph_recursive(g.xadj, g.adjncy, rb_k, vwgt=g.vwgt, adjwgt=g.adjwgt,
+                                                    ncon=g.ncon, iptype=0, rtype_greedy=False,
+                                                    seed=args.seed + calls, ufactor=30)
+                    cuts.append(int(cut))
+                else:
+                    O.node_nd(g.xadj, g.adjncy, vwgt=g.vwgt, seed=args.seed + calls)
+                t1 += time.perf_counter() - t
+                calls += 1
+            e["cpu_baseline"] = {"value": round(calls / t1, 4), "unit": "graphs/s", "cores": 1, "kind": "port",
+                                 "sample": f"{calls} calls of the oracle on the same graph, seeds "
+                                           f"{args.seed}..{args.seed + calls - 1}"}
+            if cuts:
+                e["oracle_edge_cut_geomean"] = round(float(np.exp(np.mean(np.log(cuts)))), 1)
+        for a in (d_x, d_a, d_vw, d_aw, d_o1, d_o2):
+            if a is not None:
+                a.free()
+        out[name] = e
+        log(f"{name}: {e['value']} graphs/s on the GPU ({e['ms_per_call']} ms per call)")
+    if extra:
+        return out
+    return {"metric": "graphs/s (recursive bisection and nested dissection calls)",
+            "value": out["recursive"]["value"], "unit": "graphs/s", "n_gpus": 1, "steps": steps,
+            "warmup": 1, "ms_per_step": out["recursive"]["ms_per_call"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+            "config": {"workload": "siblings: recursive bisection of rgg 2^17 (k = 64), nested dissection of "
+                                   "grid 256 x 256", "seed": args.seed},
+            **out}
 
 
 def bench_serve(args):
