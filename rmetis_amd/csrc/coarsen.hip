@@ -189,8 +189,12 @@ struct Chain {
 
 // One step of a group's chain (see k_suitor): the searcher's scan and claim.
 // CAPPED applies the weight cap (cap non-null); the kernel inlines both so
-// that graphs whose cap cannot bind run the uncapped code.
-template <int G, bool CAPPED>
+// that graphs whose cap cannot bind run the uncapped code.  Each lane takes
+// B neighbours per round (G * B per group): their ids, then their weights,
+// then their suitor words are all loaded before the first compare, at
+// indices clamped into the row (the current vertex has edges), so that one
+// round costs two dependent memory round trips whatever B is.
+template <int G, int B, bool CAPPED>
 __device__ __forceinline__ void suitor_step(const MatchArgs& a, int gl, uint64_t seed, int32_t base,
                                             const long long* cap, Chain& ch) {
     const int32_t cur = ch.cur;
@@ -213,36 +217,48 @@ __device__ __forceinline__ void suitor_step(const MatchArgs& a, int gl, uint64_t
         bw = 0;
         br = 0;
         bseen = 0;
-        for (int32_t j = beg + gl; j < end; j += G) {
-            const int32_t u = a.adjncy[j];
-            if (CAPPED && filt && !pair_fits(a, cur, u, cap)) continue;   // the pair would be too heavy
-            const int32_t w = a.adjwgt ? a.adjwgt[j] : 1;
-            uint64_t sv = fresh ? ld_agent64(&a.suitor[u]) : a.suitor[u];
-            const uint32_t ru = owner_rank(sv, u, a.xadj);
-            if (u == fu && sv != fword) {
-                // keep the newer of the loaded word and the CAS result
-                const int32_t s1 = suitor_id(sv);
-                const int32_t s2 = suitor_id(fword);
-                if (s1 < 0) sv = fword;
-                else if (s2 >= 0) {
-                    const EdgeKey k1 = suitor_key(sv, s1, u, ru, a.xadj, seed, a.shem, base);
-                    const EdgeKey k2 = suitor_key(fword, s2, u, ru, a.xadj, seed, a.shem, base);
-                    if (edge_before(k2, s2 - base, u - base, k1, s1 - base, u - base)) sv = fword;
+        for (int32_t j0 = beg + gl; j0 < end; j0 += G * B) {
+            int32_t uu[B], ww[B];
+            uint64_t ss[B];
+#pragma unroll
+            for (int i = 0; i < B; ++i) uu[i] = a.adjncy[min(j0 + i * G, end - 1)];
+#pragma unroll
+            for (int i = 0; i < B; ++i) ww[i] = a.adjwgt ? a.adjwgt[min(j0 + i * G, end - 1)] : 1;
+#pragma unroll
+            for (int i = 0; i < B; ++i) ss[i] = fresh ? ld_agent64(&a.suitor[uu[i]]) : a.suitor[uu[i]];
+#pragma unroll
+            for (int i = 0; i < B; ++i) {
+                if (j0 + i * G >= end) break;
+                const int32_t u = uu[i];
+                if (CAPPED && filt && !pair_fits(a, cur, u, cap)) continue;   // the pair would be too heavy
+                const int32_t w = ww[i];
+                uint64_t sv = ss[i];
+                const uint32_t ru = owner_rank(sv, u, a.xadj);
+                if (u == fu && sv != fword) {
+                    // keep the newer of the loaded word and the CAS result
+                    const int32_t s1 = suitor_id(sv);
+                    const int32_t s2 = suitor_id(fword);
+                    if (s1 < 0) sv = fword;
+                    else if (s2 >= 0) {
+                        const EdgeKey k1 = suitor_key(sv, s1, u, ru, a.xadj, seed, a.shem, base);
+                        const EdgeKey k2 = suitor_key(fword, s2, u, ru, a.xadj, seed, a.shem, base);
+                        if (edge_before(k2, s2 - base, u - base, k1, s1 - base, u - base)) sv = fword;
+                    }
                 }
-            }
-            const int32_t sid = suitor_id(sv);
-            if (sid == cur) continue;
-            const EdgeKey k = edge_key_r(cur - base, u - base, w, min(rc, ru), seed, a.shem);
-            if (sid >= 0) {
-                const EdgeKey ks = suitor_key(sv, sid, u, ru, a.xadj, seed, a.shem, base);
-                if (!edge_before(k, cur - base, u - base, ks, sid - base, u - base)) continue;
-            }
-            if (bu < 0 || edge_before(k, cur - base, u - base, best, cur - base, bu - base)) {
-                best = k;
-                bu = u;
-                bw = w;
-                br = ru;
-                bseen = sv;
+                const int32_t sid = suitor_id(sv);
+                if (sid == cur) continue;
+                const EdgeKey k = edge_key_r(cur - base, u - base, w, min(rc, ru), seed, a.shem);
+                if (sid >= 0) {
+                    const EdgeKey ks = suitor_key(sv, sid, u, ru, a.xadj, seed, a.shem, base);
+                    if (!edge_before(k, cur - base, u - base, ks, sid - base, u - base)) continue;
+                }
+                if (bu < 0 || edge_before(k, cur - base, u - base, best, cur - base, bu - base)) {
+                    best = k;
+                    bu = u;
+                    bw = w;
+                    br = ru;
+                    bseen = sv;
+                }
             }
         }
 #pragma unroll
@@ -299,7 +315,8 @@ __device__ __forceinline__ void suitor_step(const MatchArgs& a, int gl, uint64_t
     }
 }
 
-// G lanes per searching vertex.  A vertex v of the level starts a chain: the
+// G lanes per searching vertex (a group), B neighbours per lane and scan
+// round, at least W waves per SIMD.  A vertex v of the level starts a chain: the
 // current searcher scans its neighbours for the best one (edge order) whose
 // suitor it beats, and claims it with a 64-bit CAS on the suitor word it
 // compared against; on success the displaced suitor (if any) becomes the
@@ -316,8 +333,8 @@ __device__ __forceinline__ void suitor_step(const MatchArgs& a, int gl, uint64_t
 // do not wait for the longest chain of the wave, while the waves still walk
 // the level together round by round (neighbouring vertices' words stay in
 // the caches).
-template <int G, bool CAPPED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_suitor(MatchArgs a) {
+template <int G, int B, bool CAPPED, int W = 1>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) void k_suitor(MatchArgs a) {
     constexpr int GPW = 64 / G;
     const int gl = group_lane<G>();
     const int lane = __lane_id();
@@ -348,8 +365,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
             const int64_t v = w0 + (t % GPW) + (t / GPW) * S;
             const bool inr = ask && v < a.n;
             const int64_t vv = inr ? v : 0;
-            bool take = inr && a.xadj[vv + 1] != a.xadj[vv];
-            const int32_t gi = a.gseg && take ? a.gseg[vv] : 0;
+            // the row bounds and the graph are loaded together
+            const int32_t x0 = a.xadj[vv], x1 = a.xadj[vv + 1];
+            const int32_t gi = a.gseg ? a.gseg[vv] : 0;
+            bool take = inr && x1 != x0;
             if (a.gseg) take = take && a.gon[gi] != 0;
             if (ask) more = inr;
             if (a.gseg) {
@@ -368,8 +387,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
             ch.fresh = take ? false : ch.fresh;
         }
         if (ch.cur < 0) break;                                 // only when no slot is left
-        if (CAPPED && cap) suitor_step<G, true>(a, gl, seed, base, cap, ch);
-        else suitor_step<G, false>(a, gl, seed, base, nullptr, ch);
+        if (CAPPED && cap) suitor_step<G, B, true>(a, gl, seed, base, cap, ch);
+        else suitor_step<G, B, false>(a, gl, seed, base, nullptr, ch);
     }
 }
 
@@ -761,6 +780,35 @@ inline int grid_cap(int64_t items, int per_block, int cap) {
 
 }  // namespace
 
+// Blocks of kBlock threads of `kern` resident on one CU (the occupancy
+// query, cached per slot): persistent grids are sized to exactly what runs
+// at once, since their waves own fixed shares of the work.
+int resident_blocks(int slot, void (*kern)(MatchArgs)) {
+    static std::atomic<int> cache[64] = {};
+    int nb = cache[slot].load(std::memory_order_relaxed);
+    if (nb <= 0) {
+        RK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kBlock, 0));
+        nb = nb > 0 ? nb : 1;
+        cache[slot].store(nb, std::memory_order_relaxed);
+    }
+    return nb;
+}
+
+// The suitor kernel's shape for a level by its mean degree: G lanes per
+// chain and B neighbours per lane and round.  Fewer lanes per chain keep
+// more chains in flight (a step is a few dependent memory round trips, so
+// the level's time is its steps over the chains in flight); G * B covers a
+// mean row in one or two rounds.  Measured on the finest levels of the
+// 1,024-graph batch (mean degree 8.2: 8 lanes 50.6 ms, 4 lanes 53 ms, (1, 4)
+// 29.3 ms, (2, 4) 33 ms, (1, 8) 32 ms) and of rgg20 (mean degree 12: 8 lanes
+// 295 us, (2, 4) 233 us, (1, 4) 306 us).
+int suitor_variant(int32_t n, int32_t m) {
+    const double avg = n > 0 ? (double)m / (double)n : 0.0;
+    if (avg <= 10.0) return 0;
+    if (avg <= 24.0) return 1;
+    return 2;
+}
+
 void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, CoarsenBufs& b, hipStream_t s,
                  const GraphSegs* gs, const long long* cap) {
     if (g.n <= 0) return;
@@ -769,18 +817,21 @@ void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, CoarsenBufs& 
                       gs ? gs->gseed : nullptr, gs ? gs->gon : nullptr};
     const int grid = grid_cap(g.n, kBlock, 4096);
     hipLaunchKernelGGL(k_suitor_init, dim3(grid), dim3(kBlock), 0, s, g.n, g.xadj, b.suitor);
-    const int G = pick_group(g.n, g.m) > 8 ? 8 : pick_group(g.n, g.m);   // short scans: chains dominate
-    // persistent grid of exactly the resident blocks: the vertices are dealt
-    // to the groups in one pass, with no blocks waiting for a second round
+    const int var = suitor_variant(g.n, g.m);
     const bool capped = a.cap != nullptr;   // the capped variant only where a pair can reach the cap
-    if (G == 4) {
-        const int sgrid = grid_cap(g.n, kBlock / 4, device_cus() * 8);
-        if (capped) hipLaunchKernelGGL((k_suitor<4, true>), dim3(sgrid), dim3(kBlock), 0, s, a);
-        else hipLaunchKernelGGL((k_suitor<4, false>), dim3(sgrid), dim3(kBlock), 0, s, a);
-    } else {
-        const int sgrid = grid_cap(g.n, kBlock / 8, device_cus() * 8);
-        if (capped) hipLaunchKernelGGL((k_suitor<8, true>), dim3(sgrid), dim3(kBlock), 0, s, a);
-        else hipLaunchKernelGGL((k_suitor<8, false>), dim3(sgrid), dim3(kBlock), 0, s, a);
+    switch (var) {
+#define RK_SUITOR(IDX, G_, B_, W_)                                                                            \
+    case IDX: {                                                                                           \
+        const auto kern = capped ? k_suitor<G_, B_, true, W_> : k_suitor<G_, B_, false, W_>;              \
+        const int sgrid = grid_cap(g.n, kBlock / G_, device_cus() * resident_blocks(2 * IDX + capped, kern)); \
+        hipLaunchKernelGGL(kern, dim3(sgrid), dim3(kBlock), 0, s, a);                                     \
+        break;                                                                                            \
+    }
+        RK_SUITOR(0, 1, 4, 1)
+        RK_SUITOR(1, 2, 4, 1)
+        RK_SUITOR(2, 8, 1, 8)
+#undef RK_SUITOR
+        default: break;
     }
     hipLaunchKernelGGL(k_suitor_pairs, dim3(grid), dim3(kBlock), 0, s, g.n, b.suitor, b.match);
     RK_HIP(hipGetLastError());
