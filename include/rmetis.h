@@ -110,6 +110,27 @@ int32_t rmetis_write_graph(const char* path, idx_t nvtxs, idx_t ncon, const idx_
                            const idx_t* vwgt, const idx_t* vsize, const idx_t* adjwgt);
 void rmetis_free(void* p);
 
+/* The tuning table: launch shapes and thresholds measured by the autotuner
+ * (rmetis_amd/autotune.py) and read on first use from tuning.json beside
+ * the library (or the file named by the environment variable
+ * RMETIS_HIP_TUNING); compiled-in defaults otherwise.  No reference
+ * counterpart: every entry is exact (results never depend on it).
+ * values[0..24) suitor matching shape per level bucket, size-major: size
+ *               bucket (n in [0,2^19), [2^19,2^21), [2^21,2^23), [2^23,inf))
+ *               times 6 plus mean-degree bucket (m/n in [0,6), [6,10),
+ *               [10,16), [16,24), [24,48), [48,inf)); shapes 0 = 1 lane x 4
+ *               neighbours per round, 1 = 2 x 4, 2 = 4 x 2, 3 = 8 x 1;
+ * values[24]    single-graph levels of at least this many vertices run
+ *               incremental cut passes;
+ * values[25]    admission speculates on levels with at least this many
+ *               vertices per part.
+ * get returns the number of entries written (at most 26); set installs the
+ * first count entries for the whole process, or returns METIS_ERROR_INPUT
+ * for an out-of-range value. */
+int32_t rmetis_hip_get_tuning(int32_t* values, int32_t count);
+int32_t rmetis_hip_set_tuning(const int32_t* values, int32_t count);
+const char* rmetis_hip_tuning_source(void);
+
 /* Per-level figures of the last attempt of a k-way call (finest first; for
  * a batched call, of its last pass over the union of the attempts it ran
  * together). */

@@ -87,6 +87,9 @@ SIGNATURES = {
     "rmetis_hip_level_thresholds": (C.c_int32, [C.c_int32, C.c_int32, f32p, i32p, f32p, C.c_int32,
                                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "rmetis_hip_debug_flags": (None, [C.c_int32]),
+    "rmetis_hip_get_tuning": (C.c_int32, [i32p, C.c_int32]),
+    "rmetis_hip_set_tuning": (C.c_int32, [i32p, C.c_int32]),
+    "rmetis_hip_tuning_source": (C.c_char_p, []),
     "rmetis_hip_device_alloc": (C.c_void_p, [C.c_int64]),
     "rmetis_hip_device_free": (None, [C.c_void_p]),
     "rmetis_hip_memcpy": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32]),

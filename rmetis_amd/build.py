@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librmetis_hip.so")
 SOURCES = ["scan.hip", "reduce.hip", "refine.hip", "admit.hip", "coarsen.hip", "fm.hip", "fm2.hip", "grow.hip",
-           "split.hip", "driver.cpp", "batch.cpp", "recursive.cpp", "graphio.cpp", "serve.cpp", "capi.cpp"]
+           "split.hip", "tuning.cpp", "driver.cpp", "batch.cpp", "recursive.cpp", "graphio.cpp", "serve.cpp",
+           "capi.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -31,6 +31,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "tuning.hpp"
 
 namespace rk {
 
@@ -399,7 +400,6 @@ RK_INL void resolve_chunk(const Ctx& x, S& st, int s0, int cnt, int first = 0) {
 }
 
 constexpr int kSpecRounds = 8;   // speculative rounds per chunk; the walk finishes the rest
-constexpr int kSpecMinPerPart = 24;   // vertices per part below which a level is not speculated
 constexpr int kSpecMinRun = 4;   // shortest run of slots between barriers worth a speculative call
 
 // Net weight moved into a part by the lanes of masks mi (in) and mo (out):
@@ -603,7 +603,7 @@ RK_INL int spec_chunk(const Ctx& x, uint64_t* m_in, uint64_t* m_out, int s0, int
 // registers (k <= 64*KP, ncon == NC <= 2).  NC > 0: cut-optimisation chunks
 // are resolved speculatively where spec_chunk applies.
 template <bool L, int KP, int NC>
-RK_INL void admit_body(const AdmitArgs& a, const int mode) {
+RK_INL void admit_body(const AdmitArgs& a, const int mode, const int32_t spec_min) {
     static_assert(KP == 0 || L, "register slack needs the LDS tables");
     using Slack = std::conditional_t<(KP > 0), RegSlack<(KP > 0 ? KP : 1), NC>, TableSlack<L>>;
     extern __shared__ int32_t s_dyn[];
@@ -747,10 +747,10 @@ RK_INL void admit_body(const AdmitArgs& a, const int mode) {
         __syncthreads();
         // a speculative call that runs out of rounds (many moves conflict)
         // turns speculation off for the rest of the window.  Levels with
-        // fewer than kSpecMinPerPart vertices per part (the coarsest, whose
+        // fewer than spec_min vertices per part (the tuning table; the coarsest, whose
         // vertices weigh a large share of a part's slack) conflict so often
         // that the walk alone is faster there; both are exact.
-        bool spec_on = mode == 1 && (int64_t)a.ntiles * 64 >= (int64_t)kSpecMinPerPart * a.k;
+        bool spec_on = mode == 1 && (int64_t)a.ntiles * 64 >= (int64_t)spec_min * a.k;
         if ((tid >> 6) == 0)
             for (int s0 = 0; s0 < wn; s0 += 64) {
                 const int cnt = min(64, wn - s0);
@@ -812,7 +812,7 @@ template <bool L, int KP, int NC>
 __global__ __launch_bounds__(kAT) void k_admit(AdmitArgs arg, const AdmitArgs* __restrict__ table) {
     AdmitArgs a = arg;
     if (table) a = table[blockIdx.x];
-    admit_body<L, KP, NC>(a, arg.mode);
+    admit_body<L, KP, NC>(a, arg.mode, arg.spec_min);
 }
 
 }  // namespace
@@ -821,7 +821,9 @@ namespace {
 
 // Launches k_admit for graphs of k*ncon = kc at most: one workgroup with the
 // arguments aa, or one per graph of a batched level with arguments from table.
-void launch_admit(const AdmitArgs& aa, const AdmitArgs* table, int grid, int32_t k, int32_t ncon, hipStream_t s) {
+void launch_admit(const AdmitArgs& aa0, const AdmitArgs* table, int grid, int32_t k, int32_t ncon, hipStream_t s) {
+    AdmitArgs aa = aa0;
+    aa.spec_min = tune(kTuneSpecMin);
     const int kc = k * ncon;
     const size_t win = (size_t)kWinInts * sizeof(int32_t);
     const size_t cntb = 2 * (size_t)kCntWin * sizeof(int32_t);
@@ -854,7 +856,7 @@ void launch_admit(const AdmitArgs& aa, const AdmitArgs* table, int grid, int32_t
 void admit_pass(const DevGraph& g, int32_t k, int mode, int ntiles, RefineBufs& b, int32_t* part, hipStream_t s) {
     const AdmitArgs aa{mode, k, g.ncon, ntiles, b.tile_cnt, b.tile_bits, b.tile_off, b.tile_rec, b.mov, b.pool,
                        g.xadj, g.vwgt, b.fmax, b.omax, b.pw, part, b.moves, b.nmoves, b.done, b.passes,
-                       b.adm_scratch, g.adjncy, nullptr, b.full, 0};
+                       b.adm_scratch, g.adjncy, nullptr, b.full, 0, 0};
     launch_admit(aa, nullptr, 1, k, g.ncon, s);
 }
 

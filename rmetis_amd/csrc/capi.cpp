@@ -17,6 +17,7 @@
 #include "../../include/rmetis.h"
 #include "driver.hpp"
 #include "graphio.hpp"
+#include "tuning.hpp"
 #include "profiler.hpp"
 #include "serve.hpp"
 
@@ -1028,6 +1029,24 @@ int32_t rmetis_hip_level_thresholds(idx_t nparts, idx_t ncon, const real_t* tpwg
         return METIS_OK;
     });
 }
+
+int32_t rmetis_hip_get_tuning(int32_t* values, int32_t count) {
+    if (!values || count <= 0) return 0;
+    const int c = count < kTuneCount ? count : kTuneCount;
+    for (int i = 0; i < c; ++i) values[i] = tune(i);
+    return c;
+}
+
+int32_t rmetis_hip_set_tuning(const int32_t* values, int32_t count) {
+    const char* why = "";
+    if (!set_tuning(values, count, &why)) {
+        g_last_error = why;
+        return METIS_ERROR_INPUT;
+    }
+    return METIS_OK;
+}
+
+const char* rmetis_hip_tuning_source(void) { return tuning_source(); }
 
 void rmetis_hip_debug_flags(int32_t flags) { set_debug_flags(flags); }
 

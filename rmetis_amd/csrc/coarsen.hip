@@ -24,6 +24,7 @@
 //    first position and weight sum (k_contract_big).
 #include "common.hpp"
 #include "kernels.hpp"
+#include "tuning.hpp"
 
 namespace rk {
 
@@ -749,6 +750,12 @@ __global__ __launch_bounds__(kBlock) void k_contract_big(const int32_t* __restri
     if (__ballot(not_one) && __lane_id() == 0) atomicOr(&counters[4], 1);
 }
 
+// The merged lists from the degree-sum layout of tmp to the coarse CSR.  A
+// wave takes 64 consecutive coarse vertices, whose coarse ranges are one
+// contiguous span: lane q of a round copies span position q, found by a
+// binary search over the 64 range starts in LDS (the last vertex starting
+// at or before q owns it), so that the stores of a round are one coalesced
+// run whatever the degrees.
 __global__ __launch_bounds__(kBlock) void k_contract_copy(int32_t n_max, const int32_t* __restrict__ cn_dev,
                                                           const int32_t* __restrict__ tmp_off,
                                                           const int32_t* __restrict__ tmp_adj,
@@ -756,18 +763,35 @@ __global__ __launch_bounds__(kBlock) void k_contract_copy(int32_t n_max, const i
                                                           const int32_t* __restrict__ cxadj,
                                                           int32_t* __restrict__ cadj, int32_t* __restrict__ cw,
                                                           int32_t* __restrict__ counters) {
-    constexpr int G = 16, NG = kBlock / G;
-    const int gl = group_lane<G>();
+    constexpr int WPB = kBlock / 64;
+    __shared__ int32_t s_dst_all[WPB][64], s_src_all[WPB][64];
+    const int wv = threadIdx.x >> 6, lane = __lane_id();
+    int32_t* s_dst = s_dst_all[wv];
+    int32_t* s_src = s_src_all[wv];
     const int32_t cn = *cn_dev;
     if (blockIdx.x == 0 && threadIdx.x == 0) counters[1] = cxadj[n_max];   // cm, read with the other counters
-    for (int64_t cv = (int64_t)blockIdx.x * NG + threadIdx.x / G; cv < cn; cv += (int64_t)gridDim.x * NG) {
-        const int32_t src = tmp_off[cv];
-        const int32_t dst = cxadj[cv];
-        const int32_t cnt = cxadj[cv + 1] - dst;
-        for (int32_t i = gl; i < cnt; i += G) {
-            cadj[dst + i] = tmp_adj[src + i];
-            cw[dst + i] = tmp_w[src + i];
+    const int64_t ntiles = ((int64_t)cn + 63) / 64;
+    for (int64_t t = (int64_t)blockIdx.x * WPB + wv; t < ntiles; t += (int64_t)gridDim.x * WPB) {
+        const int64_t v0 = t * 64;
+        const int last = (int)min<int64_t>(64, cn - v0);            // vertices of this tile
+        const int64_t cv = v0 + (lane < last ? lane : last - 1);
+        const int32_t d0 = cxadj[v0];
+        const int32_t span = cxadj[v0 + last] - d0;
+        s_dst[lane] = cxadj[cv] - d0;
+        s_src[lane] = tmp_off[cv];
+        wave_sync();
+        for (int32_t q = lane; q < span; q += 64) {
+            int lo = 0, hi = last - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_dst[mid] <= q) lo = mid;
+                else hi = mid - 1;
+            }
+            const int32_t from = s_src[lo] + (q - s_dst[lo]);
+            cadj[d0 + q] = tmp_adj[from];
+            cw[d0 + q] = tmp_w[from];
         }
+        wave_sync();
     }
 }
 
@@ -794,21 +818,12 @@ int resident_blocks(int slot, void (*kern)(MatchArgs)) {
     return nb;
 }
 
-// The suitor kernel's shape for a level by its mean degree: G lanes per
-// chain and B neighbours per lane and round.  Fewer lanes per chain keep
-// more chains in flight (a step is a few dependent memory round trips, so
-// the level's time is its steps over the chains in flight); G * B covers a
-// mean row in one or two rounds.  Measured on the finest levels of the
-// 1,024-graph batch (mean degree 8.2: 8 lanes 50.6 ms, 4 lanes 53 ms, (1, 4)
-// 29.3 ms, (2, 4) 33 ms, (1, 8) 32 ms) and of rgg20 (mean degree 12: 8 lanes
-// 295 us, (2, 4) 233 us, (1, 4) 306 us).
-int suitor_variant(int32_t n, int32_t m) {
-    const double avg = n > 0 ? (double)m / (double)n : 0.0;
-    if (avg <= 10.0) return 0;
-    if (avg <= 24.0) return 1;
-    return 2;
-}
-
+// The suitor kernel's shape for a level: G lanes per chain and B neighbours
+// per lane and round, chosen per level size and mean degree by the tuning
+// table.
+// Fewer lanes per chain keep more chains in flight (a step is a few
+// dependent memory round trips and a CAS, so a level's time is its steps
+// over the chains in flight); G * B covers a mean row in one or two rounds.
 void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, CoarsenBufs& b, hipStream_t s,
                  const GraphSegs* gs, const long long* cap) {
     if (g.n <= 0) return;
@@ -817,7 +832,7 @@ void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, CoarsenBufs& 
                       gs ? gs->gseed : nullptr, gs ? gs->gon : nullptr};
     const int grid = grid_cap(g.n, kBlock, 4096);
     hipLaunchKernelGGL(k_suitor_init, dim3(grid), dim3(kBlock), 0, s, g.n, g.xadj, b.suitor);
-    const int var = suitor_variant(g.n, g.m);
+    const int var = tune(level_bucket(g.n, g.m));
     const bool capped = a.cap != nullptr;   // the capped variant only where a pair can reach the cap
     switch (var) {
 #define RK_SUITOR(IDX, G_, B_, W_)                                                                            \
@@ -829,7 +844,8 @@ void match_graph(const DevGraph& g, int ctype_shem, uint64_t seed, CoarsenBufs& 
     }
         RK_SUITOR(0, 1, 4, 1)
         RK_SUITOR(1, 2, 4, 1)
-        RK_SUITOR(2, 8, 1, 8)
+        RK_SUITOR(2, 4, 2, 1)
+        RK_SUITOR(3, 8, 1, 8)
 #undef RK_SUITOR
         default: break;
     }
@@ -870,7 +886,7 @@ ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, 
                        b.members, b.degsum, b.tmp_off, b.tmp_adj, b.tmp_w, b.cdeg, b.big_list, b.counters,
                        b.hash_keys, b.hash_first, b.hash_sum);
     exclusive_scan_i32(b.cdeg, cg.xadj, n, cg.xadj + n, b.scan_tmp, s);   // cxadj[cn..n] = cm
-    hipLaunchKernelGGL(k_contract_copy, dim3(grid_cap(n, kBlock / 16, 65536)), dim3(kBlock), 0, s, n, cn_dev,
+    hipLaunchKernelGGL(k_contract_copy, dim3(grid_cap(((int64_t)n + 63) / 64, kBlock / 64, 65536)), dim3(kBlock), 0, s, n, cn_dev,
                        b.tmp_off, b.tmp_adj, b.tmp_w, cg.xadj, cg.adjncy, cg.adjwgt, b.counters);
     if (heavy_list) heavy_list_build(n, cn_dev, cg.xadj, heavy_list, b.counters + 8, s, true);
     // one read of counters[0..8] (cn, cm, ..., the weight flag, nheavy)

@@ -6,6 +6,7 @@
 // finer level after projecting (kway.rs:73-92).  Keeps the strictly best cut
 // over the ncuts attempts (kway.rs:96-98).
 #include "driver.hpp"
+#include "tuning.hpp"
 
 #include "profiler.hpp"
 
@@ -260,7 +261,7 @@ void refine_level(Context& cx, const DevGraph& g, int32_t k, const long long* d_
     refine_pass(g, k, 0, nheavy, heavy_list, B.rb, d_part, s);                       // greedy.rs:60-123
     // cut rounds (126-190); on large levels, after the first, only the tiles
     // around the last round's moves are recomputed
-    const bool inc = g.n >= kIncrementalMinN;
+    const bool inc = g.n >= tune(kTuneIncMinN);
     for (int it = 0; it < niter; ++it)
         refine_pass(g, k, 1, nheavy, heavy_list, B.rb, d_part, s, inc && it > 0, inc && it + 1 < niter);
     if (st) {

@@ -146,6 +146,7 @@ struct AdmitArgs {
     uint32_t* tile_dirty;
     int32_t* full;
     int32_t v0;          // first vertex of the graph (tiles count from it)
+    int32_t spec_min;    // speculate on levels with at least this many vertices per part (set at launch)
 };
 
 // Heavy-vertex list (degree > kGainHeavy) of a level with n = *n_dev (or n_max
@@ -163,9 +164,6 @@ int heavy_list_sync(const DevGraph& g, int32_t* list, RefineBufs& b, hipStream_t
 // for an incremental next pass.
 void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32_t* heavy_list, RefineBufs& b,
                  int32_t* part, hipStream_t s, bool incremental = false, bool mark_next = false);
-// Levels from this size up run their cut passes after the first
-// incrementally; below it a full gain launch costs less than marking.
-constexpr int32_t kIncrementalMinN = 1 << 18;
 // ---- admit.hip ----
 // Ordered admission of the candidates that the gain kernels left in
 // b.tile_rec / b.tile_cnt (one workgroup; updates b.pw, part, b.moves).

@@ -8,7 +8,7 @@ use std::process::Command;
 // The library's translation units, in the order rmetis_amd/build.py uses.
 const SOURCES: &[&str] = &[
     "scan.hip", "reduce.hip", "refine.hip", "admit.hip", "coarsen.hip", "fm.hip", "fm2.hip", "grow.hip",
-    "split.hip", "driver.cpp", "batch.cpp", "recursive.cpp", "graphio.cpp", "serve.cpp", "capi.cpp",
+    "split.hip", "tuning.cpp", "driver.cpp", "batch.cpp", "recursive.cpp", "graphio.cpp", "serve.cpp", "capi.cpp",
 ];
 
 fn main() {
