@@ -133,10 +133,14 @@ def roofline(tag, ln, ms, by, workload):
     traffic, src = None, None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(tpath):
+        # HBM bytes per launch of each kernel family at the finest level, from
+        # two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the same
+        # counters on a device copy of known size
         with open(tpath) as fh:
             tj = json.load(fh)
-        if tj.get("tag") == tag:
-            traffic, src = tj.get("traffic_bytes_per_launch"), tj.get("summary")
+        fam = tj.get(tag)
+        if isinstance(fam, dict) and fam.get("traffic_bytes_per_launch"):
+            traffic, src = fam["traffic_bytes_per_launch"], tj.get("summary")
     return {"bound": "hbm", "kernel": TAG_KERNELS.get(tag, tag), "tag": tag, "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": src, "bytes_per_launch": int(per), "avg_launch_us": round(avg_ms * 1e3, 2),

@@ -328,8 +328,9 @@ __device__ __forceinline__ void suitor_step(const MatchArgs& a, int gl, uint64_t
 // ends.
 //
 // Each loop iteration is one step of every group's chain.  A wave owns the
-// vertex slots w*GPW + i + r*S (i < GPW groups per wave, rounds r, S groups
-// in the grid), dealt in that order: a group whose chain ended takes the
+// vertex slots lo + w*GPW + i + r*S (i < GPW groups per wave, rounds r; w
+// the wave's index and S the groups among the waves walking the part of the
+// level from lo, below), dealt in that order: a group whose chain ended takes the
 // wave's next slot at once (a ballot ranks the groups asking), so the groups
 // do not wait for the longest chain of the wave, while the waves still walk
 // the level together round by round (neighbouring vertices' words stay in
@@ -339,8 +340,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) voi
     constexpr int GPW = 64 / G;
     const int gl = group_lane<G>();
     const int lane = __lane_id();
-    const int64_t S = (int64_t)gridDim.x * (kBlock / G);
-    const int64_t w0 = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / 64 * GPW;
+    // Blocks are dealt to the XCDs round-robin; when the grid is a multiple
+    // of their count, each XCD's waves walk their own contiguous eighth of
+    // the level (neighbours' rows and words, mostly in the same eighth, then
+    // meet in that XCD's L2), else all waves walk the whole level.
+    int64_t S, w0, hi = a.n;
+    if (gridDim.x % kXcds == 0) {
+        const int64_t reg = ((int64_t)a.n + kXcds - 1) / kXcds;
+        const int64_t lo = min((int64_t)(blockIdx.x % kXcds) * reg, (int64_t)a.n);
+        hi = min(lo + reg, (int64_t)a.n);
+        S = (int64_t)(gridDim.x / kXcds) * (kBlock / G);
+        w0 = lo + ((int64_t)(blockIdx.x / kXcds) * kBlock + threadIdx.x) / 64 * GPW;
+    } else {
+        S = (int64_t)gridDim.x * (kBlock / G);
+        w0 = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / 64 * GPW;
+    }
     int64_t slot = 0;             // wave-uniform: the wave's next slot
     Chain ch;
     bool more = true;             // group-uniform: slots may remain for the group
@@ -359,12 +373,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W))) voi
             const int rank = __popcll(askers & ((1ull << lane) - 1ull));   // valid on the group's lane 0
             const int64_t t = slot + __shfl(rank, lane & ~(G - 1), 64);
             slot += __popcll(askers);
-            // The slot's vertex, taken unless it is past the level (every
-            // later slot is past it too), has no edges, or belongs to a graph
+            // The slot's vertex, taken unless it is past the wave's part of
+            // the level (every later slot is past it too), has no edges, or belongs to a graph
             // not matched here.  Every value is written on every asking lane
             // (selects, no divergent assignments).
             const int64_t v = w0 + (t % GPW) + (t / GPW) * S;
-            const bool inr = ask && v < a.n;
+            const bool inr = ask && v < hi;
             const int64_t vv = inr ? v : 0;
             // the row bounds and the graph are loaded together
             const int32_t x0 = a.xadj[vv], x1 = a.xadj[vv + 1];
@@ -458,8 +472,6 @@ __device__ inline int32_t seq_edge(const int32_t* xadj, int32_t a, int32_t b, in
 
 constexpr int kCWaves = 4;               // waves (independent tiles) per contraction block
 constexpr int kCCap = 1024;              // merged-sequence positions a wave stages per batch
-constexpr int kCItems = kCCap / 64;      // staged positions per lane per batch
-constexpr int kContractBlocksPerCU = 3;  // resident contraction blocks per CU (LDS- and VGPR-limited)
 constexpr int kCSlots = 16;              // unique coarse neighbours a lane merges in registers
 
 __device__ inline int wave_excl_scan_c(int x, int* total) {
@@ -484,7 +496,8 @@ __device__ inline int wave_excl_scan_c(int x, int* total) {
 // gathers into LDS.  Each lane then merges its coarse vertex in LDS:
 // internal entries dropped, equal coarse neighbours summed onto their first
 // occurrence, and the survivors written in sequence order.
-__global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* __restrict__ cn_dev,
+template <int CAP, int W = 1>
+__global__ __launch_bounds__(64 * kCWaves) __attribute__((amdgpu_waves_per_eu(W))) void k_contract_tile(const int32_t* __restrict__ cn_dev,
                                                                const int32_t* __restrict__ xadj,
                                                                const int32_t* __restrict__ adjncy,
                                                                const int32_t* __restrict__ adjwgt,
@@ -500,9 +513,9 @@ __global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* _
     // one slot of padding per lane spreads the lanes' ranges over the LDS
     // banks (degree sums with a common factor would otherwise put every lane
     // of the merge loops on a few banks)
-    __shared__ int32_t s_cu_all[kCWaves][kCCap + 64];
-    __shared__ int32_t s_w_all[kCWaves][kCCap + 64];
-    __shared__ uint8_t s_own_all[kCWaves][kCCap];
+    __shared__ int32_t s_cu_all[kCWaves][CAP + 64];
+    __shared__ int32_t s_w_all[kCWaves][CAP + 64];
+    __shared__ uint8_t s_own_all[kCWaves][CAP];
     __shared__ int32_t s_loff_all[kCWaves][65], s_xa_all[kCWaves][64], s_xb_all[kCWaves][64], s_da_all[kCWaves][64];
     const int wv = threadIdx.x >> 6, lane = __lane_id();
     int32_t* s_cu = s_cu_all[wv];
@@ -544,15 +557,15 @@ __global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* _
         if (lane == 63) s_loff[64] = total;
         for (int vs = 0; vs < 64;) {
             const int P0 = __shfl(loff, vs, 64);
-            const uint64_t fit = __ballot(lane >= vs && loff + sds - P0 <= kCCap);
+            const uint64_t fit = __ballot(lane >= vs && loff + sds - P0 <= CAP);
             const int ve = 64 - __builtin_clzll(fit);               // fit is lanes [vs, ve), ve > vs
             const int P1 = __shfl(loff + sds, ve - 1, 64);
             if (lane >= vs && lane < ve)
                 for (int e = 0; e < sds; ++e) s_own[loff - P0 + e] = (uint8_t)lane;
             wave_sync();
-            int32_t uu[kCItems], ww[kCItems], oo[kCItems];
+            int32_t uu[(CAP / 64)], ww[(CAP / 64)], oo[(CAP / 64)];
 #pragma unroll
-            for (int i = 0; i < kCItems; ++i) {
+            for (int i = 0; i < (CAP / 64); ++i) {
                 const int p = P0 + lane + i * 64;
                 uu[i] = 0;
                 ww[i] = 1;
@@ -568,11 +581,11 @@ __global__ __launch_bounds__(64 * kCWaves) void k_contract_tile(const int32_t* _
             }
             // all gathers in flight before the first LDS write (items past
             // the batch read cmap[0])
-            int32_t cc[kCItems];
+            int32_t cc[(CAP / 64)];
 #pragma unroll
-            for (int i = 0; i < kCItems; ++i) cc[i] = cmap[uu[i]];
+            for (int i = 0; i < (CAP / 64); ++i) cc[i] = cmap[uu[i]];
 #pragma unroll
-            for (int i = 0; i < kCItems; ++i) {
+            for (int i = 0; i < (CAP / 64); ++i) {
                 const int p = P0 + lane + i * 64;
                 if (p < P1) {
                     s_cu[p - P0 + oo[i]] = cc[i];
@@ -877,8 +890,19 @@ ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, 
     exclusive_scan_i32(b.degsum, b.tmp_off, n, nullptr, b.scan_tmp, s);
     {
         const int blocks = (n + 64 * kCWaves - 1) / (64 * kCWaves);
-        const int resident = device_cus() * kContractBlocksPerCU;
-        hipLaunchKernelGGL(k_contract_tile, dim3(blocks < resident ? blocks : resident), dim3(64 * kCWaves), 0, s,
+        // LDS (4 waves x ~10.7 KB) and 115 VGPRs allow 3 blocks per CU; a
+        // 512-entry stage (4 blocks) was 20 % slower on C4's levels, 256
+        // entries with 8 waves per SIMD (spilling) 50 % slower
+        const auto kern = k_contract_tile<kCCap>;
+        static std::atomic<int> occ{0};
+        int nb = occ.load(std::memory_order_relaxed);
+        if (nb <= 0) {
+            RK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 64 * kCWaves, 0));
+            nb = nb > 0 ? nb : 1;
+            occ.store(nb, std::memory_order_relaxed);
+        }
+        const int resident = device_cus() * nb;
+        hipLaunchKernelGGL(kern, dim3(blocks < resident ? blocks : resident), dim3(64 * kCWaves), 0, s,
                            cn_dev, g.xadj, g.adjncy, g.adjwgt, cmap, b.members, b.degsum, b.tmp_off, b.tmp_adj,
                            b.tmp_w, b.cdeg, b.big_list, b.counters);
     }
