@@ -14,8 +14,10 @@ namespace rk {
 
 namespace {
 
-constexpr int kScanItems = 8;
-constexpr int kScanChunk = kBlock * kScanItems;   // 2048 elements per block
+constexpr int kScanItems = 16;
+constexpr int kScanChunk = kBlock * kScanItems;   // 4096 elements per block
+constexpr int kChunkThreads = 1024;               // the chunk-sum scan: one block ...
+constexpr int kChunkItems = 16;                   // ... of 16 consecutive sums per thread and round
 
 constexpr int kOneThreads = 1024;
 constexpr int kOneRows = 16;
@@ -57,41 +59,74 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const int32_t* __restric
     if (threadIdx.x == 0) chunk_sums[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_chunks(long long* __restrict__ chunk_sums, int nchunks,
-                                                        int32_t* __restrict__ total_out) {
-    __shared__ long long s_w[kBlock / 64];
+// The chunk sums in place to their exclusive prefix: rounds of
+// kChunkThreads * kChunkItems sums, each thread scanning kChunkItems
+// consecutive ones between one block scan of the threads' totals.
+__global__ __launch_bounds__(kChunkThreads) void k_scan_chunks(long long* __restrict__ chunk_sums, int nchunks,
+                                                               int32_t* __restrict__ total_out) {
+    __shared__ long long s_w[kChunkThreads / 64];
     long long carry = 0;
-    for (int b = 0; b < nchunks; b += kBlock) {
-        int idx = b + threadIdx.x;
-        long long x = idx < nchunks ? chunk_sums[idx] : 0;
+    for (int b = 0; b < nchunks; b += kChunkThreads * kChunkItems) {
+        const int i0 = b + threadIdx.x * kChunkItems;
+        long long x[kChunkItems];
+        long long sum = 0;
+#pragma unroll
+        for (int i = 0; i < kChunkItems; ++i) {
+            x[i] = i0 + i < nchunks ? chunk_sums[i0 + i] : 0;
+            sum += x[i];
+        }
         long long tot;
-        long long ex = block_excl_scan_ll(x, s_w, &tot);
-        if (idx < nchunks) chunk_sums[idx] = carry + ex;
+        long long run = carry + block_excl_scan_ll<kChunkThreads>(sum, s_w, &tot);
+#pragma unroll
+        for (int i = 0; i < kChunkItems; ++i) {
+            if (i0 + i < nchunks) chunk_sums[i0 + i] = run;
+            run += x[i];
+        }
         carry += tot;
     }
     if (threadIdx.x == 0 && total_out) *total_out = (int32_t)carry;
 }
 
+// Each thread rescans kScanItems consecutive elements of its block's chunk;
+// VEC: in and out are 16-byte aligned, and a thread's whole run inside the
+// input moves as 16-byte vectors.
+template <bool VEC>
 __global__ __launch_bounds__(kBlock) void k_scan_down(const int32_t* __restrict__ in, int64_t n,
                                                       const long long* __restrict__ chunk_off,
                                                       int32_t* __restrict__ out) {
     __shared__ long long s_w[kBlock / 64];
     const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanItems;
+    const bool whole = VEC && base + kScanItems <= n;
     int32_t v[kScanItems];
+    if (whole) {
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            const int4 t = *reinterpret_cast<const int4*>(in + base + 4 * q);
+            v[4 * q] = t.x, v[4 * q + 1] = t.y, v[4 * q + 2] = t.z, v[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) v[i] = base + i < n ? in[base + i] : 0;
+    }
     long long sum = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        int64_t idx = base + i;
-        v[i] = idx < n ? in[idx] : 0;
-        sum += v[i];
-    }
+    for (int i = 0; i < kScanItems; ++i) sum += v[i];
     long long tot;
     long long run = block_excl_scan_ll(sum, s_w, &tot) + chunk_off[blockIdx.x];
+    int32_t o[kScanItems];
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
-        int64_t idx = base + i;
-        if (idx < n) out[idx] = (int32_t)run;
+        o[i] = (int32_t)run;
         run += v[i];
+    }
+    if (whole) {
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q)
+            *reinterpret_cast<int4*>(out + base + 4 * q) = make_int4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i)
+            if (base + i < n) out[base + i] = o[i];
     }
 }
 
@@ -163,8 +198,11 @@ void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* tot
     int nchunks = (int)((n + kScanChunk - 1) / kScanChunk);
     long long* sums = (long long*)temp;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nchunks), dim3(kBlock), 0, s, in, n, sums);
-    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(kBlock), 0, s, sums, nchunks, total_dev);
-    hipLaunchKernelGGL(k_scan_down, dim3(nchunks), dim3(kBlock), 0, s, in, n, sums, out);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(kChunkThreads), 0, s, sums, nchunks, total_dev);
+    if (((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0)
+        hipLaunchKernelGGL(k_scan_down<true>, dim3(nchunks), dim3(kBlock), 0, s, in, n, sums, out);
+    else
+        hipLaunchKernelGGL(k_scan_down<false>, dim3(nchunks), dim3(kBlock), 0, s, in, n, sums, out);
     RK_HIP(hipGetLastError());
 }
 
