@@ -85,6 +85,7 @@ Fm2Level fm2_level(Context& cx, const int32_t* gseg, const std::vector<Extent>& 
     std::vector<int32_t> of(G, -1);
     int64_t tb = 0;
     int nlvl = 1;
+    int32_t max_n = 0;
     for (int32_t i = 0; i < G; ++i) {
         if (ext[i].n <= 0) continue;
         Fm2Graph d;
@@ -96,8 +97,10 @@ Fm2Level fm2_level(Context& cx, const int32_t* gseg, const std::vector<Extent>& 
         gd.push_back(d);
         tb += fm2_tree_nodes(ext[i].n);
         nlvl = std::max(nlvl, (int)fm2_tree_levels(ext[i].n));
+        max_n = std::max(max_n, ext[i].n);
     }
     Fm2Level lv;
+    lv.max_n = max_n;
     lv.ngraphs = (int32_t)gd.size();
     lv.gtotal = G;
     lv.nlvl = nlvl;

@@ -19,7 +19,8 @@
 //  * k_fm2_tree (grid-wide, one launch per tree level): a 64-ary max tree
 //    of keys above each graph's leaves;
 //  * k_fm2_pass: the workgroup copies the graph's queue and vertex state
-//    into LDS when they fit (else they stay in global memory), then its
+//    into LDS when they fit (else the tree and state bytes alone, else
+//    nothing), then its
 //    first wave runs the pops without block
 //    barriers: every lane reads the root, lane 0 decides the move, the lanes
 //    recompute the moved vertex's unlocked neighbours from their rows (one
@@ -227,13 +228,23 @@ __device__ inline void f2_tree_update(int nlvl, const int64_t* s_len, const int6
     }
 }
 
-// The pops of one pass (fm.rs:65-119) for graph blockIdx.x.  LDS: the
-// graph's queue and vertex state (tree, state bytes, gains and boundary
-// counts with pair-symmetric rows, leaves) are copied into LDS, for graphs
-// where they all fit; otherwise they stay in global memory (the other
-// instantiation; each launch skips the graphs of the other).  Separate
-// instantiations keep every access of the LDS form an LDS instruction.
-template <bool LDS>
+// The pops of one pass (fm.rs:65-119) for graph blockIdx.x.  MODE says
+// where the graph's queue and vertex state live during the pops: 2 = all in
+// LDS (tree, state bytes, gains and boundary counts with pair-symmetric
+// rows, leaves), for graphs where they all fit; 1 = the tree above the
+// leaves and the state bytes in LDS, leaves, gains and counts in global
+// memory, for graphs where only those fit; 0 = all in global memory.  Each
+// launch takes the graphs of its own mode and skips the others; separate
+// instantiations keep every access of an LDS array an LDS instruction.
+// Whether a graph of n vertices has everything of a pass in LDS even with
+// gains and counts (pair-symmetric rows): then only the first pass launch
+// can have work.
+inline bool fm2_fits_lds(int32_t n) {
+    const size_t tree_b = (size_t)fm2_tree_nodes(n) * 8, state_b = ((size_t)n + 15) & ~(size_t)15;
+    return tree_b + state_b + (size_t)n * 16 <= kF2Lds;
+}
+
+template <int MODE>
 __global__ __launch_bounds__(kF2Threads) void k_fm2_pass(Fm2Args a) {
     extern __shared__ __align__(16) unsigned char s_dyn[];
     __shared__ int32_t s_pw[2 * kF2MaxCon];
@@ -254,7 +265,8 @@ __global__ __launch_bounds__(kF2Threads) void k_fm2_pass(Fm2Args a) {
     const int64_t nodes = lvl_off(n, nlvl);       // the tree region's size
     const size_t tree_b = (size_t)nodes * 8, state_b = ((size_t)n + 15) & ~(size_t)15;
     const size_t ge_b = sym ? (size_t)n * 8 : 0, leaf_b = (size_t)n * 8;
-    if ((tree_b + state_b + ge_b + leaf_b <= kF2Lds) != LDS) return;
+    const int mode = tree_b + state_b + ge_b + leaf_b <= kF2Lds ? 2 : (tree_b + state_b <= kF2Lds ? 1 : 0);
+    if (mode != MODE) return;
     int32_t* pwg = a.pw + a.gkc[gd.g];
     const long long* fm = a.fmax + a.gkc[gd.g];
     if (tid < nlvl) {
@@ -266,7 +278,7 @@ __global__ __launch_bounds__(kF2Threads) void k_fm2_pass(Fm2Args a) {
     int32_t* gains;
     int32_t* exts;
     uint64_t* leaves;
-    if constexpr (LDS) {
+    if constexpr (MODE == 2) {
         tree = (uint64_t*)s_dyn;
         state = s_dyn + tree_b;
         gains = (int32_t*)(s_dyn + tree_b + state_b);
@@ -282,6 +294,14 @@ __global__ __launch_bounds__(kF2Threads) void k_fm2_pass(Fm2Args a) {
                 gains[i] = a.gain[v0 + i];
                 exts[i] = a.ext[v0 + i];
             }
+    } else if constexpr (MODE == 1) {
+        tree = (uint64_t*)s_dyn;
+        state = s_dyn + tree_b;
+        gains = a.gain + v0;
+        exts = a.ext + v0;
+        leaves = a.leaf + v0;
+        for (int64_t i = tid; i < nodes; i += kF2Threads) tree[i] = a.tree[gd.tbase + i];
+        for (int32_t i = tid; i < n; i += kF2Threads) state[i] = a.state[v0 + i];
     } else {
         tree = a.tree + gd.tbase;
         state = a.state + v0;
@@ -593,16 +613,22 @@ void fm2_refine_batch(const DevGraph& g, const Fm2Level& lv, Fm2Bufs& f, const i
     a.done = gdone;
     a.err = f.err;
     static const hipError_t lds_ok =
-        hipFuncSetAttribute((const void*)k_fm2_pass<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kF2Lds);
+        hipFuncSetAttribute((const void*)k_fm2_pass<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kF2Lds);
+    static const hipError_t lds_ok1 =
+        hipFuncSetAttribute((const void*)k_fm2_pass<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kF2Lds);
     RK_HIP(lds_ok);
+    RK_HIP(lds_ok1);
     for (int it = 0; it < niter; ++it) {                                  // fm.rs:22-26
         hipLaunchKernelGGL(k_fm2_init, dim3(grid_of(g.n, kBlock / kF2G)), dim3(kBlock), 0, s, a, g.n);
         for (int l = 1; l < lv.nlvl; ++l)
             hipLaunchKernelGGL(k_fm2_tree, dim3(grid_of(lv.level_nodes[l], kBlock)), dim3(kBlock), 0, s, a, l,
                                lv.nodes + (int64_t)l * (lv.ngraphs + 1));
         edge_cut_batch(g, part, lv.gseg, lv.gtotal, cutacc, f.cut0, s);   // fm.rs:60
-        hipLaunchKernelGGL(k_fm2_pass<true>, dim3(lv.ngraphs), dim3(kF2Threads), kF2Lds, s, a);
-        hipLaunchKernelGGL(k_fm2_pass<false>, dim3(lv.ngraphs), dim3(kF2Threads), 0, s, a);
+        hipLaunchKernelGGL(k_fm2_pass<2>, dim3(lv.ngraphs), dim3(kF2Threads), kF2Lds, s, a);
+        if (!fm2_fits_lds(lv.max_n)) {
+            hipLaunchKernelGGL(k_fm2_pass<1>, dim3(lv.ngraphs), dim3(kF2Threads), kF2Lds, s, a);
+            hipLaunchKernelGGL(k_fm2_pass<0>, dim3(lv.ngraphs), dim3(kF2Threads), 0, s, a);
+        }
         edge_cut_batch(g, part, lv.gseg, lv.gtotal, cutacc, f.cut1, s);   // fm.rs:122
         hipLaunchKernelGGL(k_fm2_undo, dim3(lv.ngraphs), dim3(kF2Threads), 0, s, a);
     }

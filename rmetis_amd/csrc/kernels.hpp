@@ -249,6 +249,7 @@ struct Fm2Level {
     int32_t ngraphs = 0;
     int32_t gtotal = 0;                 // graph ids of the batch
     int nlvl = 0;                       // tree levels of the largest graph present
+    int32_t max_n = 0;                  // vertices of the largest graph present
     const int64_t* nodes = nullptr;     // [kFm2MaxLvl][ngraphs + 1] prefix node counts per level (device)
     int64_t level_nodes[kFm2MaxLvl] = {};   // total nodes per level (host)
 };
