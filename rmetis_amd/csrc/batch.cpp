@@ -494,7 +494,17 @@ void batch_attempt(Context& cx, const DevGraph& g, int32_t G, const int32_t* gvt
                 upload_thresholds([&](int32_t i, int32_t* t) {
                     for (int c = 0; c < ncon; ++c) t[c] = L.ext[i].n > 0 ? L.ext[i].n : 1;
                 });
-            part_weights_batch(L.g, pf, L.gseg, d_gkc, d_gk, kc_total, B.rb.pw, s);   // greedy.rs:50
+            // greedy.rs:50.  With explicit vertex weights a graph projected from
+            // the coarser level keeps that level's final table (coarse weights
+            // are sums of fine ones, so the projected partition's table is
+            // the same); only the graphs grown at this level are summed.
+            if (o.rtype_greedy && !unit && l + 1 < (int64_t)nl) {
+                std::vector<uint8_t> mask(G, 0);
+                for (int32_t i = 0; i < G; ++i) mask[i] = lvl_of[i] == l ? 1 : 0;
+                part_weights_batch(L.g, pf, L.gseg, d_gkc, d_gk, kc_total, B.rb.pw, s, put(cx, mask), G);
+            } else {
+                part_weights_batch(L.g, pf, L.gseg, d_gkc, d_gk, kc_total, B.rb.pw, s);
+            }
             RK_HIP(hipMemsetAsync(d_gdone, 0, sizeof(int32_t) * (size_t)G, s));
             if (o.rtype_greedy) {
                 refine_pass_batch(L.g, bl, 0, L.nheavy, L.heavy, B.rb, pf, s);              // greedy.rs:60-123

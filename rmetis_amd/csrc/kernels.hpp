@@ -13,8 +13,11 @@ void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* tot
 // ---- reduce.hip ----
 // Batched levels (graphs laid out one after another; gseg[v] = graph of v,
 // per-graph k*ncon tables at offset gkc[g]):
+// gmask (null: every graph): only the graphs g with gmask[g] set (of
+// ngraphs) get their tables cleared and recomputed; the others keep theirs.
 void part_weights_batch(const DevGraph& g, const int32_t* part, const int32_t* gseg, const int32_t* gkc,
-                        const int32_t* gk, int64_t kc_total, int32_t* pw, hipStream_t s);
+                        const int32_t* gk, int64_t kc_total, int32_t* pw, hipStream_t s,
+                        const uint8_t* gmask = nullptr, int32_t ngraphs = 0);
 // cut_out[g] = edge cut of graph g (i32 wrap, halved); acc is ngraphs u64 scratch.
 void edge_cut_batch(const DevGraph& g, const int32_t* part, const int32_t* gseg, int32_t ngraphs,
                     unsigned long long* acc, int32_t* cut_out, hipStream_t s);

@@ -177,14 +177,16 @@ __global__ __launch_bounds__(kBlock) void k_project(int32_t n, const int32_t* __
 
 // ---- batched levels: several graphs laid out one after another ----
 
-// pw of every graph at its table offset gkc[g]; a block whose vertices all
-// belong to one graph sums in LDS first.
+// pw of every graph at its table offset gkc[g], or (gmask non-null) of the
+// graphs with gmask[g] set only, whose tables start at zero; a block whose
+// vertices all belong to one graph sums in LDS first.
 __global__ __launch_bounds__(kBlock) void k_part_weights_batch(int32_t n, int32_t ncon,
                                                                const int32_t* __restrict__ vwgt,
                                                                const int32_t* __restrict__ part,
                                                                const int32_t* __restrict__ gseg,
                                                                const int32_t* __restrict__ gkc,
                                                                const int32_t* __restrict__ gk,
+                                                               const uint8_t* __restrict__ gmask,
                                                                int32_t* __restrict__ pw) {
     __shared__ int32_t s_h[kPwLds];
     const int64_t beg = (int64_t)blockIdx.x * kPwVerts;
@@ -192,6 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_part_weights_batch(int32_t n, int32_
     const int32_t g0 = gseg[beg], g1 = gseg[end - 1];
     const int kc = gk[g0] * ncon;
     if (g0 == g1 && kc <= kPwLds) {
+        if (gmask && !gmask[g0]) return;
         for (int i = threadIdx.x; i < kc; i += kBlock) s_h[i] = 0;
         __syncthreads();
         for (int64_t v = beg + threadIdx.x; v < end; v += kBlock) {
@@ -204,10 +207,24 @@ __global__ __launch_bounds__(kBlock) void k_part_weights_batch(int32_t n, int32_
             if (s_h[i] != 0) atomicAdd(&out[i], s_h[i]);
     } else {
         for (int64_t v = beg + threadIdx.x; v < end; v += kBlock) {
-            int32_t* out = pw + gkc[gseg[v]];
+            const int32_t g = gseg[v];
+            if (gmask && !gmask[g]) continue;
+            int32_t* out = pw + gkc[g];
             const int32_t p = part[v];
             for (int c = 0; c < ncon; ++c) atomicAdd(&out[p * ncon + c], vwgt ? vwgt[v * ncon + c] : 1);
         }
+    }
+}
+
+// Zeroes the part-weight tables of the graphs with gmask[g] set (block g).
+__global__ __launch_bounds__(kBlock) void k_zero_masked(int32_t ngraphs, int32_t ncon, const int32_t* __restrict__ gkc,
+                                                        const int32_t* __restrict__ gk,
+                                                        const uint8_t* __restrict__ gmask, int32_t* __restrict__ pw) {
+    for (int32_t g = blockIdx.x; g < ngraphs; g += gridDim.x) {
+        if (!gmask[g]) continue;
+        int32_t* out = pw + gkc[g];
+        const int kc = gk[g] * ncon;
+        for (int i = threadIdx.x; i < kc; i += kBlock) out[i] = 0;
     }
 }
 
@@ -474,11 +491,16 @@ void column_sums(const DevGraph& g, long long* out, hipStream_t s, long long* ou
 }
 
 void part_weights_batch(const DevGraph& g, const int32_t* part, const int32_t* gseg, const int32_t* gkc,
-                        const int32_t* gk, int64_t kc_total, int32_t* pw, hipStream_t s) {
-    RK_HIP(hipMemsetAsync(pw, 0, kc_total * sizeof(int32_t), s));
+                        const int32_t* gk, int64_t kc_total, int32_t* pw, hipStream_t s, const uint8_t* gmask,
+                        int32_t ngraphs) {
+    if (gmask)
+        hipLaunchKernelGGL(k_zero_masked, dim3(ngraphs < 1024 ? (ngraphs > 0 ? ngraphs : 1) : 1024), dim3(kBlock), 0,
+                           s, ngraphs, g.ncon, gkc, gk, gmask, pw);
+    else
+        RK_HIP(hipMemsetAsync(pw, 0, kc_total * sizeof(int32_t), s));
     if (g.n > 0)
         hipLaunchKernelGGL(k_part_weights_batch, dim3((g.n + kPwVerts - 1) / kPwVerts), dim3(kBlock), 0, s, g.n,
-                           g.ncon, g.vwgt, part, gseg, gkc, gk, pw);
+                           g.ncon, g.vwgt, part, gseg, gkc, gk, gmask, pw);
     RK_HIP(hipGetLastError());
 }
 
