@@ -302,12 +302,24 @@ __global__ __launch_bounds__(kFmThreads) void k_fm_pass(FmArgs a) {
             }
         }
         // the tree above the changed leaves, one level per barrier; past
-        // kFmChg changes every node is rebuilt
+        // kFmChg changes every node is rebuilt.  Changed leaves are mostly
+        // neighbours that share parents: each distinct node of a level is
+        // rebuilt once, by the wave of its first entry (the lanes compare
+        // the earlier entries' nodes, 64 at a time).
         const int nchg = s_nchg;
+        const bool all = nchg > kFmChg;
         for (int l = 1; l < a.nlvl; ++l) {
-            const bool all = nchg > kFmChg;
             const int64_t cnt = all ? a.len[l] : nchg;
             for (int64_t i = wave; i < cnt; i += kFmWaves) {
+                if (!all) {
+                    const int32_t nd = s_chg[i] >> (6 * l);
+                    bool dup = false;
+                    for (int64_t j0 = 0; j0 < i && !dup; j0 += 64) {
+                        const int64_t j = j0 + lane;
+                        dup = __ballot(j < i && (s_chg[j] >> (6 * l)) == nd) != 0;
+                    }
+                    if (dup) continue;
+                }
                 const int64_t node = all ? i : ((int64_t)s_chg[i] >> (6 * l));
                 const int64_t c = node * 64 + lane;
                 uint64_t x = c < a.len[l - 1] ? a.tree[a.off[l - 1] + c] : 0;
