@@ -7,11 +7,13 @@
 // (196-204) and the error-code mapping (205-207).  Every call runs on its own
 // pooled stream and arena, so concurrent calls are safe.
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstring>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rmetis.h"
@@ -83,25 +85,83 @@ int32_t guarded(F&& f) {
 }
 
 // graph/validate.rs:16-142.  Returns false for MetisError::Input.
+// Runs body(t, lo, hi) over [0, count) in chunks on up to 16 host threads
+// (one for small inputs); returns true when every chunk returned true.
+template <typename F>
+bool parallel_all(int64_t count, int64_t work, F body) {
+    int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (work < (1 << 20)) nt = 1;
+    if (nt == 1) return body(0, (int64_t)0, count);
+    std::vector<char> ok(nt, 1);
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int t = 0; t < nt; ++t) {
+        const int64_t lo = count * t / nt, hi = count * (t + 1) / nt;
+        th.emplace_back([&, t, lo, hi]() { ok[t] = body(t, lo, hi) ? 1 : 0; });
+    }
+    for (auto& x : th) x.join();
+    for (char c : ok)
+        if (!c) return false;
+    return true;
+}
+
+// graph/validate.rs:16-142: xadj starts at 0 and never decreases, every
+// neighbour is in range and not the vertex itself, every vertex appears in
+// as many rows as its own row is long (the count check of a symmetric
+// adjacency), vertex weights are positive.  Large graphs are checked on
+// several host threads.
 bool validate_csr(int32_t n, int32_t ncon, const int32_t* xadj, const int32_t* adjncy, const int32_t* vwgt) {
     if (n <= 0 || ncon <= 0) return false;
     if (xadj[0] != 0) return false;
-    for (int32_t i = 0; i < n; ++i)
-        if (xadj[i] > xadj[i + 1]) return false;
-    const int64_t ne = xadj[n];
-    for (int32_t v = 0; v < n; ++v)
-        for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j) {
-            const int32_t u = adjncy[j];
-            if (u < 0 || u >= n || u == v) return false;
-        }
-    std::vector<int32_t> cnt(n, 0);
-    for (int64_t j = 0; j < ne; ++j) cnt[adjncy[j]]++;
-    for (int32_t v = 0; v < n; ++v)
-        if (cnt[v] != xadj[v + 1] - xadj[v]) return false;
-    if (vwgt)
-        for (int64_t i = 0; i < (int64_t)n * ncon; ++i)
-            if (vwgt[i] <= 0) return false;
+    const int64_t work = (int64_t)n + std::max<int64_t>(0, xadj[n]);
+    if (!parallel_all(n, work, [&](int, int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; ++i)
+                if (xadj[i] > xadj[i + 1]) return false;
+            return true;
+        }))
+        return false;
+    // per-vertex in-degree counts, incremented by every thread
+    std::unique_ptr<std::atomic<int32_t>[]> cnt(new std::atomic<int32_t>[n]);
+    if (!parallel_all(n, work, [&](int, int64_t lo, int64_t hi) {
+            for (int64_t v = lo; v < hi; ++v) cnt[v].store(0, std::memory_order_relaxed);
+            return true;
+        }))
+        return false;
+    if (!parallel_all(n, work, [&](int, int64_t lo, int64_t hi) {
+            for (int64_t v = lo; v < hi; ++v)
+                for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j) {
+                    const int32_t u = adjncy[j];
+                    if (u < 0 || u >= n || u == (int32_t)v) return false;
+                    cnt[u].fetch_add(1, std::memory_order_relaxed);
+                }
+            return true;
+        }))
+        return false;
+    if (!parallel_all(n, work, [&](int, int64_t lo, int64_t hi) {
+            for (int64_t v = lo; v < hi; ++v)
+                if (cnt[v].load(std::memory_order_relaxed) != xadj[v + 1] - xadj[v]) return false;
+            return true;
+        }))
+        return false;
+    if (vwgt &&
+        !parallel_all((int64_t)n * ncon, work, [&](int, int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; ++i)
+                if (vwgt[i] <= 0) return false;
+            return true;
+        }))
+        return false;
     return true;
+}
+
+// xadj[0] == 0 and xadj never decreases (the first checks of
+// graph/validate.rs), on several host threads for large graphs.
+bool xadj_valid(int32_t n, const int32_t* xadj) {
+    if (n <= 0 || xadj[0] != 0) return false;
+    return parallel_all(n, n, [&](int, int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i)
+            if (xadj[i] > xadj[i + 1]) return false;
+        return true;
+    });
 }
 
 // Per-constraint vertex-weight totals below 2^30, so that running part
@@ -216,11 +276,21 @@ int32_t RMETIS_PartGraphKway(const idx_t* nvtxs, const idx_t* ncon, const idx_t*
         if (!nvtxs || !ncon || !xadj || !adjncy || !nparts || !part) return METIS_ERROR_INPUT;
         const int32_t n = *nvtxs, nc = *ncon, np = *nparts;
         if (n <= 0 || nc <= 0 || np < 2) return METIS_ERROR_INPUT;
-        if (!validate_csr(n, nc, xadj, adjncy, vwgt)) return METIS_ERROR_INPUT;
-        if (np > n) return METIS_ERROR_INPUT;                       // kway.rs:34-38
+        // validate.rs:16-142: xadj on the host (it sizes the upload), the
+        // rows and weights on the device once uploaded
+        if (!xadj_valid(n, xadj)) return METIS_ERROR_INPUT;
+        if (np > n) return METIS_ERROR_INPUT;                       // kway.rs:34-38 (either way an input error)
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+            // no device to run on: still report a malformed graph as such
+            if (!validate_csr(n, nc, xadj, adjncy, vwgt)) return METIS_ERROR_INPUT;
+        }
         const Options o = parse_options(options);
         ContextGuard cx;
         DevGraph g = upload_graph(*cx.c, n, nc, xadj, adjncy, vwgt, adjwgt, nullptr);
+        if (!validate_csr_device(g, cx->arena.get<int32_t>(n), cx->arena.get<int32_t>(1), cx->host_small(),
+                                 cx->stream))
+            return METIS_ERROR_INPUT;
         (void)vsize;  // the k-way greedy path never reads vsize
         int32_t* d_part = cx->arena.get<int32_t>(n);
         const int32_t obj = kway_device(*cx.c, g, np, tpwgts, ubvec, o, d_part, nullptr);

@@ -13,6 +13,10 @@ void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* tot
 // ---- reduce.hip ----
 // Batched levels (graphs laid out one after another; gseg[v] = graph of v,
 // per-graph k*ncon tables at offset gkc[g]):
+// The validation of graph/validate.rs:16-142 that remains once the host has
+// checked xadj[0] == 0 and xadj non-decreasing, on the uploaded graph (cnt:
+// n ints of scratch; bad_host: pinned); true when the graph is valid.
+bool validate_csr_device(const DevGraph& g, int32_t* cnt, int32_t* bad_dev, int32_t* bad_host, hipStream_t s);
 // gmask (null: every graph): only the graphs g with gmask[g] set (of
 // ngraphs) get their tables cleared and recomputed; the others keep theirs.
 void part_weights_batch(const DevGraph& g, const int32_t* part, const int32_t* gseg, const int32_t* gkc,

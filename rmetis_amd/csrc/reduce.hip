@@ -216,6 +216,42 @@ __global__ __launch_bounds__(kBlock) void k_part_weights_batch(int32_t n, int32_
     }
 }
 
+// graph/validate.rs:16-142 on the device for a graph already uploaded whose
+// xadj starts at 0 and never decreases (checked on the host): every
+// neighbour is in range and not the vertex itself (rows are read only
+// inside [0, m)), and every vertex appears in as many rows as its own row is
+// long; then vertex weights are positive.  *bad is set by any violation.
+__global__ __launch_bounds__(kBlock) void k_validate_rows(int32_t n, int64_t m, const int32_t* __restrict__ xadj,
+                                                          const int32_t* __restrict__ adjncy,
+                                                          int32_t* __restrict__ cnt, int32_t* __restrict__ bad) {
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
+        const int64_t b = xadj[v], e = xadj[v + 1];
+        if (b < 0 || b > e || e > m) {
+            *bad = 1;
+            continue;
+        }
+        for (int64_t j = b; j < e; ++j) {
+            const int32_t u = adjncy[j];
+            if (u < 0 || u >= n || u == (int32_t)v) {
+                *bad = 1;
+                break;
+            }
+            atomicAdd(&cnt[u], 1);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_validate_counts(int32_t n, int32_t ncon, const int32_t* __restrict__ xadj,
+                                                            const int32_t* __restrict__ vwgt,
+                                                            const int32_t* __restrict__ cnt, int32_t* __restrict__ bad) {
+    for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBlock) {
+        if (cnt[v] != xadj[v + 1] - xadj[v]) *bad = 1;
+        if (vwgt)
+            for (int c = 0; c < ncon; ++c)
+                if (vwgt[v * ncon + c] <= 0) *bad = 1;
+    }
+}
+
 // Zeroes the part-weight tables of the graphs with gmask[g] set (block g).
 __global__ __launch_bounds__(kBlock) void k_zero_masked(int32_t ngraphs, int32_t ncon, const int32_t* __restrict__ gkc,
                                                         const int32_t* __restrict__ gk,
@@ -591,6 +627,19 @@ int pick_group(int32_t n, int32_t m) {
     if (avg <= 24.0) return 16;
     if (avg <= 48.0) return 32;
     return 64;
+}
+
+bool validate_csr_device(const DevGraph& g, int32_t* cnt, int32_t* bad_dev, int32_t* bad_host, hipStream_t s) {
+    RK_HIP(hipMemsetAsync(cnt, 0, sizeof(int32_t) * (size_t)g.n, s));
+    RK_HIP(hipMemsetAsync(bad_dev, 0, sizeof(int32_t), s));
+    const int grid = (int)std::min<int64_t>(((int64_t)g.n + kBlock - 1) / kBlock, 8192);
+    hipLaunchKernelGGL(k_validate_rows, dim3(grid), dim3(kBlock), 0, s, g.n, (int64_t)g.m, g.xadj, g.adjncy, cnt,
+                       bad_dev);
+    hipLaunchKernelGGL(k_validate_counts, dim3(grid), dim3(kBlock), 0, s, g.n, g.ncon, g.xadj, g.vwgt, cnt, bad_dev);
+    RK_HIP(hipGetLastError());
+    RK_HIP(hipMemcpyAsync(bad_host, bad_dev, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    RK_HIP(hipStreamSynchronize(s));
+    return *bad_host == 0;
 }
 
 }  // namespace rk
