@@ -19,7 +19,7 @@
 //  * coarse adjacency of {a < b}: first occurrence over adj(a) then adj(b)
 //    in CSR order, internal edges dropped, weights of merged entries summed
 //    (323-341).  Coarse vertices with degree sum <= kContractBig are merged
-//    in LDS tiles (k_contract_tile); larger ones use a block per vertex and
+//    in per-wave LDS hash tables (k_contract_tile); larger ones use a block per vertex and
 //    a hash table in global memory that records each coarse neighbour's
 //    first position and weight sum (k_contract_big).
 #include "common.hpp"
@@ -470,9 +470,8 @@ __device__ inline int32_t seq_edge(const int32_t* xadj, int32_t a, int32_t b, in
 }
 
 
-constexpr int kCWaves = 4;               // waves (independent tiles) per contraction block
-constexpr int kCCap = 1024;              // merged-sequence positions a wave stages per batch
-constexpr int kCSlots = 16;              // unique coarse neighbours a lane merges in registers
+constexpr int kCWaves = 2;               // waves (independent tiles) per contraction block
+constexpr int kCCap = 256;               // merged-sequence positions a wave merges per batch
 
 __device__ inline int wave_excl_scan_c(int x, int* total) {
     const int lane = __lane_id();
@@ -490,45 +489,54 @@ __device__ inline int wave_excl_scan_c(int x, int* total) {
 // vertex) on its own, with its own LDS area and no block barriers; waves are
 // persistent over XCD-contiguous tile sequences (wave_tiles).  The merged
 // sequences adj(a) ++ adj(b) of the tile's ordinary coarse vertices are
-// staged in batches of whole vertices of at most kCCap positions: each lane
-// marks its own positions in a byte array of owners, then kCItems
-// independent coalesced loads per lane fetch adjncy/adjwgt and the cmap[]
-// gathers into LDS.  Each lane then merges its coarse vertex in LDS:
-// internal entries dropped, equal coarse neighbours summed onto their first
-// occurrence, and the survivors written in sequence order.
-template <int CAP, int W = 1>
-__global__ __launch_bounds__(64 * kCWaves) __attribute__((amdgpu_waves_per_eu(W))) void k_contract_tile(const int32_t* __restrict__ cn_dev,
-                                                               const int32_t* __restrict__ xadj,
-                                                               const int32_t* __restrict__ adjncy,
-                                                               const int32_t* __restrict__ adjwgt,
-                                                               const int32_t* __restrict__ cmap,
-                                                               const int32_t* __restrict__ members,
-                                                               const int32_t* __restrict__ degsum,
-                                                               const int32_t* __restrict__ tmp_off,
-                                                               int32_t* __restrict__ tmp_adj,
-                                                               int32_t* __restrict__ tmp_w, int32_t* __restrict__ cdeg,
-                                                               int32_t* __restrict__ big_list,
-                                                               int32_t* __restrict__ counters) {
-    // staged entries of the lane owning batch position q sit at q + lane:
-    // one slot of padding per lane spreads the lanes' ranges over the LDS
-    // banks (degree sums with a common factor would otherwise put every lane
-    // of the merge loops on a few banks)
-    __shared__ int32_t s_cu_all[kCWaves][CAP + 64];
-    __shared__ int32_t s_w_all[kCWaves][CAP + 64];
-    __shared__ uint8_t s_own_all[kCWaves][CAP];
-    __shared__ int32_t s_loff_all[kCWaves][65], s_xa_all[kCWaves][64], s_xb_all[kCWaves][64], s_da_all[kCWaves][64];
+// taken in batches of whole vertices of at most CAP positions: each lane
+// marks its own positions in a byte array of owners, then CAP / 64
+// independent coalesced loads per lane fetch adjncy/adjwgt, and the cmap[]
+// gathers follow.  The whole wave merges the batch at once in an LDS hash
+// table: each coarse vertex of the batch owns a sub-table of twice its
+// degree sum, where an entry claims its coarse neighbour's slot (CAS), adds
+// its weight and keeps the smallest position (the first occurrence);
+// internal entries are dropped.  The first occurrences, ranked in position
+// order by one ballot per round, go out in sequence order at the vertex's
+// tmp offset, so that the work of a batch is its entry count, whatever the
+// degrees of its vertices.
+template <int CAP, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_contract_tile(const int32_t* __restrict__ cn_dev,
+                                                              const int32_t* __restrict__ xadj,
+                                                              const int32_t* __restrict__ adjncy,
+                                                              const int32_t* __restrict__ adjwgt,
+                                                              const int32_t* __restrict__ cmap,
+                                                              const int32_t* __restrict__ members,
+                                                              const int32_t* __restrict__ degsum,
+                                                              const int32_t* __restrict__ tmp_off,
+                                                              int32_t* __restrict__ tmp_adj,
+                                                              int32_t* __restrict__ tmp_w, int32_t* __restrict__ cdeg,
+                                                              int32_t* __restrict__ big_list,
+                                                              int32_t* __restrict__ counters, int32_t big_min) {
+    constexpr int R = CAP / 64;               // batch positions per lane
+    constexpr int T = 2 * CAP;                // hash slots
+    __shared__ int32_t s_key_all[WAVES][T];   // coarse neighbour (-1: empty)
+    __shared__ int32_t s_sum_all[WAVES][T];   // weight sum
+    __shared__ int32_t s_pos_all[WAVES][T];   // first batch position
+    __shared__ uint8_t s_own_all[WAVES][CAP];
+    __shared__ int32_t s_loff_all[WAVES][65], s_xa_all[WAVES][64], s_xb_all[WAVES][64], s_da_all[WAVES][64],
+        s_toff_all[WAVES][64], s_qb_all[WAVES][64], s_qe_all[WAVES][64];
     const int wv = threadIdx.x >> 6, lane = __lane_id();
-    int32_t* s_cu = s_cu_all[wv];
-    int32_t* s_w = s_w_all[wv];
+    int32_t* s_key = s_key_all[wv];
+    int32_t* s_sum = s_sum_all[wv];
+    int32_t* s_pos = s_pos_all[wv];
     uint8_t* s_own = s_own_all[wv];
     int32_t *s_loff = s_loff_all[wv], *s_xa = s_xa_all[wv], *s_xb = s_xb_all[wv], *s_da = s_da_all[wv];
+    int32_t *s_toff = s_toff_all[wv], *s_qb = s_qb_all[wv], *s_qe = s_qe_all[wv];
+    const uint64_t below = (1ull << lane) - 1ull;
     const int32_t cn = *cn_dev;
     const int ntiles = (cn + 63) / 64;
-    const WaveTiles wt = wave_tiles(ntiles, kCWaves);
+    const WaveTiles wt = wave_tiles(ntiles, WAVES);
     bool not_one = false;
     for (int it = 0; it < wt.count; ++it) {
         const int tile = wt.base + it * wt.stride;
-        const int64_t cv = (int64_t)tile * 64 + lane;
+        const int32_t tbase = tile * 64;
+        const int64_t cv = (int64_t)tbase + lane;
         const bool own = cv < cn;
         int32_t ds = 0, xa = 0, xb = 0, da = 0, toff = 0;
         if (own) {
@@ -539,13 +547,13 @@ __global__ __launch_bounds__(64 * kCWaves) __attribute__((amdgpu_waves_per_eu(W)
             da = xadj[a + 1] - xa;
             xb = b >= 0 ? xadj[b] : 0;
         }
-        const bool big = own && ds > kContractBig;
+        const bool big = own && ds > big_min;
         const uint64_t bigm = __ballot(big);
         if (bigm) {                                   // wave-aggregated append to the big list
             int base = 0;
             if (lane == 0) base = atomicAdd(&counters[2], __popcll(bigm));
             base = __shfl(base, 0, 64);
-            if (big) big_list[base + __popcll(bigm & ((1ull << lane) - 1ull))] = (int32_t)cv;
+            if (big) big_list[base + __popcll(bigm & below)] = (int32_t)cv;
         }
         const int sds = (own && !big) ? ds : 0;
         int total;
@@ -554,119 +562,99 @@ __global__ __launch_bounds__(64 * kCWaves) __attribute__((amdgpu_waves_per_eu(W)
         s_xa[lane] = xa;
         s_xb[lane] = xb;
         s_da[lane] = da;
+        s_toff[lane] = toff;
         if (lane == 63) s_loff[64] = total;
         for (int vs = 0; vs < 64;) {
             const int P0 = __shfl(loff, vs, 64);
             const uint64_t fit = __ballot(lane >= vs && loff + sds - P0 <= CAP);
             const int ve = 64 - __builtin_clzll(fit);               // fit is lanes [vs, ve), ve > vs
-            const int P1 = __shfl(loff + sds, ve - 1, 64);
+            const int np = __shfl(loff + sds, ve - 1, 64) - P0;     // batch positions
             if (lane >= vs && lane < ve)
                 for (int e = 0; e < sds; ++e) s_own[loff - P0 + e] = (uint8_t)lane;
+            for (int i = lane; i < 2 * np; i += 64) {
+                s_key[i] = -1;
+                s_sum[i] = 0;
+                s_pos[i] = INT32_MAX;
+            }
             wave_sync();
-            int32_t uu[(CAP / 64)], ww[(CAP / 64)], oo[(CAP / 64)];
+            int32_t uu[R], ww[R], oo[R];
 #pragma unroll
-            for (int i = 0; i < (CAP / 64); ++i) {
-                const int p = P0 + lane + i * 64;
+            for (int i = 0; i < R; ++i) {
+                const int p = lane + i * 64;
                 uu[i] = 0;
                 ww[i] = 1;
                 oo[i] = 0;
-                if (p < P1) {
-                    const int o = s_own[p - P0];
+                if (p < np) {
+                    const int o = s_own[p];
                     oo[i] = o;
-                    const int32_t tl = p - s_loff[o];
+                    const int32_t tl = P0 + p - s_loff[o];
                     const int32_t j = tl < s_da[o] ? s_xa[o] + tl : s_xb[o] + (tl - s_da[o]);
                     uu[i] = adjncy[j];
                     if (adjwgt) ww[i] = adjwgt[j];
                 }
             }
-            // all gathers in flight before the first LDS write (items past
-            // the batch read cmap[0])
-            int32_t cc[(CAP / 64)];
+            // all gathers in flight before the first LDS access (positions
+            // past the batch read cmap[0])
+            int32_t cc[R];
 #pragma unroll
-            for (int i = 0; i < (CAP / 64); ++i) cc[i] = cmap[uu[i]];
+            for (int i = 0; i < R; ++i) cc[i] = cmap[uu[i]];
+            // insert: the slot of (owner, coarse neighbour) by linear probing
+            // in the owner's sub-table, then its weight and first position
+            int32_t sl[R];
 #pragma unroll
-            for (int i = 0; i < (CAP / 64); ++i) {
-                const int p = P0 + lane + i * 64;
-                if (p < P1) {
-                    s_cu[p - P0 + oo[i]] = cc[i];
-                    s_w[p - P0 + oo[i]] = ww[i];
+            for (int i = 0; i < R; ++i) {
+                const int p = lane + i * 64;
+                sl[i] = -1;
+                if (p < np && cc[i] != tbase + oo[i]) {                 // internal edge (graph/mod.rs:330-332)
+                    const int o = oo[i];
+                    const int r0 = 2 * (s_loff[o] - P0);
+                    const int r1 = 2 * (s_loff[o + 1] - P0);
+                    int k = r0 + (int)(((uint64_t)mix32((uint32_t)cc[i]) * (uint32_t)(r1 - r0)) >> 32);
+                    for (;;) {
+                        const int32_t prev = atomicCAS(&s_key[k], -1, cc[i]);
+                        if (prev == -1 || prev == cc[i]) break;
+                        k = (k + 1 == r1) ? r0 : k + 1;
+                    }
+                    atomicAdd(&s_sum[k], ww[i]);                          // merge (graph/mod.rs:335-339)
+                    atomicMin(&s_pos[k], p);
+                    sl[i] = k;
                 }
             }
             wave_sync();
-            // Each lane compacts its coarse vertex's unique neighbours, in
-            // first-occurrence order (internal edges dropped): an entry adds
-            // its weight to an earlier equal entry or is appended.  The first
-            // kCSlots unique entries live in registers (compared with unrolled
-            // selects, constant indices only); more are appended in place in
-            // LDS after slot kCSlots of the lane's range and searched there.
-            // Appends never overtake the read position.  At the end the
-            // register slots go to the front of the range.
-            int pos = 0;
-            const bool mine_v = lane >= vs && lane < ve && own && !big;
-            const int lb = loff - P0 + lane;
-            if (mine_v) {
-                const int le = lb + ds;
-                const int32_t me = (int32_t)cv;
-                int32_t sc[kCSlots];
-                uint32_t sw[kCSlots];
+            // rank the first occurrences in position order; the lanes holding
+            // a vertex's first and last positions record the ranks around it
+            int carry = 0;
+            int32_t qq[R], sm[R];
 #pragma unroll
-                for (int j = 0; j < kCSlots; ++j) sc[j] = -1, sw[j] = 0;
-                for (int e = lb; e < le; ++e) {
-                    const int32_t cu = s_cu[e];
-                    const uint32_t w = (uint32_t)s_w[e];
-                    if (cu == me) continue;                                 // internal edge (graph/mod.rs:330-332)
-                    bool hit = false;
-#pragma unroll
-                    for (int j = 0; j < kCSlots; ++j)
-                        if (sc[j] == cu) sw[j] += w, hit = true;            // merge (graph/mod.rs:335-339)
-                    if (hit) continue;
-                    if (pos < kCSlots) {
-#pragma unroll
-                        for (int j = 0; j < kCSlots; ++j)
-                            if (j == pos) sc[j] = cu, sw[j] = w;
-                    } else {
-                        int k = lb + kCSlots;
-                        while (k < lb + pos && s_cu[k] != cu) ++k;
-                        if (k < lb + pos) {
-                            s_w[k] = (int32_t)((uint32_t)s_w[k] + w);
-                            continue;
-                        }
-                        s_cu[lb + pos] = cu;
-                        s_w[lb + pos] = (int32_t)w;
-                    }
-                    ++pos;
+            for (int i = 0; i < R; ++i) {
+                const int p = lane + i * 64;
+                bool f = false;
+                sm[i] = 0;
+                if (sl[i] >= 0) {
+                    f = s_pos[sl[i]] == p;
+                    sm[i] = s_sum[sl[i]];
                 }
-#pragma unroll
-                for (int j = 0; j < kCSlots; ++j)
-                    if (j < pos) {
-                        s_cu[lb + j] = sc[j];
-                        s_w[lb + j] = (int32_t)sw[j];
-                        if (sw[j] != 1u) not_one = true;
-                    }
-                for (int k = lb + kCSlots; k < lb + pos; ++k)
-                    if (s_w[k] != 1) not_one = true;
-                cdeg[cv] = pos;
+                const uint64_t fm = __ballot(f);
+                const int q = carry + __popcll(fm & below);
+                carry += __popcll(fm);
+                qq[i] = f ? q : -1;
+                if (p < np) {
+                    const int o = oo[i];
+                    if (P0 + p == s_loff[o]) s_qb[o] = q;
+                    if (P0 + p + 1 == s_loff[o + 1]) s_qe[o] = q + (f ? 1 : 0);
+                }
             }
             wave_sync();
-            // Without big vertices among the batch's lanes, batch position i
-            // is tmp position tmp_off[vs] + i (tmp_off is the prefix of the
-            // degree sums): the batch goes out in coalesced stores, the slots
-            // after each vertex's unique entries included (never read).
-            // Otherwise each lane writes its own entries.
-            const uint64_t bmask = ((ve >= 64 ? ~0ull : ((1ull << ve) - 1ull)) >> vs) << vs;
-            if ((bigm & bmask) == 0) {
-                const int32_t base = __shfl(toff, vs, 64);
-                for (int i = lane; i < P1 - P0; i += 64) {
-                    const int q = i + s_own[i];
-                    tmp_adj[base + i] = s_cu[q];
-                    tmp_w[base + i] = s_w[q];
+            if (lane >= vs && lane < ve && own && !big) cdeg[cv] = sds > 0 ? s_qe[lane] - s_qb[lane] : 0;
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+                if (qq[i] >= 0) {
+                    const int o = oo[i];
+                    const int32_t at = s_toff[o] + (qq[i] - s_qb[o]);
+                    tmp_adj[at] = cc[i];
+                    tmp_w[at] = sm[i];
+                    if (sm[i] != 1) not_one = true;
                 }
-            } else if (mine_v) {
-                for (int k = 0; k < pos; ++k) {
-                    tmp_adj[toff + k] = s_cu[lb + k];
-                    tmp_w[toff + k] = s_w[lb + k];
-                }
-            }
             wave_sync();
             vs = ve;
         }
@@ -889,11 +877,11 @@ ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, 
                        b.members, b.degsum, cg.vwgt, cg.vsize, gseg, gon, gs ? gs->cgseg : nullptr);
     exclusive_scan_i32(b.degsum, b.tmp_off, n, nullptr, b.scan_tmp, s);
     {
-        const int blocks = (n + 64 * kCWaves - 1) / (64 * kCWaves);
-        // LDS (4 waves x ~10.7 KB) and 115 VGPRs allow 3 blocks per CU; a
-        // 512-entry stage (4 blocks) was 20 % slower on C4's levels, 256
-        // entries with 8 waves per SIMD (spilling) 50 % slower
-        const auto kern = k_contract_tile<kCCap>;
+        // 256-position batches in blocks of 2 waves (78 VGPRs, ~8 KB of LDS
+        // per wave): on the 1,024-graph batch's levels 1.26x faster than
+        // 512 x 4 and 1.02-1.09x faster than 256 x 4, 128 x 8 or 384 x 4
+        static_assert(kContractBig <= kCCap, "a tile batch holds every ordinary coarse vertex");
+        const auto kern = k_contract_tile<kCCap, kCWaves>;
         static std::atomic<int> occ{0};
         int nb = occ.load(std::memory_order_relaxed);
         if (nb <= 0) {
@@ -901,10 +889,11 @@ ContractResult contract(const DevGraph& g, const int32_t* match, int32_t* cmap, 
             nb = nb > 0 ? nb : 1;
             occ.store(nb, std::memory_order_relaxed);
         }
+        const int blocks = (n + 64 * kCWaves - 1) / (64 * kCWaves);
         const int resident = device_cus() * nb;
-        hipLaunchKernelGGL(kern, dim3(blocks < resident ? blocks : resident), dim3(64 * kCWaves), 0, s,
-                           cn_dev, g.xadj, g.adjncy, g.adjwgt, cmap, b.members, b.degsum, b.tmp_off, b.tmp_adj,
-                           b.tmp_w, b.cdeg, b.big_list, b.counters);
+        hipLaunchKernelGGL(kern, dim3(blocks < resident ? blocks : resident), dim3(64 * kCWaves), 0, s, cn_dev,
+                           g.xadj, g.adjncy, g.adjwgt, cmap, b.members, b.degsum, b.tmp_off, b.tmp_adj, b.tmp_w,
+                           b.cdeg, b.big_list, b.counters, kContractBig);
     }
     hipLaunchKernelGGL(k_contract_big, dim3(1024), dim3(kBlock), 0, s, g.xadj, g.adjncy, g.adjwgt, cmap,
                        b.members, b.degsum, b.tmp_off, b.tmp_adj, b.tmp_w, b.cdeg, b.big_list, b.counters,

@@ -349,7 +349,7 @@ struct CoarsenBufs {
     void* scan_tmp = nullptr;
 };
 
-constexpr int kContractBig = 64;   // coarse vertices of larger degree sum use the global-hash path
+constexpr int kContractBig = 256;  // coarse vertices of larger degree sum use the global-hash path
 
 // Deterministic seeded matching (ctype 1 heavy edge, 0 random): the greedy
 // matching in edge-key order, computed by the suitor algorithm.

@@ -381,6 +381,30 @@ def test_build_coarse_graph_scan_sizes(rows, cols):
     _coarse_equal(x, a, vw, aw, None, 1, match)
 
 
+@pytest.mark.parametrize("seed", [1, 2])
+def test_build_coarse_graph_degree_mix(seed):
+    # degree sums of coarse vertices spread over 1..~500 within the same
+    # 64-vertex tiles: LDS batches of one to dozens of vertices, a vertex
+    # filling a batch alone, and the global-hash path above 256; neighbours
+    # drawn mostly from a window around the vertex, so that many entries merge
+    from graphs import build_undirected
+    rng = np.random.default_rng(seed)
+    n = 3000
+    deg = np.where(rng.random(n) < 0.05, rng.integers(100, 420, n), rng.integers(1, 12, n))
+    edges = set()
+    for v in range(n):
+        for _ in range(int(deg[v])):
+            u = int(v + rng.integers(-150, 151)) if rng.random() < 0.8 else int(rng.integers(0, n))
+            if 0 <= u < n and u != v:
+                edges.add((min(u, v), max(u, v)))
+    x, a = build_undirected(n, sorted(edges))
+    aw = _symmetric_weights(x, a, rng.integers(1, 9, len(a)).astype(np.int32))
+    assert np.diff(x).max() > 256
+    for m in (O.shem_matching(x, a, aw, O.rng(seed)), O.random_matching(x, a, O.rng(seed + 7))):
+        _coarse_equal(x, a, None, aw, None, 1, m)
+        _coarse_equal(x, a, None, None, None, 1, m)
+
+
 def _symmetric_weights(x, a, aw):
     """Edge weights with w(u,v) == w(v,u): the weight of the smaller-id end's entry."""
     n = len(x) - 1
