@@ -264,29 +264,64 @@ __global__ __launch_bounds__(kBlock) void k_zero_masked(int32_t ngraphs, int32_t
     }
 }
 
-// Edge cut of every graph: each wave sums its vertices' cut weight, one
-// atomic per wave when its vertices belong to one graph.
+// Edge cut of every graph.  A wave takes 64 consecutive vertices, whose rows
+// form one contiguous span of entries: lane q of a round reads span position
+// q (coalesced adjncy/adjwgt loads, then the part[] gather), its vertex found
+// by a binary search over the 64 row starts in LDS.  Sums wrap in 32 bits
+// (only the low 32 bits of a graph's total are used).  When the 64 vertices
+// belong to one graph the wave adds its sum with one atomic; otherwise each
+// vertex's sum is gathered in LDS and added to its own graph.
 __global__ __launch_bounds__(kBlock) void k_edge_cut_batch(int32_t n, const int32_t* __restrict__ xadj,
                                                            const int32_t* __restrict__ adjncy,
                                                            const int32_t* __restrict__ adjwgt,
                                                            const int32_t* __restrict__ part,
                                                            const int32_t* __restrict__ gseg,
                                                            unsigned long long* __restrict__ acc) {
-    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    long long sum = 0;
-    int32_t gi = -1;
-    if (v < n) {
-        gi = gseg[v];
-        const int32_t pv = part[v];
-        for (int32_t j = xadj[v]; j < xadj[v + 1]; ++j)
-            if (part[adjncy[j]] != pv) sum += adjwgt ? adjwgt[j] : 1;
-    }
-    const int32_t gfirst = __shfl(gi, 0, 64);
-    if (__ballot(v < n && gi != gfirst) == 0) {
-        const long long tot = wave_incl_scan(sum);
-        if (__lane_id() == 63 && gfirst >= 0) atomicAdd(&acc[gfirst], (unsigned long long)__shfl(tot, 63, 64));
-    } else if (v < n && sum) {
-        atomicAdd(&acc[gi], (unsigned long long)sum);
+    constexpr int WPB = kBlock / 64;
+    __shared__ int32_t s_beg_all[WPB][64], s_part_all[WPB][64];
+    __shared__ uint32_t s_cut_all[WPB][64];
+    const int wv = threadIdx.x >> 6, lane = __lane_id();
+    int32_t* s_beg = s_beg_all[wv];
+    int32_t* s_part = s_part_all[wv];
+    uint32_t* s_cut = s_cut_all[wv];
+    const int64_t ntiles = ((int64_t)n + 63) / 64;
+    for (int64_t t = (int64_t)blockIdx.x * WPB + wv; t < ntiles; t += (int64_t)gridDim.x * WPB) {
+        const int64_t v0 = t * 64;
+        const int last = (int)min<int64_t>(64, (int64_t)n - v0);     // vertices of this tile
+        const int64_t v = v0 + (lane < last ? lane : last - 1);
+        const int32_t e0 = xadj[v0];
+        const int32_t span = xadj[v0 + last] - e0;
+        const int32_t gi = gseg[v];
+        const bool mixed = __ballot(gi != __shfl(gi, 0, 64)) != 0;
+        s_beg[lane] = xadj[v] - e0;
+        s_part[lane] = part[v];
+        s_cut[lane] = 0u;
+        wave_sync();
+        uint32_t sum = 0u;
+        for (int32_t q = lane; q < span; q += 64) {
+            int lo = 0, hi = last - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_beg[mid] <= q) lo = mid;
+                else hi = mid - 1;
+            }
+            const int32_t j = e0 + q;
+            const uint32_t w = part[adjncy[j]] != s_part[lo] ? (adjwgt ? (uint32_t)adjwgt[j] : 1u) : 0u;
+            if (mixed) {
+                if (w) atomicAdd(&s_cut[lo], w);
+            } else {
+                sum += w;
+            }
+        }
+        if (!mixed) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) sum += (uint32_t)__shfl_xor((int)sum, o, 64);
+            if (lane == 0 && sum) atomicAdd(&acc[gi], (unsigned long long)sum);
+        } else {
+            wave_sync();
+            if (lane < last && s_cut[lane]) atomicAdd(&acc[gi], (unsigned long long)s_cut[lane]);
+        }
+        wave_sync();
     }
 }
 
@@ -544,8 +579,9 @@ void edge_cut_batch(const DevGraph& g, const int32_t* part, const int32_t* gseg,
                     unsigned long long* acc, int32_t* cut_out, hipStream_t s) {
     RK_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * ngraphs, s));
     if (g.n > 0)
-        hipLaunchKernelGGL(k_edge_cut_batch, dim3((g.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g.n, g.xadj,
-                           g.adjncy, g.adjwgt, part, gseg, acc);
+        hipLaunchKernelGGL(k_edge_cut_batch,
+                           dim3((int)std::min<int64_t>(((int64_t)g.n + kBlock - 1) / kBlock, 65536)), dim3(kBlock),
+                           0, s, g.n, g.xadj, g.adjncy, g.adjwgt, part, gseg, acc);
     hipLaunchKernelGGL(k_finish_cut_batch, dim3((ngraphs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, ngraphs, acc,
                        cut_out);
     RK_HIP(hipGetLastError());
