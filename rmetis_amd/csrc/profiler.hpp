@@ -12,13 +12,15 @@
 namespace rk {
 
 enum ProfTag {
-    kProfGain = 0,      // gain launches over every tile (boundary, gain tables, candidates)
+    kProfGain = 0,      // cut-pass gain launches over every tile (boundary, gain tables, candidates)
     kProfCompact = 1,   // tile-count scan + ordered compaction
     kProfAdmit = 2,     // ordered admission (one workgroup)
     kProfMatch = 3,     // matching (suitor init, chains, pairing)
     kProfContract = 4,  // coarse numbering + contraction
     kProfOther = 5,     // part weights, projection, edge cut
-    kProfGainInc = 6,   // incremental gain launches (only tiles near the last pass's moves)
+    kProfGainInc = 6,   // gain launches over a data-dependent part of the level: incremental
+                        // cut passes (tiles near the last pass's moves) and balance-repair
+                        // passes (tiles with a vertex in an overweight part)
     kProfTags = 7
 };
 

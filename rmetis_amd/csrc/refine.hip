@@ -224,10 +224,16 @@ __global__ __launch_bounds__(64 * kGWaves) __attribute__((amdgpu_waves_per_eu(4,
         if (lane == 63) s_loff[kTile] = total;
         const bool anyheavy = __ballot(heavy) != 0;
         const int32_t x0 = __builtin_amdgcn_readfirstlane(beg);   // lane 0 owns v0 < n
+        // A balance-repair pass moves only vertices of overweight parts
+        // (greedy.rs:66-72): when none of the tile's ordinary vertices is in
+        // one, none of them gets a record and their edges are not read (a
+        // heavy vertex's record comes from k_gain_heavy either way).
+        const bool skip =
+            a.mode == 0 && __ballot(own && !heavy && part_overweight(pwg, omg, a.ncon, pv)) == 0;
         wave_sync();
         int4 rec = make_int4(-1, 0, 0, 0);
         // batches of whole vertices [vs, ve) with at most kGCap staged edges
-        for (int vs = 0; vs < kTile;) {
+        for (int vs = skip ? kTile : 0; vs < kTile;) {
             const int P0 = __shfl(loff, vs, 64);
             const uint64_t fit = __ballot(lane >= vs && loff + sdeg - P0 <= kGCap);
             const int ve = 64 - __builtin_clzll(fit);            // fit is lanes [vs, ve), ve > vs
@@ -622,9 +628,11 @@ void refine_pass(const DevGraph& g, int32_t k, int mode, int nheavy, const int32
     {
         // algorithmic bytes of a full launch: xadj, adjncy, adjwgt, part[adjncy] gather, part[v]
         const double bytes = 4.0 * (g.n + 1) + 4.0 * g.n + (g.adjwgt ? 12.0 : 8.0) * g.m;
-        // an incremental launch reads a data-dependent part of the level: its
-        // bytes are not modelled
-        ProfScope ps(ga.incremental ? kProfGainInc : kProfGain, ga.incremental ? 0.0 : bytes, g.n, s);
+        // an incremental launch, and a balance-repair launch (which reads
+        // the rows of tiles with a vertex in an overweight part only), read a
+        // data-dependent part of the level: their bytes are not modelled
+        const bool partial = ga.incremental || mode == 0;
+        ProfScope ps(partial ? kProfGainInc : kProfGain, partial ? 0.0 : bytes, g.n, s);
         launch_gain(ga, nheavy, heavy_list, b, s);
     }
     {
@@ -648,9 +656,11 @@ void refine_pass_batch(const DevGraph& g, const BatchLevel& bl, int mode, int nh
     ga.incremental = incremental && b.tile_dirty && bl.gfull && mode == 1 ? 1 : 0;
     {
         const double bytes = 4.0 * (g.n + 1) + 4.0 * g.n + (g.adjwgt ? 12.0 : 8.0) * g.m;
-        // an incremental launch reads a data-dependent part of the level: its
-        // bytes are not modelled
-        ProfScope ps(ga.incremental ? kProfGainInc : kProfGain, ga.incremental ? 0.0 : bytes, g.n, s);
+        // an incremental launch, and a balance-repair launch (which reads
+        // the rows of tiles with a vertex in an overweight part only), read a
+        // data-dependent part of the level: their bytes are not modelled
+        const bool partial = ga.incremental || mode == 0;
+        ProfScope ps(partial ? kProfGainInc : kProfGain, partial ? 0.0 : bytes, g.n, s);
         launch_gain(ga, nheavy, heavy_list, b, s, bl.ntiles);
     }
     {
